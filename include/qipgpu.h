@@ -1,0 +1,159 @@
+/*
+ * qipgpu.h -- C ABI of the MI355X (gfx950) state-vector engine for QIP.
+ *
+ * The engine is the native layer under a QIP StateType backend (qip/backend.py:14-65). Every
+ * entry point below replaces one native function of the reference's CythonBackend stack
+ * (qip/backend.py:68-175 -> qip/ext/kronprod.pyx, qip/ext/func_apply.pyx) and cites it.
+ *
+ * Conventions (the reference's):
+ *   - qubit q is state-index bit (n-1-q) (qip/ext/kronprod.pyx:168,187-189);
+ *   - inside a group of qubit indices the first index is the most significant bit of the
+ *     sub-matrix / register value (kronprod.pyx:186-189, func_apply.pyx:82-85);
+ *   - amplitudes are interleaved (re, im): complex128 (QIPGPU_C128) or complex64 (QIPGPU_C64),
+ *     the memory layout of numpy.complex128 / numpy.complex64;
+ *   - matrices are row-major, 2^k x 2^k, interleaved complex128.
+ * All entry points return QIPGPU_OK (0) or a negative status; qipgpu_last_error() then holds a
+ * message (per calling thread). Calls on one state handle must come from one thread at a time;
+ * each handle owns one HIP stream. Calls that return host data synchronise that stream.
+ */
+#ifndef QIPGPU_H
+#define QIPGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qipgpu_state qipgpu_state;
+typedef struct qipgpu_table qipgpu_table;
+
+enum { QIPGPU_C128 = 0, QIPGPU_C64 = 1 };
+
+enum {
+    QIPGPU_OK = 0,
+    QIPGPU_EINVAL = -1,   /* bad argument: Python raises ValueError */
+    QIPGPU_ENOMEM = -2,   /* device allocation failed: MemoryError */
+    QIPGPU_EHIP = -3,     /* HIP runtime error: RuntimeError */
+    QIPGPU_EZERODIV = -4, /* measured probability is 0: ZeroDivisionError, as 1.0/p raises in
+                             kronprod.pyx:428 */
+    QIPGPU_EUNSUP = -5    /* operator outside what the engine applies: NotImplementedError */
+};
+
+/* Largest number of (non-control) target qubits of one dense operator. */
+#define QIPGPU_MAX_DENSE 6
+/* Largest number of qubits of a state. */
+#define QIPGPU_MAX_QUBITS 34
+
+const char *qipgpu_last_error(void);
+int qipgpu_device_count(int32_t *count);
+const char *qipgpu_version(void);
+
+/* ---- state lifecycle: CythonBackend.make_state (qip/backend.py:73-104) ---------------------- */
+
+/* Allocate a 2^n-amplitude state on the current device. Contents undefined until an init. */
+int qipgpu_create(int32_t n, int32_t dtype, qipgpu_state **out);
+int qipgpu_destroy(qipgpu_state *s);
+int qipgpu_info(const qipgpu_state *s, int32_t *n, int32_t *dtype);
+
+/* Product state: amplitude i = prod over groups g (in order) of feed_g[value of i on group g],
+ * zero where any unfed qubit is 1 (qip/backend.py:88-101, qip/util.py:108-124). With
+ * n_groups == 0 the state is |0...0> (backend.py:90-91). `feeds` concatenates the 2^len_g
+ * complex128 entries of each feed. */
+int qipgpu_init_product(qipgpu_state *s, int32_t n_groups, const int32_t *group_len,
+                        const int32_t *group_idx, const double *feeds);
+
+/* Copies of amplitude windows [start, end) in the state's own dtype:
+ *   get:  get_state / get_relative_range   (qip/backend.py:106-107,168-169)
+ *   set:  overwrite_relative_range          (qip/backend.py:171-172)
+ *   add:  addto_relative_range              (qip/backend.py:174-175) */
+int qipgpu_get_range(qipgpu_state *s, int64_t start, int64_t end, void *host);
+int qipgpu_set_range(qipgpu_state *s, int64_t start, int64_t end, const void *host);
+int qipgpu_add_range(qipgpu_state *s, int64_t start, int64_t end, const void *host);
+
+/* ---- gate application: cdot_loop (qip/ext/kronprod.pyx:41-200) -------------------------------
+ * Applies, in place, the operator that is `mat` on the `targets` qubits, conditioned on every
+ * `controls` qubit being 1 (the C_MAT structure diag(I, M), kronprod.pyx:215-225). The engine
+ * finds further control qubits and diagonal structure inside `mat` itself, so a caller may pass
+ * a whole controlled operator as one dense matrix. k <= 12 before that reduction and at most
+ * QIPGPU_MAX_DENSE targets after it. */
+int qipgpu_apply_matrix(qipgpu_state *s, int32_t k, const int32_t *targets, const double *mat,
+                        int32_t n_ctrl, const int32_t *controls);
+
+/* Diagonal operator diag(d) on `targets`, conditioned on `controls` (k <= QIPGPU_MAX_DENSE). */
+int qipgpu_apply_diagonal(qipgpu_state *s, int32_t k, const int32_t *targets, const double *diag,
+                          int32_t n_ctrl, const int32_t *controls);
+
+/* ---- classical functions: func_apply (qip/ext/func_apply.pyx:11-112) --------------------------
+ * |x>|q>|r> -> |x>|f(x) xor q>|r> with x read MSB-first from reg1 and only the low n_reg2 bits of
+ * f(x) used (func_apply.pyx:82-97). `table[x]` = f(x) for x in [0, 2^n_reg1). In place. */
+int qipgpu_func_apply(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, int32_t n_reg2,
+                      const int32_t *reg2, const int32_t *table);
+/* The same with the f table kept resident on the device across calls. */
+int qipgpu_table_create(int32_t n_reg1, const int32_t *table, qipgpu_table **out);
+int qipgpu_table_destroy(qipgpu_table *t);
+int qipgpu_func_apply_table(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, int32_t n_reg2,
+                            const int32_t *reg2, const qipgpu_table *t);
+
+/* ---- reductions and measurement (qip/ext/kronprod.pyx:238-490) ------------------------------- */
+
+/* prob_magnitude / total_prob: sum |psi_i|^2 in FP64 (kronprod.pyx:318-326, backend.py:123-124) */
+int qipgpu_total_prob(qipgpu_state *s, double *out);
+
+/* measure_probabilities (kronprod.pyx:238-261): out[2^k], qubit indices[j] on bit j (LSB-first) */
+int qipgpu_probabilities(qipgpu_state *s, int32_t k, const int32_t *indices, double *out);
+
+/* Marginals in measurement order: out[m] = sum of |psi_i|^2 over i whose measured bits read m,
+ * the smallest measured qubit being the most significant bit of m (entwine_bit, util.pyx:2-23). */
+int qipgpu_marginals(qipgpu_state *s, int32_t k, const int32_t *indices, double *out);
+
+/* measure_top_probabilities (kronprod.pyx:264-315): the top_k largest marginals (measurement
+ * order), descending, ties by ascending m. Writes *count = min(top_k, 2^k) entries. */
+int qipgpu_top_probabilities(qipgpu_state *s, int32_t k, const int32_t *indices, int32_t top_k,
+                             int64_t *ms, double *ps, int32_t *count);
+
+/* soft_measure (kronprod.pyx:329-383) with the uniform draw u supplied by the caller:
+ * r = u * total_prob, then the sequential walk r -= p_m; stop at the first r <= 0. With
+ * measured >= 0 only that outcome is evaluated. Returns the raw marginal in *p. */
+int qipgpu_soft_measure(qipgpu_state *s, int32_t k, const int32_t *indices, double u,
+                        int64_t measured, int64_t *m, double *p);
+
+/* measure (kronprod.pyx:386-446): soft_measure unless both measured >= 0 and measured_prob > 0
+ * are given, then collapse in place: psi_i *= sqrt(1/p) where the measured bits read m, else 0. */
+int qipgpu_measure(qipgpu_state *s, int32_t k, const int32_t *indices, double u, int64_t measured,
+                   double measured_prob, int64_t *m, double *p);
+
+/* reduce_measure (kronprod.pyx:449-490): like measure, but amplitude i of the result is the
+ * collapsed amplitude with measured bits m and unmeasured bits the low n-k bits of i, so the
+ * reduced state appears tiled 2^k times (the reference keeps its 2^n arena, backend.py:139-142). */
+int qipgpu_reduce_measure(qipgpu_state *s, int32_t k, const int32_t *indices, double u,
+                          int64_t measured, double measured_prob, int64_t *m, double *p);
+
+/* ---- precision form: fused complex64 blocks on MFMA (no reference counterpart: it applies the
+ * product of several consecutive cdot_loop operators at once) ---------------------------------- */
+
+/* Apply a dense 2^k x 2^k operator (k <= 6; row-major complex128, targets[0] = MSB) to a
+ * QIPGPU_C64 state with split-BF16 MFMA products and FP32 accumulation. The state is first
+ * multiplied by the pending FP64 renormalisation factor left by the previous fused call; the
+ * norm of the result is accumulated in FP64 and becomes the next pending factor. */
+int qipgpu_fused_apply(qipgpu_state *s, int32_t k, const int32_t *targets, const double *mat);
+/* Apply the pending renormalisation factor (if any) with one scaling sweep. */
+int qipgpu_fused_finish(qipgpu_state *s);
+/* Norm of the state after the last fused call, before its renormalisation, in FP64. */
+int qipgpu_fused_last_norm(qipgpu_state *s, double *out);
+
+/* ---- comparison helpers ------------------------------------------------------------------------ */
+
+/* <a|b> in FP64 (either dtype each, same n) and both squared norms. */
+int qipgpu_overlap(qipgpu_state *a, qipgpu_state *b, double *re, double *im, double *norm_a,
+                   double *norm_b);
+/* max_i |a_i - b_i| in FP64. */
+int qipgpu_max_abs_diff(qipgpu_state *a, qipgpu_state *b, double *out);
+/* Block until all work queued on the state's stream has finished. */
+int qipgpu_synchronize(qipgpu_state *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QIPGPU_H */

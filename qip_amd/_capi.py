@@ -1,0 +1,113 @@
+"""ctypes binding of libqipgpu.so (declared in include/qipgpu.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` into ``qip_amd/lib/``. There is no
+CPU fallback: if the library or a GPU is missing, every backend call raises.
+"""
+import ctypes
+import os
+
+import numpy
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libqipgpu.so")
+
+C128, C64 = 0, 1
+OK, EINVAL, ENOMEM, EHIP, EZERODIV, EUNSUP = 0, -1, -2, -3, -4, -5
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_PP = ctypes.POINTER(ctypes.c_void_p)
+_PI32 = ctypes.POINTER(ctypes.c_int32)
+_PI64 = ctypes.POINTER(ctypes.c_int64)
+_PD = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes); mirrors include/qipgpu.h
+SIGNATURES = {
+    "qipgpu_last_error": (ctypes.c_char_p, []),
+    "qipgpu_device_count": (ctypes.c_int, [_PI32]),
+    "qipgpu_version": (ctypes.c_char_p, []),
+    "qipgpu_create": (ctypes.c_int, [_I32, _I32, _PP]),
+    "qipgpu_destroy": (ctypes.c_int, [_P]),
+    "qipgpu_info": (ctypes.c_int, [_P, _PI32, _PI32]),
+    "qipgpu_init_product": (ctypes.c_int, [_P, _I32, _P, _P, _P]),
+    "qipgpu_get_range": (ctypes.c_int, [_P, _I64, _I64, _P]),
+    "qipgpu_set_range": (ctypes.c_int, [_P, _I64, _I64, _P]),
+    "qipgpu_add_range": (ctypes.c_int, [_P, _I64, _I64, _P]),
+    "qipgpu_apply_matrix": (ctypes.c_int, [_P, _I32, _P, _P, _I32, _P]),
+    "qipgpu_apply_diagonal": (ctypes.c_int, [_P, _I32, _P, _P, _I32, _P]),
+    "qipgpu_func_apply": (ctypes.c_int, [_P, _I32, _P, _I32, _P, _P]),
+    "qipgpu_table_create": (ctypes.c_int, [_I32, _P, _PP]),
+    "qipgpu_table_destroy": (ctypes.c_int, [_P]),
+    "qipgpu_func_apply_table": (ctypes.c_int, [_P, _I32, _P, _I32, _P, _P]),
+    "qipgpu_total_prob": (ctypes.c_int, [_P, _PD]),
+    "qipgpu_probabilities": (ctypes.c_int, [_P, _I32, _P, _P]),
+    "qipgpu_marginals": (ctypes.c_int, [_P, _I32, _P, _P]),
+    "qipgpu_top_probabilities": (ctypes.c_int, [_P, _I32, _P, _I32, _P, _P, _PI32]),
+    "qipgpu_soft_measure": (ctypes.c_int, [_P, _I32, _P, _D, _I64, _PI64, _PD]),
+    "qipgpu_measure": (ctypes.c_int, [_P, _I32, _P, _D, _I64, _D, _PI64, _PD]),
+    "qipgpu_reduce_measure": (ctypes.c_int, [_P, _I32, _P, _D, _I64, _D, _PI64, _PD]),
+    "qipgpu_fused_apply": (ctypes.c_int, [_P, _I32, _P, _P]),
+    "qipgpu_fused_finish": (ctypes.c_int, [_P]),
+    "qipgpu_fused_last_norm": (ctypes.c_int, [_P, _PD]),
+    "qipgpu_overlap": (ctypes.c_int, [_P, _P, _PD, _PD, _PD, _PD]),
+    "qipgpu_max_abs_diff": (ctypes.c_int, [_P, _P, _PD]),
+    "qipgpu_synchronize": (ctypes.c_int, [_P]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libqipgpu.so and bind every entry point; raises OSError if it is not built."""
+    global _lib
+    if _lib is not None and path == LIB_PATH:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError("libqipgpu.so is not built at {} (run __graft_entry__.build())".format(path))
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path == LIB_PATH:
+        _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    msg = load().qipgpu_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int, what: str = "") -> None:
+    """Turn a status code into the exception the reference would raise for the same fault."""
+    if rc == OK:
+        return
+    msg = last_error()
+    if what:
+        msg = "{}: {}".format(what, msg) if msg else what
+    if rc == EINVAL:
+        raise ValueError(msg)
+    if rc == ENOMEM:
+        raise MemoryError(msg)
+    if rc == EZERODIV:
+        raise ZeroDivisionError("float division by zero")
+    if rc == EUNSUP:
+        raise NotImplementedError(msg)
+    raise RuntimeError(msg)
+
+
+def device_count() -> int:
+    c = ctypes.c_int32(0)
+    rc = load().qipgpu_device_count(ctypes.byref(c))
+    return int(c.value) if rc == OK else 0
+
+
+def i32(seq) -> numpy.ndarray:
+    return numpy.ascontiguousarray(numpy.asarray(seq, dtype=numpy.int64).astype(numpy.int32))
+
+
+def ptr(a: numpy.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)

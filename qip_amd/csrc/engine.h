@@ -1,0 +1,103 @@
+// engine.h -- internal declarations shared by the translation units of libqipgpu.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "qipgpu.h"
+
+namespace qipgpu {
+
+// Interleaved complex amplitude, aligned so one amplitude is one vector load
+// (16 B dwordx4 for complex128, 8 B dwordx2 for complex64).
+template <typename R>
+struct __attribute__((aligned(2 * sizeof(R)))) cx {
+    R re, im;
+};
+
+// Ascending list of state-bit positions at which zero bits are inserted into a compressed
+// index (the positions of target and control qubits). Lives in kernel arguments.
+struct PosList {
+    int32_t count;
+    uint8_t p[44];
+};
+
+// Insert a zero bit at each listed position (ascending), turning an enumeration index over
+// the free bits into a state index with those bits clear.
+__device__ __forceinline__ uint64_t insert_zeros(uint64_t g, const PosList &pl) {
+    for (int i = 0; i < pl.count; ++i) {
+        const uint32_t p = pl.p[i];
+        const uint64_t lo = g & ((1ull << p) - 1ull);
+        g = ((g ^ lo) << 1) | lo;
+    }
+    return g;
+}
+
+constexpr int kRingSlots = 8;
+constexpr size_t kRingSlotBytes = 64 * 64 * 16;  // one 64x64 complex128 matrix
+
+}  // namespace qipgpu
+
+struct qipgpu_state {
+    int32_t n = 0;
+    int32_t dtype = 0;
+    int32_t device = 0;
+    uint64_t len = 0;          // 2^n amplitudes
+    size_t amp_bytes = 0;      // bytes per amplitude
+    void *amps = nullptr;
+    hipStream_t stream = nullptr;
+    // growable device scratch (reduction partials, staging)
+    void *scratch = nullptr;
+    size_t scratch_cap = 0;
+    // device staging ring for operators too large for kernel arguments
+    void *d_ring = nullptr;
+    void *h_ring = nullptr;  // pinned
+    hipEvent_t ring_ev[qipgpu::kRingSlots] = {};
+    bool ring_used[qipgpu::kRingSlots] = {};
+    int ring_next = 0;
+    // fused path: d_fused[0] = pending renormalisation factor, d_fused[1] = last norm,
+    // d_fused[2..] = per-block norm partials
+    double *d_fused = nullptr;
+    size_t fused_partials = 0;
+    bool fused_pending = false;
+    // fused path operator staging (split BF16 planes), device
+    void *d_fmat = nullptr;
+    size_t d_fmat_bytes = 0;
+};
+
+namespace qipgpu {
+
+int set_error(int code, const char *fmt, ...);
+
+#define QIP_HIP(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::qipgpu::set_error(QIPGPU_EHIP, "%s failed: %s", #expr,                \
+                                       hipGetErrorString(e_));                             \
+    } while (0)
+
+#define QIP_LAUNCHED()                                                                     \
+    do {                                                                                   \
+        hipError_t e_ = hipGetLastError();                                                 \
+        if (e_ != hipSuccess)                                                              \
+            return ::qipgpu::set_error(QIPGPU_EHIP, "kernel launch failed: %s",            \
+                                       hipGetErrorString(e_));                             \
+    } while (0)
+
+// Make sure s->scratch holds at least `bytes`.
+int ensure_scratch(qipgpu_state *s, size_t bytes);
+// Validate a list of qubit indices against n (range, distinctness); returns QIPGPU_OK or EINVAL.
+int check_qubits(const qipgpu_state *s, int32_t k, const int32_t *q, const char *what);
+// Reserve a staging ring slot, copy `bytes` from `src` to the device, return device pointer.
+int ring_upload(qipgpu_state *s, const void *src, size_t bytes, void **dev);
+// Apply a fused path's pending renormalisation factor before any other use of the state.
+int settle(qipgpu_state *s);
+// Sum of |psi|^2 in FP64, result left on the device at `dst` (enqueued on s->stream).
+int norm_to_device(qipgpu_state *s, double *dst);
+
+}  // namespace qipgpu

@@ -1,0 +1,1386 @@
+// engine.hip -- plain state-vector engine: state lifecycle, gate application, func_apply,
+// reductions and measurement, for complex128 and complex64 states on gfx950.
+//
+// Every kernel here is a streaming sweep bound by HBM bandwidth: each amplitude a gate can
+// change is read once and written once, 16 B (complex128) or 8 B (complex64) per access, with
+// consecutive lanes on consecutive enumeration indices so loads coalesce whenever the lowest
+// state bits are free. Reductions accumulate in FP64 and combine partials in a fixed order, so
+// results do not depend on scheduling.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+#include "engine.h"
+
+using qipgpu::cx;
+using qipgpu::insert_zeros;
+using qipgpu::PosList;
+
+namespace qipgpu {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int ensure_scratch(qipgpu_state *s, size_t bytes) {
+    if (bytes <= s->scratch_cap) return QIPGPU_OK;
+    if (s->scratch) {
+        QIP_HIP(hipStreamSynchronize(s->stream));
+        QIP_HIP(hipFree(s->scratch));
+        s->scratch = nullptr;
+        s->scratch_cap = 0;
+    }
+    size_t cap = std::max<size_t>(bytes, 1 << 20);
+    if (hipMalloc(&s->scratch, cap) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(QIPGPU_ENOMEM, "cannot allocate %zu bytes of device scratch", cap);
+    }
+    s->scratch_cap = cap;
+    return QIPGPU_OK;
+}
+
+int check_qubits(const qipgpu_state *s, int32_t k, const int32_t *q, const char *what) {
+    if (k < 0 || k > s->n) return set_error(QIPGPU_EINVAL, "%s: bad qubit count %d for n=%d", what, k, s->n);
+    uint64_t seen = 0;
+    for (int i = 0; i < k; ++i) {
+        if (q[i] < 0 || q[i] >= s->n)
+            return set_error(QIPGPU_EINVAL, "%s: qubit index %d out of range for n=%d", what, q[i], s->n);
+        uint64_t b = 1ull << q[i];
+        if (seen & b) return set_error(QIPGPU_EINVAL, "%s: qubit index %d repeated", what, q[i]);
+        seen |= b;
+    }
+    return QIPGPU_OK;
+}
+
+int ring_upload(qipgpu_state *s, const void *src, size_t bytes, void **dev) {
+    if (bytes > kRingSlotBytes) return set_error(QIPGPU_EINVAL, "operator of %zu bytes exceeds staging slot", bytes);
+    if (!s->d_ring) {
+        QIP_HIP(hipMalloc(&s->d_ring, kRingSlots * kRingSlotBytes));
+        QIP_HIP(hipHostMalloc(&s->h_ring, kRingSlots * kRingSlotBytes, hipHostMallocDefault));
+        for (int i = 0; i < kRingSlots; ++i) QIP_HIP(hipEventCreateWithFlags(&s->ring_ev[i], hipEventDisableTiming));
+    }
+    const int slot = s->ring_next;
+    s->ring_next = (slot + 1) % kRingSlots;
+    if (s->ring_used[slot]) QIP_HIP(hipEventSynchronize(s->ring_ev[slot]));
+    char *h = static_cast<char *>(s->h_ring) + slot * kRingSlotBytes;
+    char *d = static_cast<char *>(s->d_ring) + slot * kRingSlotBytes;
+    std::memcpy(h, src, bytes);
+    QIP_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s->stream));
+    QIP_HIP(hipEventRecord(s->ring_ev[slot], s->stream));
+    s->ring_used[slot] = true;
+    *dev = d;
+    return QIPGPU_OK;
+}
+
+}  // namespace qipgpu
+
+using namespace qipgpu;
+
+// ------------------------------------------------------------------------------------------------
+// small device helpers
+// ------------------------------------------------------------------------------------------------
+
+template <typename R>
+__device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) {
+    cx<R> r;
+    r.re = a.re * b.re - a.im * b.im;
+    r.im = a.re * b.im + a.im * b.re;
+    return r;
+}
+
+template <typename R>
+__device__ __forceinline__ void cfma(cx<R> &acc, cx<R> a, cx<R> b) {
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(-a.im, b.im, acc.re);
+    acc.im = fma(a.re, b.im, acc.im);
+    acc.im = fma(a.im, b.re, acc.im);
+}
+
+// Scatter the low `count` bits of v (bit b of v) onto state bits pos[b].
+__device__ __forceinline__ uint64_t scatter_bits(uint64_t v, const uint8_t *pos, int count) {
+    uint64_t r = 0;
+    for (int b = 0; b < count; ++b) r |= ((v >> b) & 1ull) << pos[b];
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *lds) {
+    // 256-thread block: wave-level shuffle then LDS across the 4 waves, fixed order.
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    T r = 0;
+    if (threadIdx.x == 0) r = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+    __syncthreads();
+    return r;
+}
+
+static inline uint32_t blocks_for(uint64_t work, uint64_t per_block) {
+    uint64_t b = (work + per_block - 1) / per_block;
+    return (uint32_t)std::max<uint64_t>(b, 1);
+}
+
+// ------------------------------------------------------------------------------------------------
+// state lifecycle
+// ------------------------------------------------------------------------------------------------
+
+struct InitArgs {
+    uint64_t len;
+    uint64_t unfed_mask;  // state bits of qubits no feed covers: amplitude is 0 unless they are 0
+    int32_t n_groups;
+    const int32_t *glen;   // device
+    const int32_t *gpos;   // device, state-bit position of each group index, MSB-first per group
+    const int64_t *goff;   // device, offset of each feed in `feeds`
+    const cx<double> *feeds;
+};
+
+// qip/backend.py:88-101: amplitude = prod_g feed_g[value of the index on group g], computed in
+// group order from 1.0 with numpy's complex product (no fused multiply-add), rounded to the
+// state's precision once at the end.
+template <typename R>
+__global__ __launch_bounds__(256) void k_init_product(cx<R> *amps, InitArgs a) {
+#pragma clang fp contract(off)
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.len; i += stride) {
+        double re = 0.0, im = 0.0;
+        if ((i & a.unfed_mask) == 0) {
+            re = 1.0;
+            int pos = 0;
+            for (int g = 0; g < a.n_groups; ++g) {
+                const int L = a.glen[g];
+                uint64_t sub = 0;
+                for (int j = 0; j < L; ++j) sub = (sub << 1) | ((i >> a.gpos[pos + j]) & 1ull);
+                pos += L;
+                const cx<double> f = a.feeds[a.goff[g] + (int64_t)sub];
+                const double nre = re * f.re - im * f.im;
+                const double nim = re * f.im + im * f.re;
+                re = nre;
+                im = nim;
+            }
+        }
+        cx<R> o;
+        o.re = (R)re;
+        o.im = (R)im;
+        amps[i] = o;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_add_range(cx<R> *dst, const cx<R> *src, uint64_t count) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        cx<R> a = dst[i], b = src[i];
+        a.re += b.re;
+        a.im += b.im;
+        dst[i] = a;
+    }
+}
+
+extern "C" {
+
+const char *qipgpu_last_error(void) { return g_last_error.c_str(); }
+
+const char *qipgpu_version(void) { return "qipgpu 0.1 gfx950"; }
+
+int qipgpu_device_count(int32_t *count) {
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *count = 0;
+        return set_error(QIPGPU_EHIP, "hipGetDeviceCount: %s", hipGetErrorString(e));
+    }
+    *count = c;
+    return QIPGPU_OK;
+}
+
+int qipgpu_create(int32_t n, int32_t dtype, qipgpu_state **out) {
+    *out = nullptr;
+    if (n < 1 || n > QIPGPU_MAX_QUBITS) return set_error(QIPGPU_EINVAL, "n=%d outside 1..%d", n, QIPGPU_MAX_QUBITS);
+    if (dtype != QIPGPU_C128 && dtype != QIPGPU_C64) return set_error(QIPGPU_EINVAL, "unknown dtype %d", dtype);
+    qipgpu_state *s = new qipgpu_state();
+    s->n = n;
+    s->dtype = dtype;
+    s->len = 1ull << n;
+    s->amp_bytes = dtype == QIPGPU_C128 ? 16 : 8;
+    QIP_HIP(hipGetDevice(&s->device));
+    if (hipMalloc(&s->amps, s->len * s->amp_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        delete s;
+        return set_error(QIPGPU_ENOMEM, "cannot allocate a %d-qubit state (%llu bytes)", n,
+                         (unsigned long long)(s->len * (dtype == QIPGPU_C128 ? 16 : 8)));
+    }
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipFree(s->amps);
+        delete s;
+        return set_error(QIPGPU_EHIP, "hipStreamCreate failed");
+    }
+    *out = s;
+    return QIPGPU_OK;
+}
+
+int qipgpu_destroy(qipgpu_state *s) {
+    if (!s) return QIPGPU_OK;
+    (void)hipStreamSynchronize(s->stream);
+    if (s->amps) (void)hipFree(s->amps);
+    if (s->scratch) (void)hipFree(s->scratch);
+    if (s->d_ring) (void)hipFree(s->d_ring);
+    if (s->h_ring) (void)hipHostFree(s->h_ring);
+    for (int i = 0; i < kRingSlots; ++i)
+        if (s->ring_ev[i]) (void)hipEventDestroy(s->ring_ev[i]);
+    if (s->d_fused) (void)hipFree(s->d_fused);
+    if (s->d_fmat) (void)hipFree(s->d_fmat);
+    (void)hipStreamDestroy(s->stream);
+    delete s;
+    return QIPGPU_OK;
+}
+
+int qipgpu_info(const qipgpu_state *s, int32_t *n, int32_t *dtype) {
+    *n = s->n;
+    *dtype = s->dtype;
+    return QIPGPU_OK;
+}
+
+int qipgpu_synchronize(qipgpu_state *s) {
+    QIP_HIP(hipStreamSynchronize(s->stream));
+    return QIPGPU_OK;
+}
+
+int qipgpu_init_product(qipgpu_state *s, int32_t n_groups, const int32_t *group_len,
+                        const int32_t *group_idx, const double *feeds) {
+    s->fused_pending = false;  // every amplitude is rewritten
+    if (n_groups < 0) return set_error(QIPGPU_EINVAL, "negative group count");
+    uint64_t fed = 0;
+    int64_t total_idx = 0, total_feed = 0;
+    std::vector<int64_t> goff(std::max(n_groups, 1));
+    for (int g = 0; g < n_groups; ++g) {
+        if (group_len[g] < 1 || group_len[g] > s->n) return set_error(QIPGPU_EINVAL, "bad feed group length %d", group_len[g]);
+        goff[g] = total_feed;
+        total_feed += 1ll << group_len[g];
+        for (int j = 0; j < group_len[g]; ++j) {
+            int q = group_idx[total_idx + j];
+            if (q < 0 || q >= s->n) return set_error(QIPGPU_EINVAL, "feed qubit index %d out of range", q);
+            fed |= 1ull << (s->n - 1 - q);
+        }
+        total_idx += group_len[g];
+    }
+    if (n_groups == 0) {
+        // |0...0> (qip/backend.py:90-91)
+        QIP_HIP(hipMemsetAsync(s->amps, 0, s->len * s->amp_bytes, s->stream));
+        if (s->dtype == QIPGPU_C128) {
+            const double one[2] = {1.0, 0.0};
+            QIP_HIP(hipMemcpyAsync(s->amps, one, sizeof(one), hipMemcpyHostToDevice, s->stream));
+        } else {
+            const float one[2] = {1.0f, 0.0f};
+            QIP_HIP(hipMemcpyAsync(s->amps, one, sizeof(one), hipMemcpyHostToDevice, s->stream));
+        }
+        QIP_HIP(hipStreamSynchronize(s->stream));
+        return QIPGPU_OK;
+    }
+    // device copies of the descriptor: [glen | gpos | pad | goff | feeds]
+    std::vector<int32_t> gpos(total_idx);
+    for (int64_t i = 0; i < total_idx; ++i) gpos[i] = s->n - 1 - group_idx[i];
+    const size_t b_glen = sizeof(int32_t) * n_groups, b_gpos = sizeof(int32_t) * total_idx;
+    const size_t o_goff = ((b_glen + b_gpos + 15) / 16) * 16;
+    const size_t o_feed = o_goff + ((sizeof(int64_t) * n_groups + 15) / 16) * 16;
+    const size_t bytes = o_feed + sizeof(double) * 2 * total_feed;
+    int rc = ensure_scratch(s, bytes);
+    if (rc) return rc;
+    char *d = static_cast<char *>(s->scratch);
+    std::vector<char> h(bytes);
+    std::memcpy(h.data(), group_len, b_glen);
+    std::memcpy(h.data() + b_glen, gpos.data(), b_gpos);
+    std::memcpy(h.data() + o_goff, goff.data(), sizeof(int64_t) * n_groups);
+    std::memcpy(h.data() + o_feed, feeds, sizeof(double) * 2 * total_feed);
+    QIP_HIP(hipMemcpyAsync(d, h.data(), bytes, hipMemcpyHostToDevice, s->stream));
+    InitArgs a;
+    a.len = s->len;
+    a.unfed_mask = (s->len - 1) & ~fed;
+    a.n_groups = n_groups;
+    a.glen = reinterpret_cast<const int32_t *>(d);
+    a.gpos = reinterpret_cast<const int32_t *>(d + b_glen);
+    a.goff = reinterpret_cast<const int64_t *>(d + o_goff);
+    a.feeds = reinterpret_cast<const cx<double> *>(d + o_feed);
+    const uint32_t nb = std::min<uint32_t>(blocks_for(s->len, 256), 1u << 20);
+    if (s->dtype == QIPGPU_C128)
+        hipLaunchKernelGGL(k_init_product<double>, dim3(nb), dim3(256), 0, s->stream, (cx<double> *)s->amps, a);
+    else
+        hipLaunchKernelGGL(k_init_product<float>, dim3(nb), dim3(256), 0, s->stream, (cx<float> *)s->amps, a);
+    QIP_LAUNCHED();
+    QIP_HIP(hipStreamSynchronize(s->stream));  // host descriptor buffer goes out of scope
+    return QIPGPU_OK;
+}
+
+static int check_range(qipgpu_state *s, int64_t start, int64_t end) {
+    if (start < 0 || end < start || (uint64_t)end > s->len)
+        return set_error(QIPGPU_EINVAL, "range [%lld, %lld) outside the %llu-amplitude state", (long long)start,
+                         (long long)end, (unsigned long long)s->len);
+    return QIPGPU_OK;
+}
+
+int qipgpu_get_range(qipgpu_state *s, int64_t start, int64_t end, void *host) {
+    int rc = check_range(s, start, end);
+    if (rc) return rc;
+    if (end == start) return QIPGPU_OK;
+    rc = settle(s);
+    if (rc) return rc;
+    QIP_HIP(hipStreamSynchronize(s->stream));
+    QIP_HIP(hipMemcpy(host, static_cast<char *>(s->amps) + start * s->amp_bytes, (end - start) * s->amp_bytes,
+                      hipMemcpyDeviceToHost));
+    return QIPGPU_OK;
+}
+
+int qipgpu_set_range(qipgpu_state *s, int64_t start, int64_t end, const void *host) {
+    int rc = check_range(s, start, end);
+    if (rc) return rc;
+    if (end == start) return QIPGPU_OK;
+    rc = settle(s);
+    if (rc) return rc;
+    QIP_HIP(hipStreamSynchronize(s->stream));
+    QIP_HIP(hipMemcpy(static_cast<char *>(s->amps) + start * s->amp_bytes, host, (end - start) * s->amp_bytes,
+                      hipMemcpyHostToDevice));
+    return QIPGPU_OK;
+}
+
+int qipgpu_add_range(qipgpu_state *s, int64_t start, int64_t end, const void *host) {
+    int rc = check_range(s, start, end);
+    if (rc) return rc;
+    if (end == start) return QIPGPU_OK;
+    const uint64_t count = end - start;
+    rc = settle(s);
+    if (rc) return rc;
+    rc = ensure_scratch(s, count * s->amp_bytes);
+    if (rc) return rc;
+    QIP_HIP(hipStreamSynchronize(s->stream));
+    QIP_HIP(hipMemcpy(s->scratch, host, count * s->amp_bytes, hipMemcpyHostToDevice));
+    const uint32_t nb = std::min<uint32_t>(blocks_for(count, 256), 1u << 20);
+    if (s->dtype == QIPGPU_C128)
+        hipLaunchKernelGGL(k_add_range<double>, dim3(nb), dim3(256), 0, s->stream,
+                           (cx<double> *)s->amps + start, (const cx<double> *)s->scratch, count);
+    else
+        hipLaunchKernelGGL(k_add_range<float>, dim3(nb), dim3(256), 0, s->stream, (cx<float> *)s->amps + start,
+                           (const cx<float> *)s->scratch, count);
+    QIP_LAUNCHED();
+    QIP_HIP(hipStreamSynchronize(s->stream));
+    return QIPGPU_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------
+// gate application
+// ------------------------------------------------------------------------------------------------
+
+// Dense operator on K <= 3 target qubits, matrix in kernel arguments. One enumeration index g
+// ("base") per item: the state index with all target bits 0 and all control bits 1; the 2^K
+// amplitudes base + off[j] form one closed group the operator mixes. Each thread handles ITEMS
+// bases spaced one block-width apart so every load instruction of a wave is contiguous when the
+// low state bits are free; all loads of a thread are issued before any arithmetic.
+template <typename R, int K>
+struct DenseArgs {
+    cx<R> *amps;
+    uint64_t nbase;
+    uint64_t ctrl_mask;
+    PosList pl;
+    uint64_t off[1 << K];
+    cx<R> U[(1 << K) * (1 << K)];
+};
+
+template <typename R, int K, int ITEMS>
+__global__ __launch_bounds__(256) void k_dense(const DenseArgs<R, K> a) {
+    constexpr int D = 1 << K;
+    const uint64_t g0 = (uint64_t)blockIdx.x * (256 * ITEMS) + threadIdx.x;
+    cx<R> v[ITEMS][D];
+    uint64_t idx[ITEMS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+        const uint64_t g = g0 + (uint64_t)it * 256;
+        idx[it] = insert_zeros(g, a.pl) | a.ctrl_mask;
+        if (g < a.nbase) {
+#pragma unroll
+            for (int j = 0; j < D; ++j) v[it][j] = a.amps[idx[it] + a.off[j]];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+        if (g0 + (uint64_t)it * 256 < a.nbase) {
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                cx<R> acc;
+                acc.re = 0;
+                acc.im = 0;
+#pragma unroll
+                for (int j = 0; j < D; ++j) cfma(acc, a.U[i * D + j], v[it][j]);
+                a.amps[idx[it] + a.off[i]] = acc;
+            }
+        }
+    }
+}
+
+// Dense operator on 4..6 target qubits: one wave per block, each lane owns one base; the lane's
+// 2^K amplitudes are staged in LDS ([j][lane], conflict-free) and the matrix is read from the
+// device with wave-uniform addresses.
+template <typename R, int K>
+__global__ __launch_bounds__(64) void k_dense_lds(cx<R> *amps, uint64_t nbase, uint64_t ctrl_mask, PosList pl,
+                                                  const uint64_t *__restrict__ off, const cx<R> *__restrict__ U) {
+    constexpr int D = 1 << K;
+    __shared__ cx<R> v[D][64];
+    const uint64_t g = (uint64_t)blockIdx.x * 64 + threadIdx.x;
+    if (g >= nbase) return;
+    const uint64_t base = insert_zeros(g, pl) | ctrl_mask;
+    for (int j = 0; j < D; ++j) v[j][threadIdx.x] = amps[base + off[j]];
+    for (int i = 0; i < D; ++i) {
+        cx<R> acc;
+        acc.re = 0;
+        acc.im = 0;
+        for (int j = 0; j < D; ++j) cfma(acc, U[i * D + j], v[j][threadIdx.x]);
+        amps[base + off[i]] = acc;
+    }
+}
+
+// Diagonal operator on K <= 6 targets (K = 0: a phase on the controlled subspace). One item per
+// amplitude of the controlled subspace.
+template <typename R, int K>
+struct DiagArgs {
+    cx<R> *amps;
+    uint64_t count;
+    uint64_t ctrl_mask;
+    PosList pl;          // control positions only
+    uint8_t tpos[8];     // state bit of target j (j = 0 is the MSB of the diagonal index)
+    cx<R> d[1 << K];
+};
+
+template <typename R, int K, int ITEMS>
+__global__ __launch_bounds__(256) void k_diag(const DiagArgs<R, K> a) {
+    const uint64_t g0 = (uint64_t)blockIdx.x * (256 * ITEMS) + threadIdx.x;
+    cx<R> v[ITEMS];
+    uint64_t idx[ITEMS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+        const uint64_t g = g0 + (uint64_t)it * 256;
+        idx[it] = insert_zeros(g, a.pl) | a.ctrl_mask;
+        if (g < a.count) v[it] = a.amps[idx[it]];
+    }
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+        if (g0 + (uint64_t)it * 256 < a.count) {
+            uint32_t j = 0;
+#pragma unroll
+            for (int b = 0; b < K; ++b) j = (j << 1) | (uint32_t)((idx[it] >> a.tpos[b]) & 1ull);
+            a.amps[idx[it]] = cmul(v[it], a.d[j]);
+        }
+    }
+}
+
+namespace {
+
+using cd = std::complex<double>;
+
+// Structure of a dense operator: control qubits hidden in it, and whether the rest is diagonal.
+struct Analysed {
+    std::vector<int32_t> targets;   // remaining target qubits, MSB first
+    std::vector<int32_t> controls;  // extra control qubits found
+    std::vector<cd> U;              // operator on `targets`
+    bool diagonal = false;
+};
+
+// Bit b (counted from the LSB of the matrix index) is a control when the operator never changes
+// it and acts as the identity where it is 0; the operator is then diag(I, M') and M' is the
+// bit-1 block (the C_MAT structure of kronprod.pyx:215-225).
+static bool is_control_bit(const std::vector<cd> &U, int k, int b) {
+    const int D = 1 << k;
+    const int m = 1 << b;
+    for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) {
+            const cd e = U[r * D + c];
+            if ((r & m) != (c & m)) {
+                if (e != 0.0) return false;
+            } else if (!(r & m)) {
+                if (e != (r == c ? 1.0 : 0.0)) return false;
+            }
+        }
+    return true;
+}
+
+static std::vector<cd> take_block(const std::vector<cd> &U, int k, int b) {
+    const int D = 1 << k, Dn = D >> 1;
+    std::vector<cd> out(Dn * Dn);
+    auto expand = [b](int x) {
+        const int lo = x & ((1 << b) - 1);
+        return ((x >> b) << (b + 1)) | (1 << b) | lo;
+    };
+    for (int r = 0; r < Dn; ++r)
+        for (int c = 0; c < Dn; ++c) out[r * Dn + c] = U[expand(r) * D + expand(c)];
+    return out;
+}
+
+static Analysed analyse(int k, const int32_t *targets, const double *mat) {
+    Analysed a;
+    a.targets.assign(targets, targets + k);
+    const int D = 1 << k;
+    a.U.resize((size_t)D * D);
+    for (int i = 0; i < D * D; ++i) a.U[i] = cd(mat[2 * i], mat[2 * i + 1]);
+    bool found = true;
+    while (found && !a.targets.empty()) {
+        found = false;
+        const int kk = (int)a.targets.size();
+        for (int b = 0; b < kk; ++b) {
+            if (is_control_bit(a.U, kk, b)) {
+                a.U = take_block(a.U, kk, b);
+                const int j = kk - 1 - b;  // targets[j] carries matrix bit b
+                a.controls.push_back(a.targets[j]);
+                a.targets.erase(a.targets.begin() + j);
+                found = true;
+                break;
+            }
+        }
+    }
+    const int kk = (int)a.targets.size();
+    const int Dk = 1 << kk;
+    a.diagonal = true;
+    for (int r = 0; r < Dk && a.diagonal; ++r)
+        for (int c = 0; c < Dk; ++c)
+            if (r != c && a.U[r * Dk + c] != 0.0) {
+                a.diagonal = false;
+                break;
+            }
+    return a;
+}
+
+static void fill_poslist(PosList &pl, std::vector<int> pos) {
+    std::sort(pos.begin(), pos.end());
+    pl.count = (int32_t)pos.size();
+    for (size_t i = 0; i < pos.size(); ++i) pl.p[i] = (uint8_t)pos[i];
+}
+
+template <typename R, int K>
+static int launch_dense_reg(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mask,
+                            const std::vector<int> &allpos, const std::vector<cd> &U) {
+    DenseArgs<R, K> a;
+    a.amps = (cx<R> *)s->amps;
+    a.nbase = s->len >> allpos.size();
+    a.ctrl_mask = ctrl_mask;
+    fill_poslist(a.pl, allpos);
+    constexpr int D = 1 << K;
+    for (int j = 0; j < D; ++j) {
+        uint64_t o = 0;
+        for (int t = 0; t < K; ++t)
+            if ((j >> (K - 1 - t)) & 1) o |= 1ull << tbits[t];
+        a.off[j] = o;
+    }
+    for (int i = 0; i < D * D; ++i) {
+        a.U[i].re = (R)U[i].real();
+        a.U[i].im = (R)U[i].imag();
+    }
+    constexpr int ITEMS = K == 1 ? 4 : (K == 2 ? 2 : 1);
+    const uint32_t nb = blocks_for(a.nbase, 256 * ITEMS);
+    hipLaunchKernelGGL((k_dense<R, K, ITEMS>), dim3(nb), dim3(256), 0, s->stream, a);
+    QIP_LAUNCHED();
+    return QIPGPU_OK;
+}
+
+template <typename R, int K>
+static int launch_dense_lds(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mask,
+                            const std::vector<int> &allpos, const std::vector<cd> &U) {
+    constexpr int D = 1 << K;
+    // upload [off (D uint64) | U (D*D cx<R>)] through the staging ring
+    std::vector<char> buf(sizeof(uint64_t) * D + sizeof(cx<R>) * D * D);
+    uint64_t *off = reinterpret_cast<uint64_t *>(buf.data());
+    cx<R> *Uh = reinterpret_cast<cx<R> *>(buf.data() + sizeof(uint64_t) * D);
+    for (int j = 0; j < D; ++j) {
+        uint64_t o = 0;
+        for (int t = 0; t < K; ++t)
+            if ((j >> (K - 1 - t)) & 1) o |= 1ull << tbits[t];
+        off[j] = o;
+    }
+    for (int i = 0; i < D * D; ++i) {
+        Uh[i].re = (R)U[i].real();
+        Uh[i].im = (R)U[i].imag();
+    }
+    void *d = nullptr;
+    int rc = ring_upload(s, buf.data(), buf.size(), &d);
+    if (rc) return rc;
+    PosList pl;
+    fill_poslist(pl, allpos);
+    const uint64_t nbase = s->len >> allpos.size();
+    const uint32_t nb = blocks_for(nbase, 64);
+    hipLaunchKernelGGL((k_dense_lds<R, K>), dim3(nb), dim3(64), 0, s->stream, (cx<R> *)s->amps, nbase, ctrl_mask, pl,
+                       (const uint64_t *)d, (const cx<R> *)((char *)d + sizeof(uint64_t) * D));
+    QIP_LAUNCHED();
+    return QIPGPU_OK;
+}
+
+template <typename R, int K>
+static int launch_diag(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mask,
+                       const std::vector<int> &cpos, const std::vector<cd> &d) {
+    DiagArgs<R, K> a;
+    a.amps = (cx<R> *)s->amps;
+    a.count = s->len >> cpos.size();
+    a.ctrl_mask = ctrl_mask;
+    fill_poslist(a.pl, cpos);
+    for (int b = 0; b < K; ++b) a.tpos[b] = (uint8_t)tbits[b];
+    for (int j = 0; j < (1 << K); ++j) {
+        a.d[j].re = (R)d[j].real();
+        a.d[j].im = (R)d[j].imag();
+    }
+    constexpr int ITEMS = 4;
+    const uint32_t nb = blocks_for(a.count, 256 * ITEMS);
+    hipLaunchKernelGGL((k_diag<R, K, ITEMS>), dim3(nb), dim3(256), 0, s->stream, a);
+    QIP_LAUNCHED();
+    return QIPGPU_OK;
+}
+
+template <typename R>
+static int dispatch(qipgpu_state *s, const Analysed &an, const std::vector<int32_t> &controls_in) {
+    const int n = s->n;
+    std::vector<int> tbits, allpos, cpos;
+    uint64_t ctrl_mask = 0;
+    for (int32_t q : an.targets) tbits.push_back(n - 1 - q);
+    std::vector<int32_t> controls(controls_in);
+    controls.insert(controls.end(), an.controls.begin(), an.controls.end());
+    for (int32_t q : controls) {
+        cpos.push_back(n - 1 - q);
+        ctrl_mask |= 1ull << (n - 1 - q);
+    }
+    allpos = cpos;
+    allpos.insert(allpos.end(), tbits.begin(), tbits.end());
+    const int k = (int)an.targets.size();
+    if (an.diagonal) {
+        const int D = 1 << k;
+        std::vector<cd> d(D);
+        bool identity = true;
+        for (int j = 0; j < D; ++j) {
+            d[j] = an.U[j * D + j];
+            identity = identity && d[j] == 1.0;
+        }
+        if (identity) return QIPGPU_OK;
+        switch (k) {
+            case 0: return launch_diag<R, 0>(s, tbits, ctrl_mask, cpos, d);
+            case 1: return launch_diag<R, 1>(s, tbits, ctrl_mask, cpos, d);
+            case 2: return launch_diag<R, 2>(s, tbits, ctrl_mask, cpos, d);
+            case 3: return launch_diag<R, 3>(s, tbits, ctrl_mask, cpos, d);
+            case 4: return launch_diag<R, 4>(s, tbits, ctrl_mask, cpos, d);
+            case 5: return launch_diag<R, 5>(s, tbits, ctrl_mask, cpos, d);
+            case 6: return launch_diag<R, 6>(s, tbits, ctrl_mask, cpos, d);
+            default: break;
+        }
+        return set_error(QIPGPU_EUNSUP, "diagonal operator on %d qubits exceeds %d", k, QIPGPU_MAX_DENSE);
+    }
+    switch (k) {
+        case 1: return launch_dense_reg<R, 1>(s, tbits, ctrl_mask, allpos, an.U);
+        case 2: return launch_dense_reg<R, 2>(s, tbits, ctrl_mask, allpos, an.U);
+        case 3: return launch_dense_reg<R, 3>(s, tbits, ctrl_mask, allpos, an.U);
+        case 4: return launch_dense_lds<R, 4>(s, tbits, ctrl_mask, allpos, an.U);
+        case 5: return launch_dense_lds<R, 5>(s, tbits, ctrl_mask, allpos, an.U);
+        case 6: return launch_dense_lds<R, 6>(s, tbits, ctrl_mask, allpos, an.U);
+        default: break;
+    }
+    return set_error(QIPGPU_EUNSUP, "dense operator on %d target qubits exceeds %d", k, QIPGPU_MAX_DENSE);
+}
+
+static int check_disjoint(const qipgpu_state *s, int32_t k, const int32_t *t, int32_t c, const int32_t *ctl) {
+    std::vector<int32_t> all(t, t + k);
+    all.insert(all.end(), ctl, ctl + c);
+    return check_qubits(s, (int32_t)all.size(), all.data(), "operator qubits");
+}
+
+}  // namespace
+
+extern "C" {
+
+int qipgpu_apply_matrix(qipgpu_state *s, int32_t k, const int32_t *targets, const double *mat, int32_t n_ctrl,
+                        const int32_t *controls) {
+    if (k < 0 || k > 12) return set_error(QIPGPU_EINVAL, "operator on %d qubits: at most 12 accepted", k);
+    if (n_ctrl < 0) return set_error(QIPGPU_EINVAL, "negative control count");
+    int rc = check_disjoint(s, k, targets, n_ctrl, controls);
+    if (rc) return rc;
+    rc = settle(s);
+    if (rc) return rc;
+    Analysed an = analyse(k, targets, mat);
+    std::vector<int32_t> ctl(controls, controls + n_ctrl);
+    return s->dtype == QIPGPU_C128 ? dispatch<double>(s, an, ctl) : dispatch<float>(s, an, ctl);
+}
+
+int qipgpu_apply_diagonal(qipgpu_state *s, int32_t k, const int32_t *targets, const double *diag, int32_t n_ctrl,
+                          const int32_t *controls) {
+    if (k < 0 || k > QIPGPU_MAX_DENSE) return set_error(QIPGPU_EINVAL, "diagonal on %d qubits", k);
+    int rc = check_disjoint(s, k, targets, n_ctrl, controls);
+    if (rc) return rc;
+    rc = settle(s);
+    if (rc) return rc;
+    Analysed an;
+    an.targets.assign(targets, targets + k);
+    const int D = 1 << k;
+    an.U.assign((size_t)D * D, cd(0.0, 0.0));
+    for (int j = 0; j < D; ++j) an.U[j * D + j] = cd(diag[2 * j], diag[2 * j + 1]);
+    an.diagonal = true;
+    std::vector<int32_t> ctl(controls, controls + n_ctrl);
+    return s->dtype == QIPGPU_C128 ? dispatch<double>(s, an, ctl) : dispatch<float>(s, an, ctl);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------
+// func_apply (qip/ext/func_apply.pyx:11-112), in place
+// ------------------------------------------------------------------------------------------------
+
+struct qipgpu_table {
+    int32_t n_reg1;
+    int32_t *d;
+};
+
+// For x with y = f(x) & (2^n2 - 1) != 0 the map q -> q xor y pairs the reg2 values; each thread
+// owns one pair (q with bit hb(y) clear, q xor y) for one rest value r and swaps the two
+// amplitudes, which is exactly output[x, q^y, r] = input[x, q, r]. x with y == 0 is untouched.
+struct FuncArgs {
+    uint64_t nthreads;
+    int32_t n1, n2, nother, rbits;
+    uint32_t ymask;
+    uint8_t xpos[QIPGPU_MAX_QUBITS];  // value bit b -> state bit
+    uint8_t qpos[QIPGPU_MAX_QUBITS];
+    uint8_t rpos[QIPGPU_MAX_QUBITS];
+};
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_func_apply(cx<R> *amps, const int32_t *__restrict__ table, FuncArgs a) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nthreads) return;
+    const uint64_t x = t >> a.rbits;
+    const uint32_t y = (uint32_t)table[x] & a.ymask;
+    if (y == 0) return;
+    const uint64_t rest = t & ((1ull << a.rbits) - 1ull);
+    const int hb = 31 - __clz(y);
+    const uint64_t qlow = rest & ((1ull << (a.n2 - 1)) - 1ull);
+    const uint64_t r = rest >> (a.n2 - 1);
+    const uint64_t q = (((qlow >> hb) << (hb + 1)) | (qlow & ((1ull << hb) - 1ull)));
+    const uint64_t ia = scatter_bits(x, a.xpos, a.n1) | scatter_bits(q, a.qpos, a.n2) | scatter_bits(r, a.rpos, a.nother);
+    const uint64_t ib = ia ^ scatter_bits(y, a.qpos, a.n2);
+    const cx<R> va = amps[ia], vb = amps[ib];
+    amps[ia] = vb;
+    amps[ib] = va;
+}
+
+namespace {
+
+static int func_apply_dev(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, int32_t n_reg2, const int32_t *reg2,
+                          const int32_t *d_table) {
+    const int n = s->n;
+    if (n_reg1 < 1 || n_reg2 < 1 || n_reg1 + n_reg2 > n)
+        return set_error(QIPGPU_EINVAL, "func_apply: register sizes %d + %d for n=%d", n_reg1, n_reg2, n);
+    std::vector<int32_t> all(reg1, reg1 + n_reg1);
+    all.insert(all.end(), reg2, reg2 + n_reg2);
+    int rc = check_qubits(s, (int32_t)all.size(), all.data(), "func_apply registers");
+    if (rc) return rc;
+    rc = settle(s);
+    if (rc) return rc;
+    FuncArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.n1 = n_reg1;
+    a.n2 = n_reg2;
+    a.nother = n - n_reg1 - n_reg2;
+    a.rbits = n - n_reg1 - 1;
+    a.ymask = n_reg2 >= 32 ? 0xffffffffu : ((1u << n_reg2) - 1u);
+    a.nthreads = 1ull << (n - 1);
+    // x is read MSB-first from reg1 (func_apply.pyx:82-85): value bit b <- reg1[n1-1-b]
+    for (int b = 0; b < n_reg1; ++b) a.xpos[b] = (uint8_t)(n - 1 - reg1[n_reg1 - 1 - b]);
+    for (int b = 0; b < n_reg2; ++b) a.qpos[b] = (uint8_t)(n - 1 - reg2[n_reg2 - 1 - b]);
+    // rest qubits ascending (func_apply.pyx:58-60), value MSB-first over them
+    std::vector<int> other;
+    uint64_t used = 0;
+    for (int32_t q : all) used |= 1ull << q;
+    for (int q = 0; q < n; ++q)
+        if (!(used & (1ull << q))) other.push_back(q);
+    for (int b = 0; b < a.nother; ++b) a.rpos[b] = (uint8_t)(n - 1 - other[a.nother - 1 - b]);
+    const uint32_t nb = blocks_for(a.nthreads, 256);
+    if (s->dtype == QIPGPU_C128)
+        hipLaunchKernelGGL(k_func_apply<double>, dim3(nb), dim3(256), 0, s->stream, (cx<double> *)s->amps, d_table, a);
+    else
+        hipLaunchKernelGGL(k_func_apply<float>, dim3(nb), dim3(256), 0, s->stream, (cx<float> *)s->amps, d_table, a);
+    QIP_LAUNCHED();
+    return QIPGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qipgpu_table_create(int32_t n_reg1, const int32_t *table, qipgpu_table **out) {
+    *out = nullptr;
+    if (n_reg1 < 1 || n_reg1 > QIPGPU_MAX_QUBITS) return set_error(QIPGPU_EINVAL, "table for %d-qubit register", n_reg1);
+    qipgpu_table *t = new qipgpu_table();
+    t->n_reg1 = n_reg1;
+    const size_t bytes = sizeof(int32_t) << n_reg1;
+    if (hipMalloc(&t->d, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        delete t;
+        return set_error(QIPGPU_ENOMEM, "cannot allocate %zu-byte function table", bytes);
+    }
+    if (hipMemcpy(t->d, table, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(t->d);
+        delete t;
+        return set_error(QIPGPU_EHIP, "function table upload failed");
+    }
+    *out = t;
+    return QIPGPU_OK;
+}
+
+int qipgpu_table_destroy(qipgpu_table *t) {
+    if (!t) return QIPGPU_OK;
+    (void)hipDeviceSynchronize();
+    (void)hipFree(t->d);
+    delete t;
+    return QIPGPU_OK;
+}
+
+int qipgpu_func_apply_table(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, int32_t n_reg2, const int32_t *reg2,
+                            const qipgpu_table *t) {
+    if (!t || t->n_reg1 != n_reg1)
+        return set_error(QIPGPU_EINVAL, "function table is for a %d-qubit register, not %d", t ? t->n_reg1 : -1, n_reg1);
+    return func_apply_dev(s, n_reg1, reg1, n_reg2, reg2, t->d);
+}
+
+int qipgpu_func_apply(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, int32_t n_reg2, const int32_t *reg2,
+                      const int32_t *table) {
+    if (n_reg1 < 1 || n_reg1 > s->n) return set_error(QIPGPU_EINVAL, "func_apply: register size %d", n_reg1);
+    const size_t bytes = sizeof(int32_t) << n_reg1;
+    int rc = ensure_scratch(s, bytes);
+    if (rc) return rc;
+    QIP_HIP(hipStreamSynchronize(s->stream));
+    QIP_HIP(hipMemcpy(s->scratch, table, bytes, hipMemcpyHostToDevice));
+    rc = func_apply_dev(s, n_reg1, reg1, n_reg2, reg2, (const int32_t *)s->scratch);
+    if (rc) return rc;
+    QIP_HIP(hipStreamSynchronize(s->stream));  // scratch may be reused by the next call
+    return QIPGPU_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------
+// reductions
+// ------------------------------------------------------------------------------------------------
+
+constexpr uint32_t kNormBlocks = 2048;
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_norm_partial(const cx<R> *__restrict__ amps, uint64_t len, double *partial) {
+    __shared__ double lds[4];
+    const uint64_t per = (len + gridDim.x - 1) / gridDim.x;
+    const uint64_t lo = (uint64_t)blockIdx.x * per, hi = min(len, lo + per);
+    double acc = 0.0;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += 256) {
+        const cx<R> v = amps[i];
+        const double re = v.re, im = v.im;
+        acc = fma(re, re, acc);
+        acc = fma(im, im, acc);
+    }
+    acc = block_sum(acc, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// Sum `count` partials in a fixed order with one block; result to dst[0].
+__global__ __launch_bounds__(256) void k_sum_final(const double *__restrict__ partial, uint32_t count, double *dst) {
+    __shared__ double lds[4];
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < count; i += 256) acc += partial[i];
+    acc = block_sum(acc, lds);
+    if (threadIdx.x == 0) dst[0] = acc;
+}
+
+namespace qipgpu {
+
+int norm_to_device(qipgpu_state *s, double *dst) {
+    const uint32_t nb = (uint32_t)std::min<uint64_t>(kNormBlocks, std::max<uint64_t>(1, s->len / 256));
+    int rc = ensure_scratch(s, sizeof(double) * (nb + 8));
+    if (rc) return rc;
+    double *partial = static_cast<double *>(s->scratch) + 8;
+    if (s->dtype == QIPGPU_C128)
+        hipLaunchKernelGGL(k_norm_partial<double>, dim3(nb), dim3(256), 0, s->stream, (const cx<double> *)s->amps,
+                           s->len, partial);
+    else
+        hipLaunchKernelGGL(k_norm_partial<float>, dim3(nb), dim3(256), 0, s->stream, (const cx<float> *)s->amps, s->len,
+                           partial);
+    QIP_LAUNCHED();
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s->stream, partial, nb, dst);
+    QIP_LAUNCHED();
+    return QIPGPU_OK;
+}
+
+}  // namespace qipgpu
+
+// Marginals: out[m] (m in [m_lo, m_lo + m_count)) = sum over r of |amps[S(m) | P(r)]|^2 where S
+// scatters the k output bits onto state bits opos[] and P inserts zero bits at the measured
+// positions (so r enumerates the unmeasured bits in ascending order).
+struct MargArgs {
+    uint64_t m_lo, m_count, terms, chunk, nchunks;
+    int32_t k;
+    uint8_t opos[QIPGPU_MAX_QUBITS];
+    PosList mpl;  // measured state-bit positions, ascending
+};
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_marg_serial(const cx<R> *__restrict__ amps, MargArgs a, double *out) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.m_count) return;
+    const uint64_t m = a.m_lo + t;
+    const uint64_t base = scatter_bits(m, a.opos, a.k);
+    double acc = 0.0;
+    for (uint64_t r = 0; r < a.terms; ++r) {
+        const cx<R> v = amps[base | insert_zeros(r, a.mpl)];
+        const double re = v.re, im = v.im;
+        acc = fma(re, re, acc);
+        acc = fma(im, im, acc);
+    }
+    out[t] = acc;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_marg_chunk(const cx<R> *__restrict__ amps, MargArgs a, double *partial) {
+    __shared__ double lds[4];
+    const uint64_t c = blockIdx.x % a.nchunks;
+    const uint64_t mt = blockIdx.x / a.nchunks;
+    const uint64_t base = scatter_bits(a.m_lo + mt, a.opos, a.k);
+    const uint64_t r0 = c * a.chunk, r1 = min(a.terms, r0 + a.chunk);
+    double acc = 0.0;
+    for (uint64_t r = r0 + threadIdx.x; r < r1; r += 256) {
+        const cx<R> v = amps[base | insert_zeros(r, a.mpl)];
+        const double re = v.re, im = v.im;
+        acc = fma(re, re, acc);
+        acc = fma(im, im, acc);
+    }
+    acc = block_sum(acc, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_marg_final(const double *__restrict__ partial, uint64_t m_count, uint64_t nchunks,
+                                                    double *out) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m_count) return;
+    double acc = 0.0;
+    for (uint64_t c = 0; c < nchunks; ++c) acc += partial[t * nchunks + c];
+    out[t] = acc;
+}
+
+namespace {
+
+enum class Order { LsbFirst, Entwine };
+
+// Compute marginals over outcomes [m_lo, m_lo + m_count) into host `out`.
+static int marginals_host(qipgpu_state *s, int32_t k, const int32_t *indices, Order order, uint64_t m_lo,
+                          uint64_t m_count, double *out) {
+    int rc = check_qubits(s, k, indices, "measured qubits");
+    if (rc) return rc;
+    rc = settle(s);
+    if (rc) return rc;
+    const int n = s->n;
+    MargArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.k = k;
+    a.m_lo = m_lo;
+    a.m_count = m_count;
+    a.terms = 1ull << (n - k);
+    std::vector<int> mpos;
+    for (int j = 0; j < k; ++j) mpos.push_back(n - 1 - indices[j]);
+    if (order == Order::LsbFirst) {
+        // kronprod.pyx:258-259: qubit indices[j] on output bit j
+        for (int j = 0; j < k; ++j) a.opos[j] = (uint8_t)(n - 1 - indices[j]);
+    } else {
+        // entwine_bit: output bits in the order of the measured state bits
+        std::vector<int> sorted(mpos);
+        std::sort(sorted.begin(), sorted.end());
+        for (int j = 0; j < k; ++j) a.opos[j] = (uint8_t)sorted[j];
+    }
+    fill_poslist(a.mpl, mpos);
+    const bool serial = a.terms <= 64;
+    a.chunk = 256 * 64;
+    a.nchunks = serial ? 1 : (a.terms + a.chunk - 1) / a.chunk;
+    const size_t out_bytes = sizeof(double) * m_count;
+    const size_t part_bytes = serial ? 0 : sizeof(double) * m_count * a.nchunks;
+    const size_t part_off = ((out_bytes + 255) / 256) * 256;
+    rc = ensure_scratch(s, part_off + part_bytes);
+    if (rc) return rc;
+    double *d_out = static_cast<double *>(s->scratch);
+    double *d_part = reinterpret_cast<double *>(static_cast<char *>(s->scratch) + part_off);
+    if (serial) {
+        const uint32_t nb = blocks_for(m_count, 256);
+        if (s->dtype == QIPGPU_C128)
+            hipLaunchKernelGGL(k_marg_serial<double>, dim3(nb), dim3(256), 0, s->stream, (const cx<double> *)s->amps, a, d_out);
+        else
+            hipLaunchKernelGGL(k_marg_serial<float>, dim3(nb), dim3(256), 0, s->stream, (const cx<float> *)s->amps, a, d_out);
+        QIP_LAUNCHED();
+    } else {
+        const uint64_t nblk = m_count * a.nchunks;
+        if (nblk > 0x7fffffffull) return set_error(QIPGPU_EUNSUP, "marginal grid too large");
+        if (s->dtype == QIPGPU_C128)
+            hipLaunchKernelGGL(k_marg_chunk<double>, dim3((uint32_t)nblk), dim3(256), 0, s->stream, (const cx<double> *)s->amps,
+                               a, d_part);
+        else
+            hipLaunchKernelGGL(k_marg_chunk<float>, dim3((uint32_t)nblk), dim3(256), 0, s->stream, (const cx<float> *)s->amps, a,
+                               d_part);
+        QIP_LAUNCHED();
+        hipLaunchKernelGGL(k_marg_final, dim3(blocks_for(m_count, 256)), dim3(256), 0, s->stream, d_part, m_count, a.nchunks,
+                           d_out);
+        QIP_LAUNCHED();
+    }
+    QIP_HIP(hipMemcpyAsync(out, d_out, out_bytes, hipMemcpyDeviceToHost, s->stream));
+    QIP_HIP(hipStreamSynchronize(s->stream));
+    return QIPGPU_OK;
+}
+
+}  // namespace
+
+// Collapse (kronprod.pyx:439-444), in place: amplitudes whose measured bits equal `target` are
+// scaled, the rest are zeroed without being read.
+template <typename R>
+__global__ __launch_bounds__(256) void k_collapse(cx<R> *amps, uint64_t len, uint64_t mask, uint64_t target, double mult) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
+        cx<R> o;
+        if ((i & mask) == target) {
+            const cx<R> v = amps[i];
+            o.re = (R)((double)v.re * mult);
+            o.im = (R)((double)v.im * mult);
+        } else {
+            o.re = 0;
+            o.im = 0;
+        }
+        amps[i] = o;
+    }
+}
+
+// reduce_measure (kronprod.pyx:482-489): out[i] = in[measured bits m, unmeasured bits = low
+// n-k bits of i] * mult, out of place.
+template <typename R>
+__global__ __launch_bounds__(256) void k_reduce_tiled(const cx<R> *__restrict__ in, cx<R> *out, uint64_t len, uint64_t mbits,
+                                                      PosList mpl, uint64_t lowmask, double mult) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
+        const cx<R> v = in[mbits | insert_zeros(i & lowmask, mpl)];
+        cx<R> o;
+        o.re = (R)((double)v.re * mult);
+        o.im = (R)((double)v.im * mult);
+        out[i] = o;
+    }
+}
+
+namespace {
+
+static uint64_t measured_bits(int n, int32_t k, const int32_t *indices, uint64_t m, uint64_t *mask_out) {
+    std::vector<int> pos;
+    uint64_t mask = 0;
+    for (int j = 0; j < k; ++j) {
+        pos.push_back(n - 1 - indices[j]);
+        mask |= 1ull << (n - 1 - indices[j]);
+    }
+    std::sort(pos.begin(), pos.end());
+    uint64_t bits = 0;
+    for (int j = 0; j < k; ++j) bits |= ((m >> j) & 1ull) << pos[j];
+    *mask_out = mask;
+    return bits;
+}
+
+static int soft_measure_impl(qipgpu_state *s, int32_t k, const int32_t *indices, double u, int64_t measured, int64_t *m_out,
+                             double *p_out) {
+    int rc = check_qubits(s, k, indices, "measured qubits");
+    if (rc) return rc;
+    if (k < 1) return set_error(QIPGPU_EINVAL, "measure needs at least one qubit");
+    const uint64_t nm = 1ull << k;
+    if (measured >= (int64_t)nm) return set_error(QIPGPU_EINVAL, "Measured value must be less than 2**len(indices)");
+    // r = u * prob_magnitude(vec) (kronprod.pyx:362-365)
+    double total = 0.0;
+    rc = qipgpu_total_prob(s, &total);
+    if (rc) return rc;
+    double r = u * total;
+    if (measured >= 0) {
+        double acc = 0.0;
+        rc = marginals_host(s, k, indices, Order::Entwine, (uint64_t)measured, 1, &acc);
+        if (rc) return rc;
+        *m_out = measured;
+        *p_out = acc;
+        return QIPGPU_OK;
+    }
+    std::vector<double> marg(nm);
+    rc = marginals_host(s, k, indices, Order::Entwine, 0, nm, marg.data());
+    if (rc) return rc;
+    // kronprod.pyx:371-381: sequential subtraction, first outcome with r <= 0; if none, the last.
+    uint64_t p_index = 0;
+    double acc = 0.0;
+    for (p_index = 0; p_index < nm; ++p_index) {
+        acc = marg[p_index];
+        r -= acc;
+        if (r <= 0) break;
+    }
+    if (p_index == nm) p_index = nm - 1;
+    *m_out = (int64_t)p_index;
+    *p_out = acc;
+    return QIPGPU_OK;
+}
+
+static int resolve_outcome(qipgpu_state *s, int32_t k, const int32_t *indices, double u, int64_t measured,
+                           double measured_prob, int64_t *m, double *p) {
+    if (k < 1) return set_error(QIPGPU_EINVAL, "measure needs at least one qubit");
+    if (measured >= 0 && (uint64_t)measured >= (1ull << k))
+        return set_error(QIPGPU_EINVAL, "Measured value must be less than 2**len(indices)");
+    if (measured_prob > 1.0) return set_error(QIPGPU_EINVAL, "measured_prob must be 0 < p <= 1");
+    if (measured < 0 || !(measured_prob > 0.0)) {
+        int rc = soft_measure_impl(s, k, indices, u, measured, m, p);
+        if (rc) return rc;
+    } else {
+        *m = measured;
+        *p = measured_prob;
+    }
+    if (*p == 0.0) return set_error(QIPGPU_EZERODIV, "float division by zero");
+    return QIPGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qipgpu_total_prob(qipgpu_state *s, double *out) {
+    int rc = settle(s);
+    if (rc) return rc;
+    // size the scratch first: norm_to_device keeps its partials behind the result slot
+    rc = ensure_scratch(s, sizeof(double) * (kNormBlocks + 8));
+    if (rc) return rc;
+    rc = norm_to_device(s, static_cast<double *>(s->scratch));
+    if (rc) return rc;
+    QIP_HIP(hipMemcpyAsync(out, s->scratch, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    QIP_HIP(hipStreamSynchronize(s->stream));
+    return QIPGPU_OK;
+}
+
+int qipgpu_probabilities(qipgpu_state *s, int32_t k, const int32_t *indices, double *out) {
+    if (k < 0 || k > s->n) return set_error(QIPGPU_EINVAL, "bad qubit count %d", k);
+    return marginals_host(s, k, indices, Order::LsbFirst, 0, 1ull << k, out);
+}
+
+int qipgpu_marginals(qipgpu_state *s, int32_t k, const int32_t *indices, double *out) {
+    if (k < 0 || k > s->n) return set_error(QIPGPU_EINVAL, "bad qubit count %d", k);
+    return marginals_host(s, k, indices, Order::Entwine, 0, 1ull << k, out);
+}
+
+int qipgpu_top_probabilities(qipgpu_state *s, int32_t k, const int32_t *indices, int32_t top_k, int64_t *ms, double *ps,
+                             int32_t *count) {
+    if (k < 0 || k > s->n) return set_error(QIPGPU_EINVAL, "bad qubit count %d", k);
+    if (top_k < 0) return set_error(QIPGPU_EINVAL, "negative top_k");
+    const uint64_t nm = 1ull << k;
+    std::vector<double> marg(nm);
+    int rc = marginals_host(s, k, indices, Order::Entwine, 0, nm, marg.data());
+    if (rc) return rc;
+    const uint64_t kk = std::min<uint64_t>((uint64_t)top_k, nm);
+    std::vector<uint64_t> order(nm);
+    for (uint64_t i = 0; i < nm; ++i) order[i] = i;
+    std::partial_sort(order.begin(), order.begin() + kk, order.end(), [&](uint64_t a, uint64_t b) {
+        return marg[a] > marg[b] || (marg[a] == marg[b] && a < b);
+    });
+    for (uint64_t i = 0; i < kk; ++i) {
+        ms[i] = (int64_t)order[i];
+        ps[i] = marg[order[i]];
+    }
+    *count = (int32_t)kk;
+    return QIPGPU_OK;
+}
+
+int qipgpu_soft_measure(qipgpu_state *s, int32_t k, const int32_t *indices, double u, int64_t measured, int64_t *m,
+                        double *p) {
+    return soft_measure_impl(s, k, indices, u, measured, m, p);
+}
+
+int qipgpu_measure(qipgpu_state *s, int32_t k, const int32_t *indices, double u, int64_t measured, double measured_prob,
+                   int64_t *m, double *p) {
+    int rc = check_qubits(s, k, indices, "measured qubits");
+    if (rc) return rc;
+    rc = resolve_outcome(s, k, indices, u, measured, measured_prob, m, p);
+    if (rc) return rc;
+    rc = settle(s);
+    if (rc) return rc;
+    const double mult = std::sqrt(1.0 / *p);
+    uint64_t mask = 0;
+    const uint64_t target = measured_bits(s->n, k, indices, (uint64_t)*m, &mask);
+    const uint32_t nb = std::min<uint32_t>(blocks_for(s->len, 256 * 4), 1u << 20);
+    if (s->dtype == QIPGPU_C128)
+        hipLaunchKernelGGL(k_collapse<double>, dim3(nb), dim3(256), 0, s->stream, (cx<double> *)s->amps, s->len, mask, target, mult);
+    else
+        hipLaunchKernelGGL(k_collapse<float>, dim3(nb), dim3(256), 0, s->stream, (cx<float> *)s->amps, s->len, mask, target, mult);
+    QIP_LAUNCHED();
+    QIP_HIP(hipStreamSynchronize(s->stream));
+    return QIPGPU_OK;
+}
+
+int qipgpu_reduce_measure(qipgpu_state *s, int32_t k, const int32_t *indices, double u, int64_t measured,
+                          double measured_prob, int64_t *m, double *p) {
+    int rc = check_qubits(s, k, indices, "measured qubits");
+    if (rc) return rc;
+    rc = resolve_outcome(s, k, indices, u, measured, measured_prob, m, p);
+    if (rc) return rc;
+    rc = settle(s);
+    if (rc) return rc;
+    const double mult = std::sqrt(1.0 / *p);
+    uint64_t mask = 0;
+    const uint64_t mbits = measured_bits(s->n, k, indices, (uint64_t)*m, &mask);
+    std::vector<int> mpos;
+    for (int j = 0; j < k; ++j) mpos.push_back(s->n - 1 - indices[j]);
+    PosList mpl;
+    fill_poslist(mpl, mpos);
+    void *out = nullptr;
+    if (hipMalloc(&out, s->len * s->amp_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(QIPGPU_ENOMEM, "reduce_measure: cannot allocate the output arena");
+    }
+    const uint64_t lowmask = (1ull << (s->n - k)) - 1ull;
+    const uint32_t nb = std::min<uint32_t>(blocks_for(s->len, 256 * 4), 1u << 20);
+    if (s->dtype == QIPGPU_C128)
+        hipLaunchKernelGGL(k_reduce_tiled<double>, dim3(nb), dim3(256), 0, s->stream, (const cx<double> *)s->amps,
+                           (cx<double> *)out, s->len, mbits, mpl, lowmask, mult);
+    else
+        hipLaunchKernelGGL(k_reduce_tiled<float>, dim3(nb), dim3(256), 0, s->stream, (const cx<float> *)s->amps,
+                           (cx<float> *)out, s->len, mbits, mpl, lowmask, mult);
+    QIP_LAUNCHED();
+    QIP_HIP(hipStreamSynchronize(s->stream));
+    QIP_HIP(hipFree(s->amps));
+    s->amps = out;
+    return QIPGPU_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------
+// comparison helpers: <a|b> and max |a - b| in FP64
+// ------------------------------------------------------------------------------------------------
+
+template <typename Ra, typename Rb>
+__global__ __launch_bounds__(256) void k_overlap_partial(const cx<Ra> *__restrict__ a, const cx<Rb> *__restrict__ b, uint64_t len,
+                                                         double *partial) {
+    __shared__ double lds[4];
+    const uint64_t per = (len + gridDim.x - 1) / gridDim.x;
+    const uint64_t lo = (uint64_t)blockIdx.x * per, hi = min(len, lo + per);
+    double re = 0, im = 0, na = 0, nb = 0;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += 256) {
+        const cx<Ra> x = a[i];
+        const cx<Rb> y = b[i];
+        const double xr = x.re, xi = x.im, yr = y.re, yi = y.im;
+        re += xr * yr + xi * yi;
+        im += xr * yi - xi * yr;
+        na += xr * xr + xi * xi;
+        nb += yr * yr + yi * yi;
+    }
+    re = block_sum(re, lds);
+    im = block_sum(im, lds);
+    na = block_sum(na, lds);
+    nb = block_sum(nb, lds);
+    if (threadIdx.x == 0) {
+        partial[4 * blockIdx.x + 0] = re;
+        partial[4 * blockIdx.x + 1] = im;
+        partial[4 * blockIdx.x + 2] = na;
+        partial[4 * blockIdx.x + 3] = nb;
+    }
+}
+
+template <typename Ra, typename Rb>
+__global__ __launch_bounds__(256) void k_maxdiff_partial(const cx<Ra> *__restrict__ a, const cx<Rb> *__restrict__ b, uint64_t len,
+                                                         double *partial) {
+    __shared__ double lds[256];
+    const uint64_t per = (len + gridDim.x - 1) / gridDim.x;
+    const uint64_t lo = (uint64_t)blockIdx.x * per, hi = min(len, lo + per);
+    double m = 0;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += 256) {
+        const cx<Ra> x = a[i];
+        const cx<Rb> y = b[i];
+        const double dr = (double)x.re - (double)y.re, di = (double)x.im - (double)y.im;
+        m = fmax(m, sqrt(dr * dr + di * di));
+    }
+    lds[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = lds[0];
+}
+
+namespace {
+
+template <typename Ra, typename Rb>
+static void launch_overlap(qipgpu_state *a, qipgpu_state *b, uint32_t nb, double *partial) {
+    hipLaunchKernelGGL((k_overlap_partial<Ra, Rb>), dim3(nb), dim3(256), 0, a->stream, (const cx<Ra> *)a->amps,
+                       (const cx<Rb> *)b->amps, a->len, partial);
+}
+
+template <typename Ra, typename Rb>
+static void launch_maxdiff(qipgpu_state *a, qipgpu_state *b, uint32_t nb, double *partial) {
+    hipLaunchKernelGGL((k_maxdiff_partial<Ra, Rb>), dim3(nb), dim3(256), 0, a->stream, (const cx<Ra> *)a->amps,
+                       (const cx<Rb> *)b->amps, a->len, partial);
+}
+
+}  // namespace
+
+extern "C" {
+
+int qipgpu_overlap(qipgpu_state *a, qipgpu_state *b, double *re, double *im, double *norm_a, double *norm_b) {
+    if (a->n != b->n) return set_error(QIPGPU_EINVAL, "overlap of states with %d and %d qubits", a->n, b->n);
+    int rc = settle(a);
+    if (rc) return rc;
+    rc = settle(b);
+    if (rc) return rc;
+    QIP_HIP(hipStreamSynchronize(b->stream));
+    const uint32_t nb = (uint32_t)std::min<uint64_t>(kNormBlocks, std::max<uint64_t>(1, a->len / 256));
+    rc = ensure_scratch(a, sizeof(double) * 4 * nb);
+    if (rc) return rc;
+    double *partial = static_cast<double *>(a->scratch);
+    const bool a128 = a->dtype == QIPGPU_C128, b128 = b->dtype == QIPGPU_C128;
+    if (a128 && b128) launch_overlap<double, double>(a, b, nb, partial);
+    else if (a128) launch_overlap<double, float>(a, b, nb, partial);
+    else if (b128) launch_overlap<float, double>(a, b, nb, partial);
+    else launch_overlap<float, float>(a, b, nb, partial);
+    QIP_LAUNCHED();
+    std::vector<double> h(4 * nb);
+    QIP_HIP(hipMemcpyAsync(h.data(), partial, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, a->stream));
+    QIP_HIP(hipStreamSynchronize(a->stream));
+    double r[4] = {0, 0, 0, 0};
+    for (uint32_t i = 0; i < nb; ++i)
+        for (int c = 0; c < 4; ++c) r[c] += h[4 * i + c];
+    *re = r[0];
+    *im = r[1];
+    *norm_a = r[2];
+    *norm_b = r[3];
+    return QIPGPU_OK;
+}
+
+int qipgpu_max_abs_diff(qipgpu_state *a, qipgpu_state *b, double *out) {
+    if (a->n != b->n) return set_error(QIPGPU_EINVAL, "comparison of states with %d and %d qubits", a->n, b->n);
+    int rc = settle(a);
+    if (rc) return rc;
+    rc = settle(b);
+    if (rc) return rc;
+    QIP_HIP(hipStreamSynchronize(b->stream));
+    const uint32_t nb = (uint32_t)std::min<uint64_t>(kNormBlocks, std::max<uint64_t>(1, a->len / 256));
+    rc = ensure_scratch(a, sizeof(double) * nb);
+    if (rc) return rc;
+    double *partial = static_cast<double *>(a->scratch);
+    const bool a128 = a->dtype == QIPGPU_C128, b128 = b->dtype == QIPGPU_C128;
+    if (a128 && b128) launch_maxdiff<double, double>(a, b, nb, partial);
+    else if (a128) launch_maxdiff<double, float>(a, b, nb, partial);
+    else if (b128) launch_maxdiff<float, double>(a, b, nb, partial);
+    else launch_maxdiff<float, float>(a, b, nb, partial);
+    QIP_LAUNCHED();
+    std::vector<double> h(nb);
+    QIP_HIP(hipMemcpyAsync(h.data(), partial, sizeof(double) * nb, hipMemcpyDeviceToHost, a->stream));
+    QIP_HIP(hipStreamSynchronize(a->stream));
+    double m = 0;
+    for (uint32_t i = 0; i < nb; ++i) m = std::max(m, h[i]);
+    *out = m;
+    return QIPGPU_OK;
+}
+
+}  // extern "C"

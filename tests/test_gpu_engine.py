@@ -1,0 +1,284 @@
+"""GPU parity of the plain engine against the CPU oracle and closed forms (marked gpu).
+
+Bars: complex128 amplitudes within 1e-12 of the oracle (the oracle and the engine order their
+floating-point sums differently, so agreement is to rounding, not to the bit); complex64 within
+1e-5; func_apply permutations, measured outcomes and product-state initialisation bit-exact.
+"""
+import random
+
+import numpy
+import pytest
+
+from oracle import oracle
+from oracle.oracle_backend import OracleBackend
+from qip_amd import CMat, GpuBackend, SwapMat, infidelity, max_abs_diff
+from qip_amd.planner import plan
+
+pytestmark = pytest.mark.gpu
+RNG = numpy.random.default_rng(4242)
+TOL = {numpy.complex128: 1e-12, numpy.complex64: 2e-6}
+
+
+def rand_state(n):
+    v = RNG.standard_normal(2 ** n) + 1j * RNG.standard_normal(2 ** n)
+    return v / numpy.linalg.norm(v)
+
+
+def rand_u(d):
+    z = (RNG.standard_normal((d, d)) + 1j * RNG.standard_normal((d, d))) / numpy.sqrt(2)
+    q, r = numpy.linalg.qr(z)
+    return q * (numpy.diagonal(r) / numpy.abs(numpy.diagonal(r)))
+
+
+def gpu_from(psi, n, dtype=numpy.complex128):
+    return GpuBackend.make_state(n, [list(range(n))], [psi], statetype=dtype)
+
+
+def oracle_apply(mats, psi, n):
+    out = numpy.zeros(2 ** n, dtype=numpy.complex128)
+    oracle.kronselect_dot(mats, psi, n, out)
+    return out
+
+
+X = numpy.array([[0, 1], [1, 0]], dtype=complex)
+
+
+def random_descriptor(n):
+    qs = [int(q) for q in RNG.permutation(n)]
+    kind = int(RNG.integers(0, 7))
+    if kind == 0:
+        return {qs[0]: rand_u(2)}
+    if kind == 1:
+        return {(qs[0], qs[1]): rand_u(4)}
+    if kind == 2:
+        return {(qs[0], qs[1]): CMat(rand_u(2)), qs[2]: rand_u(2)}
+    if kind == 3:
+        return {(qs[0], qs[1], qs[2]): CMat(CMat(X))}
+    if kind == 4:
+        w = int(RNG.integers(1, min(3, n // 2) + 1))
+        return {tuple(qs[:2 * w]): SwapMat(w)} if RNG.integers(0, 2) else {tuple(qs[:2 * w + 1]): CMat(SwapMat(w))}
+    if kind == 5:
+        return {(qs[0], qs[1]): CMat(X), (qs[0], qs[2]): CMat(X)}
+    return {(qs[0], qs[1], qs[2]): rand_u(8)}
+
+
+@pytest.mark.parametrize("dtype", [numpy.complex128, numpy.complex64])
+@pytest.mark.parametrize("trial", range(10))
+def test_kronselect_random(trial, dtype):
+    n = int(RNG.integers(5, 15))
+    psi = rand_state(n)
+    be = gpu_from(psi, n, dtype)
+    ref = psi.copy()
+    for _ in range(6):
+        mats = random_descriptor(n)
+        be.kronselect_dot(mats)
+        ref = oracle_apply(mats, ref, n)
+    got = be.get_state()
+    assert got.dtype == dtype
+    assert numpy.max(numpy.abs(got - ref)) < TOL[dtype] * 10
+    be.close()
+
+
+@pytest.mark.parametrize("k", [3, 4, 5, 6])
+def test_dense_wide(k):
+    n = 12
+    psi = rand_state(n)
+    qs = [int(q) for q in RNG.permutation(n)[:k]]
+    U = rand_u(2 ** k)
+    be = gpu_from(psi, n)
+    be.kronselect_dot({tuple(qs): U})
+    ref = oracle_apply({tuple(qs): U}, psi, n)
+    assert numpy.max(numpy.abs(be.get_state() - ref)) < 1e-12
+
+
+def test_structure_detection():
+    """Diagonal phases, controlled phases and identity blocks take the diagonal / control paths;
+    results must not change."""
+    n = 9
+    psi = rand_state(n)
+    be = gpu_from(psi, n)
+    ref = psi.copy()
+    cases = [
+        {(2, 5): numpy.diag([1, 1, 1, numpy.exp(0.3j)])},                    # controlled phase
+        {(0, 1, 2): numpy.diag(numpy.exp(1j * RNG.uniform(0, 6, 8)))},       # general diagonal
+        {4: numpy.eye(2)},                                                   # identity
+        {(3, 7): numpy.kron(numpy.eye(2), rand_u(2))},                       # identity factor
+        {(8, 6): CMat(numpy.diag([1.0, -1.0]))},                             # CZ
+    ]
+    for mats in cases:
+        be.kronselect_dot(mats)
+        ref = oracle_apply(mats, ref, n)
+    assert numpy.max(numpy.abs(be.get_state() - ref)) < 1e-13
+
+
+def test_init_product_bit_exact():
+    n = 10
+    groups = [[0, 3], [5], [9, 1, 2]]
+    feeds = [rand_state(2), rand_state(1) * 3, rand_state(3)]
+    be = GpuBackend.make_state(n, groups, feeds)
+    assert numpy.array_equal(be.get_state(), oracle.make_state_vector(n, groups, feeds))
+    be0 = GpuBackend.make_state(4, [], [])
+    assert numpy.array_equal(be0.get_state(), numpy.eye(16)[0])
+
+
+def test_ranges():
+    n = 8
+    psi = rand_state(n)
+    be = gpu_from(psi, n)
+    assert numpy.array_equal(be.get_relative_range(5, 40), psi[5:40])
+    data = rand_state(4)
+    be.overwrite_relative_range(16, 32, data)
+    psi[16:32] = data
+    be.addto_relative_range(100, 116, data)
+    psi[100:116] += data
+    assert numpy.array_equal(be.get_state(), psi)
+    assert be.get_state_size() == 256
+
+
+@pytest.mark.parametrize("trial", range(6))
+def test_func_apply_bit_exact(trial):
+    n = int(RNG.integers(4, 14))
+    perm = [int(q) for q in RNG.permutation(n)]
+    n1 = int(RNG.integers(1, n))
+    n2 = int(RNG.integers(1, n - n1 + 1))
+    reg1, reg2 = perm[:n1], perm[n1:n1 + n2]
+    table = RNG.integers(-1000, 1000, size=2 ** n1).astype(numpy.int32)
+    psi = rand_state(n)
+    be = gpu_from(psi, n)
+    be.func_apply(numpy.array(reg1, dtype=numpy.int32), numpy.array(reg2, dtype=numpy.int32), table)
+    ref = numpy.zeros_like(psi)
+    oracle.func_apply(reg1, reg2, None, psi, n, ref, faithful=False, table=table)
+    assert numpy.array_equal(be.get_state(), ref)
+
+
+def test_func_apply_callable_and_cache():
+    n = 6
+    psi = rand_state(n)
+    be = gpu_from(psi, n)
+    f = lambda x: (3 * x + 1) % 8  # noqa: E731
+    be.func_apply([0, 1, 2], [3, 4, 5], f)
+    be.func_apply([0, 1, 2], [3, 4, 5], f)      # second call uses the cached device table
+    ref = numpy.zeros_like(psi)
+    oracle.func_apply([0, 1, 2], [3, 4, 5], f, psi, n, ref)
+    ref2 = numpy.zeros_like(psi)
+    oracle.func_apply([0, 1, 2], [3, 4, 5], f, ref, n, ref2)
+    assert numpy.array_equal(be.get_state(), ref2)
+
+
+@pytest.mark.parametrize("trial", range(6))
+def test_probabilities_and_marginals(trial):
+    n = int(RNG.integers(3, 17))
+    k = int(RNG.integers(1, n + 1))
+    idx = [int(q) for q in RNG.permutation(n)[:k]]
+    psi = rand_state(n)
+    be = gpu_from(psi, n)
+    got = be.measure_probabilities(idx)
+    ref = oracle.measure_probabilities(idx, n, psi)
+    assert numpy.max(numpy.abs(got - ref)) < 1e-14
+    assert be.total_prob() == pytest.approx(1.0, abs=1e-13)
+    ms, ps = be.measure_probabilities(idx, top_k=3)
+    rms, rps = oracle.measure_top_probabilities(idx, n, 3, psi)
+    assert ms == rms and numpy.allclose(ps, rps, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("trial", range(8))
+def test_measure_bit_exact_outcomes(trial):
+    n = int(RNG.integers(3, 15))
+    k = int(RNG.integers(1, n + 1))
+    idx = [int(q) for q in RNG.permutation(n)[:k]]
+    psi = rand_state(n)
+    seed = int(RNG.integers(0, 2 ** 31))
+    be = gpu_from(psi, n)
+    ob = OracleBackend.make_state(n, [list(range(n))], [psi])
+    random.seed(seed)
+    m_g, p_g = be.measure(idx)
+    random.seed(seed)
+    m_o, p_o = ob.measure(idx)
+    assert m_g == m_o
+    assert p_g == pytest.approx(p_o, rel=1e-12)
+    assert numpy.max(numpy.abs(be.get_state() - ob.get_state())) < 1e-12
+    random.seed(seed + 1)
+    s_g = be.soft_measure(idx)
+    random.seed(seed + 1)
+    s_o = ob.soft_measure(idx)
+    assert s_g[0] == s_o[0] and s_g[1] == pytest.approx(s_o[1], rel=1e-12)
+
+
+def test_measure_forced_and_errors():
+    n = 6
+    psi = rand_state(n)
+    be = gpu_from(psi, n)
+    with pytest.raises(ValueError, match="Measured value must be less"):
+        be.measure([0, 1], measured=4)
+    with pytest.raises(ValueError, match="measured_prob must be"):
+        be.measure([0], measured=0, measured_prob=0.0)
+    m, p = be.measure([2], measured=1, measured_prob=0.5)
+    assert (m, p) == (1, 0.5)
+    z = GpuBackend.make_state(2, [], [])
+    with pytest.raises(ZeroDivisionError):
+        z.measure([0], measured=1)
+
+
+def test_reduce_measure_tiled():
+    n = 7
+    psi = rand_state(n)
+    idx = [5, 2]
+    be = gpu_from(psi, n)
+    ob = OracleBackend.make_state(n, [list(range(n))], [psi])
+    random.seed(11)
+    r_g = be.reduce_measure(idx)
+    random.seed(11)
+    r_o = ob.reduce_measure(idx)
+    assert r_g[0] == r_o[0]
+    assert numpy.max(numpy.abs(be.get_state() - ob.get_state())) < 1e-12
+
+
+def test_bad_arguments_raise():
+    be = gpu_from(rand_state(4), 4)
+    with pytest.raises(ValueError):
+        be.kronselect_dot({7: numpy.eye(2)})
+    with pytest.raises(ValueError, match="Input vector size plus offset"):
+        be.kronselect_dot({0: numpy.eye(2)}, input_offset=3)
+    with pytest.raises(Exception, match="Shape of square submatrix"):
+        be.kronselect_dot({(0, 1): numpy.eye(2)})
+    with pytest.raises(ValueError):
+        be.func_apply([0, 0], [1], lambda x: x)
+
+
+@pytest.mark.parametrize("n", [24, 26])
+def test_large_state_properties(n):
+    """Size-independent properties at full-scale widths: U then U^dagger restores the state,
+    norms are preserved, and complex64 tracks complex128 to single precision."""
+    rng = numpy.random.default_rng(n)
+    be = GpuBackend.make_state(n, [], [])
+    be.kronselect_dot({q: H_ for q, H_ in zip(range(n), [numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2)] * n)})
+    be64 = GpuBackend.make_state(n, [], [], statetype=numpy.complex64)
+    be64.kronselect_dot({q: numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2) for q in range(n)})
+    start = be.get_relative_range(0, 1024).copy()
+    gates = []
+    for _ in range(12):
+        a, b = [int(q) for q in rng.permutation(n)[:2]]
+        U = rand_u(4) if rng.integers(0, 2) else numpy.kron(rand_u(2), numpy.eye(2))
+        gates.append(((a, b), U))
+    for g, U in gates:
+        be.kronselect_dot({g: U})
+        be64.kronselect_dot({g: U})
+    assert be.total_prob() == pytest.approx(1.0, abs=1e-12)
+    assert infidelity(be, be64) < 1e-10
+    assert max_abs_diff(be, be64) < 1e-6
+    for g, U in reversed(gates):
+        be.kronselect_dot({g: U.conj().T})
+    assert numpy.max(numpy.abs(be.get_relative_range(0, 1024) - start)) < 1e-13
+    probs = be.measure_probabilities([0, n - 1, 3])
+    assert numpy.allclose(probs, 1 / 8, atol=1e-12)
+
+
+def test_plan_controls_reach_engine():
+    """A Toffoli descriptor becomes one controlled call; the engine result matches the oracle."""
+    calls = plan({(0, 1, 2): CMat(CMat(X))})
+    assert len(calls) == 1 and calls[0].controls == [0, 1]
+    n = 3
+    psi = rand_state(n)
+    be = gpu_from(psi, n)
+    be.kronselect_dot({(0, 1, 2): CMat(CMat(X))})
+    assert numpy.max(numpy.abs(be.get_state() - oracle_apply({(0, 1, 2): CMat(CMat(X))}, psi, n))) < 1e-15
