@@ -1,0 +1,250 @@
+#!/usr/bin/env python3
+"""Benchmark: 30-qubit gates/s and HBM GB/s, fused complex64 vs plain complex128, with the
+infidelity of the fused result against the complex128 one.
+
+Workload (one "step"): the 30-qubit depth-40 layered random circuit (Haar single-qubit unitary
+on every qubit, then a CNOT / CZ brickwork; 1780 gates, seed fixed) applied to |0...0> by the
+precision-form FusedBackend: reset the complex64 state, record the 1780 descriptors, fuse them
+into <= 6-qubit blocks and apply the blocks with the split-BF16 MFMA kernel, apply the final
+FP64 renormalisation, synchronise. `value` = 1780 * steps / timed seconds.
+
+Beside it, in the same run:
+  * the plain complex128 GpuBackend (the drop-in) on the same circuit, once, with its gates/s,
+    algorithmic GB/s and the HBM roofline of its dominant kernel (single-qubit dense sweep);
+  * 1 - F and max |amplitude difference| of the fused result against the complex128 result;
+  * the HBM roofline of the fused kernel from HIP events on the state's stream;
+  * cpu_baseline: the oracle's restatement of the reference's cdot_loop (oracle/liboracle_fast.so,
+    the reference's -O3 -ffast-math -fopenmp flags, all visible cores) on a time-bounded sample of
+    the circuit's own gates at full size, extrapolated to the circuit's gate mix.
+
+--no-cpu-baseline leaves out the CPU leg (for profiler runs); it does not change the timed work.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, published
+
+
+class HipEvents:
+    """hipEvent timing on an arbitrary stream, through libamdhip64."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def create(self):
+        ev = ctypes.c_void_p()
+        if self.hip.hipEventCreate(ctypes.byref(ev)) != 0:
+            raise RuntimeError("hipEventCreate failed")
+        return ev
+
+    def record(self, ev, stream: int):
+        if self.hip.hipEventRecord(ev, ctypes.c_void_p(stream)) != 0:
+            raise RuntimeError("hipEventRecord failed")
+
+    def elapsed_ms(self, a, b) -> float:
+        self.hip.hipEventSynchronize(b)
+        ms = ctypes.c_float()
+        if self.hip.hipEventElapsedTime(ctypes.byref(ms), a, b) != 0:
+            raise RuntimeError("hipEventElapsedTime failed")
+        return float(ms.value)
+
+
+def run_fused_step(be, ops, ev, t0, t1):
+    """One step: reset to |0..0>, record the circuit, fuse + apply, finish. Events bracket the
+    block launches on the state's stream."""
+    be.reinit()
+    for d in ops:
+        be.kronselect_dot(d)
+    blocks = be.plan_blocks()
+    ev.record(t0, be.stream)
+    nblocks = be.launch_blocks(blocks)
+    be.fused_finish()
+    ev.record(t1, be.stream)
+    be.synchronize()
+    return nblocks
+
+
+def plain_dominant_roofline(be, n, ev):
+    """Average duration of the plain single-qubit dense sweep (the plain path's dominant kernel:
+    1200 of 1780 gates) over one gate per qubit, from HIP events on the state's stream."""
+    import numpy
+    from qip_amd.workloads import random_product_unitaries
+    ops = random_product_unitaries(n, seed=77)
+    be.synchronize()
+    a, b = ev.create(), ev.create()
+    ev.record(a, be.stream)
+    for d in ops:
+        be.kronselect_dot(d)
+    ev.record(b, be.stream)
+    ms = ev.elapsed_ms(a, b) / len(ops)
+    bytes_per = 2 * 16 * 2 ** n
+    return {"kernel": "k_dense<double,1,4> (single-qubit dense, complex128)", "avg_ms": ms,
+            "bytes_per_launch": bytes_per, "achieved_gbs": bytes_per / (ms * 1e-3) / 1e9}
+
+
+def cpu_baseline(ops, n, seconds: float):
+    """Oracle cdot restatement (reference flags, all visible cores) on the circuit's first gates at
+    full size until `seconds` of work, alternating single- and two-qubit gates; extrapolated to
+    the whole circuit's mix."""
+    import numpy
+    from oracle import oracle
+    from qip_amd.workloads import gate_counts
+    cores = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    vec = numpy.zeros(2 ** n, dtype=numpy.complex128)
+    out = numpy.zeros(2 ** n, dtype=numpy.complex128)
+    vec.fill(0)          # touch every page before timing
+    out.fill(0)
+    vec[0] = 1.0
+    one = [d for d in ops if isinstance(next(iter(d)), int)]
+    two = [d for d in ops if not isinstance(next(iter(d)), int)]
+    t_one, t_two = [], []
+    start = time.perf_counter()
+    i = 0
+    while (time.perf_counter() - start < seconds or not t_one or not t_two) and i < 64:
+        d = one[i // 2] if i % 2 == 0 else two[i // 2]
+        t = time.perf_counter()
+        oracle.kronselect_dot(d, vec, n, out, fast=True, nthreads=cores)
+        (t_one if i % 2 == 0 else t_two).append(time.perf_counter() - t)
+        vec, out = out, vec
+        i += 1
+    n1, n2 = gate_counts(ops)
+    circuit_s = n1 * (sum(t_one) / len(t_one)) + n2 * (sum(t_two) / len(t_two))
+    return {
+        "value": (n1 + n2) / circuit_s, "unit": "gates/s", "cores": cores, "kind": "port",
+        "sample": "{} single-qubit + {} two-qubit gates of this {}-qubit circuit at full size on the oracle's "
+                  "cdot_loop restatement (-O3 -ffast-math -fopenmp), {:.1f} s of work; rate extrapolated to "
+                  "the circuit's {} + {} gate mix".format(len(t_one), len(t_two), n, sum(t_one) + sum(t_two), n1, n2),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=30)
+    ap.add_argument("--depth", type=int, default=40)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="leave out the plain complex128 comparison run")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the engine holds the whole state on a single MI355X; "
+              "a multi-GPU run is not part of this build (--gpus {} requested)".format(args.gpus), file=sys.stderr)
+        sys.exit(2)
+
+    import numpy
+    from qip_amd import GpuBackend, infidelity, max_abs_diff
+    from qip_amd.fused import FusedBackend
+    from qip_amd.workloads import algorithmic_bytes_plain, gate_counts, layered_random_circuit
+
+    n, depth = args.n, args.depth
+    ops = layered_random_circuit(n, depth, args.seed)
+    n1, n2 = gate_counts(ops)
+    ngates = len(ops)
+    ev = HipEvents()
+    log = lambda msg: print("[bench] " + msg, file=sys.stderr, flush=True)  # noqa: E731
+
+    # ---- headline: fused complex64 ------------------------------------------------------------
+    fz = FusedBackend.make_state(n, [], [])
+    t0, t1 = ev.create(), ev.create()
+    for w in range(args.warmup):
+        nb = run_fused_step(fz, ops, ev, t0, t1)
+        log("warmup {} done: {} fused blocks".format(w, nb))
+    fz.synchronize()
+    gpu_ms = []
+    start = time.perf_counter()
+    for s in range(args.steps):
+        nblocks = run_fused_step(fz, ops, ev, t0, t1)
+        gpu_ms.append(ev.elapsed_ms(t0, t1))
+    fz.synchronize()
+    elapsed = time.perf_counter() - start
+    log("fused: {} steps in {:.3f} s ({} blocks/step)".format(args.steps, elapsed, nblocks))
+    ms_per_step = elapsed * 1e3 / args.steps
+    fused_gates_s = ngates * args.steps / elapsed
+    block_bytes = 2 * 8 * 2 ** n
+    # per block launch: k_fused6 plus its one-block norm kernel (the final scaling sweep is
+    # counted as one more pass of the same bytes)
+    avg_block_ms = (sum(gpu_ms) / len(gpu_ms)) / (nblocks + 1)
+    fused_gbs = block_bytes / (avg_block_ms * 1e-3) / 1e9
+
+    result = {
+        "metric": "30-qubit gates/s + HBM GB/s: fused c64 vs plain c128 at 1-F<=1e-8",
+        "value": fused_gates_s,
+        "unit": "gates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {
+            "workload": "{}-qubit depth-{} layered random circuit (Haar 1q on every qubit + CNOT/CZ brickwork), "
+                        "fused complex64 split-BF16 MFMA vs plain complex128".format(n, depth),
+            "n_qubits": n, "depth": depth, "gates": ngates, "one_qubit_gates": n1, "two_qubit_gates": n2,
+            "seed": args.seed, "fusion_width": FusedBackend.fusion_width, "fused_blocks": nblocks,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": "k_fused6 (six-qubit split-BF16 MFMA block, complex64)",
+            "achieved": fused_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": fused_gbs / HBM_PEAK_GBS,
+            "traffic": None, "bytes_per_launch": block_bytes, "avg_launch_ms": avg_block_ms,
+        },
+        "fused_c64": {"gates_per_s": fused_gates_s, "s_per_circuit": elapsed / args.steps,
+                      "gpu_ms_per_circuit": sum(gpu_ms) / len(gpu_ms), "blocks": nblocks,
+                      "algorithmic_gb_per_circuit": block_bytes * (nblocks + 1) / 1e9},
+    }
+
+    # ---- plain complex128 on the same circuit ----------------------------------------------------
+    if not args.no_plain:
+        pl = GpuBackend.make_state(n, [], [])
+        pl.synchronize()
+        start = time.perf_counter()
+        for d in ops:
+            pl.kronselect_dot(d)
+        pl.synchronize()
+        plain_s = time.perf_counter() - start
+        plain_bytes = algorithmic_bytes_plain(ops, n, 16)
+        log("plain c128: {:.3f} s".format(plain_s))
+        fz.flush()
+        inf = infidelity(pl, fz)
+        err = max_abs_diff(pl, fz)
+        dom = plain_dominant_roofline(pl, n, ev)
+        result["plain_c128"] = {
+            "gates_per_s": ngates / plain_s, "s_per_circuit": plain_s,
+            "algorithmic_gbs": plain_bytes / plain_s / 1e9, "hbm_frac": plain_bytes / plain_s / 1e9 / HBM_PEAK_GBS,
+            "dominant_kernel": dict(dom, frac=dom["achieved_gbs"] / HBM_PEAK_GBS),
+        }
+        result["speedup_fused_vs_plain"] = plain_s / (elapsed / args.steps)
+        result["infidelity_fused_vs_c128"] = inf
+        result["max_abs_err_fused_vs_c128"] = err
+        pl.close()
+    fz.close()
+
+    # ---- CPU baseline ---------------------------------------------------------------------------
+    if not args.no_cpu_baseline:
+        log("cpu baseline: sampling")
+        result["cpu_baseline"] = cpu_baseline(ops, n, args.cpu_seconds)
+
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -151,6 +151,9 @@ int qipgpu_overlap(qipgpu_state *a, qipgpu_state *b, double *re, double *im, dou
 int qipgpu_max_abs_diff(qipgpu_state *a, qipgpu_state *b, double *out);
 /* Block until all work queued on the state's stream has finished. */
 int qipgpu_synchronize(qipgpu_state *s);
+/* The hipStream_t every kernel on this state is launched on, for callers that time or order
+ * their own work against it (events, copies). */
+int qipgpu_get_stream(qipgpu_state *s, void **stream);
 
 #ifdef __cplusplus
 }

@@ -54,6 +54,7 @@ SIGNATURES = {
     "qipgpu_overlap": (ctypes.c_int, [_P, _P, _PD, _PD, _PD, _PD]),
     "qipgpu_max_abs_diff": (ctypes.c_int, [_P, _P, _PD]),
     "qipgpu_synchronize": (ctypes.c_int, [_P]),
+    "qipgpu_get_stream": (ctypes.c_int, [_P, _PP]),
 }
 
 _lib = None

@@ -242,9 +242,20 @@ class GpuBackend(StateType):
         _capi.check(self._lib.qipgpu_init_product(self._h, len(groups), _capi.ptr(glen), _capi.ptr(gidx),
                                                   _capi.ptr(flat)), "init")
 
+    def reinit(self, index_groups: Sequence[Sequence[int]] = (), feed_list: Sequence[InitialState] = ()) -> None:
+        """Rewrite the state in place as make_state would build it (no reallocation)."""
+        self._init(list(index_groups), list(feed_list))
+
     @property
     def handle(self):
         return self._h
+
+    @property
+    def stream(self) -> int:
+        """The hipStream_t (as an integer) the engine launches this state's kernels on."""
+        st = ctypes.c_void_p()
+        _capi.check(self._lib.qipgpu_get_stream(self._h, ctypes.byref(st)), "get_stream")
+        return st.value or 0
 
     @property
     def dtype(self):
@@ -412,8 +423,15 @@ class GpuBackend(StateType):
             pass
 
 
+def _settled(*states):
+    for st in states:
+        if hasattr(st, "flush"):
+            st.flush()
+
+
 def overlap(a: GpuBackend, b: GpuBackend) -> Tuple[complex, float, float]:
     """<a|b> in FP64 and the two squared norms, computed on the device."""
+    _settled(a, b)
     re, im, na, nb = (ctypes.c_double() for _ in range(4))
     _capi.check(_capi.load().qipgpu_overlap(a.handle, b.handle, ctypes.byref(re), ctypes.byref(im),
                                             ctypes.byref(na), ctypes.byref(nb)), "overlap")
@@ -427,6 +445,7 @@ def infidelity(a: GpuBackend, b: GpuBackend) -> float:
 
 
 def max_abs_diff(a: GpuBackend, b: GpuBackend) -> float:
+    _settled(a, b)
     out = ctypes.c_double()
     _capi.check(_capi.load().qipgpu_max_abs_diff(a.handle, b.handle, ctypes.byref(out)), "max_abs_diff")
     return float(out.value)

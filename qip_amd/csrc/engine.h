@@ -38,7 +38,7 @@ __device__ __forceinline__ uint64_t insert_zeros(uint64_t g, const PosList &pl) 
 }
 
 constexpr int kRingSlots = 8;
-constexpr size_t kRingSlotBytes = 64 * 64 * 16;  // one 64x64 complex128 matrix
+constexpr size_t kRingSlotBytes = 64 * 64 * 16 + 4096;  // a 64x64 complex128 matrix + offsets
 
 }  // namespace qipgpu
 

@@ -257,6 +257,11 @@ int qipgpu_synchronize(qipgpu_state *s) {
     return QIPGPU_OK;
 }
 
+int qipgpu_get_stream(qipgpu_state *s, void **stream) {
+    *stream = (void *)s->stream;
+    return QIPGPU_OK;
+}
+
 int qipgpu_init_product(qipgpu_state *s, int32_t n_groups, const int32_t *group_len,
                         const int32_t *group_idx, const double *feeds) {
     s->fused_pending = false;  // every amplitude is rewritten

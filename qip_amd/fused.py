@@ -1,0 +1,190 @@
+"""Precision form: complex64 state, gates fused into <= 6-qubit dense blocks on MFMA.
+
+``FusedBackend`` is a StateType (qip/backend.py:14-65) like GpuBackend, but kronselect_dot only
+records: each engine call of the plan (targets, controls, matrix) becomes a dense gate on its
+controls + targets. Recorded gates are fused greedily into blocks of at most ``fusion_width``
+qubits and the blocks are applied by the split-BF16 MFMA kernel (qipgpu_fused_apply) when the
+state is next read or changed any other way (func_apply, measurement, probabilities, get_state,
+ranges). Every fused block renormalises the previous block's output with its FP64 norm.
+
+Greedy fusion. Gates are taken in program order. For gate g on qubit set S let j be the last
+block that touches S. g may join block j or any later block (those do not touch S, so g commutes
+past them) whose qubit set stays within the width; the first that fits is taken, else g opens a
+new block at the end. A block's matrix is the ordered product of its gates embedded in the
+block's qubits. On the 30-qubit depth-40 brickwork circuit this gives 128 blocks for 1780 gates.
+"""
+from typing import List, Sequence, Tuple
+
+import numpy
+
+from . import planner
+from .backend import GpuBackend, check_windows
+
+FUSED_MIN_QUBITS = 12       # the MFMA kernel tiles 12 state bits
+
+
+class Block:
+    __slots__ = ("qubits", "gates")
+
+    def __init__(self):
+        self.qubits = set()
+        self.gates = []          # [(qubits tuple MSB-first, dense matrix)]
+
+    def matrix(self) -> Tuple[List[int], numpy.ndarray]:
+        """Ordered product of the block's gates on the block qubits (ascending, first = MSB)."""
+        qs = sorted(self.qubits)
+        k = len(qs)
+        pos = {q: i for i, q in enumerate(qs)}
+        M = numpy.eye(2 ** k, dtype=numpy.complex128).reshape([2] * k + [2 ** k])
+        for gq, G in self.gates:
+            L = len(gq)
+            axes = [pos[q] for q in gq]
+            Gt = G.reshape([2] * (2 * L))
+            M = numpy.tensordot(Gt, M, axes=(list(range(L, 2 * L)), axes))
+            M = numpy.moveaxis(M, list(range(L)), axes)
+        return qs, M.reshape(2 ** k, 2 ** k)
+
+
+def controlled_dense(call: planner.MatrixCall) -> Tuple[Tuple[int, ...], numpy.ndarray]:
+    """The call as a dense gate on controls + targets (controls first, each diag(I, .))."""
+    M = call.mat
+    for _ in call.controls:
+        d = M.shape[0]
+        C = numpy.eye(2 * d, dtype=numpy.complex128)
+        C[d:, d:] = M
+        M = C
+    return tuple(call.controls) + tuple(call.targets), M
+
+
+def fuse(gates: Sequence[Tuple[Tuple[int, ...], numpy.ndarray]], width: int) -> List[Block]:
+    blocks: List[Block] = []
+    for gq, G in gates:
+        S = set(gq)
+        j = -1
+        for i in range(len(blocks) - 1, -1, -1):
+            if blocks[i].qubits & S:
+                j = i
+                break
+        target = None
+        for i in range(max(j, 0), len(blocks)):
+            if len(blocks[i].qubits | S) <= width:
+                target = blocks[i]
+                break
+        if target is None:
+            target = Block()
+            blocks.append(target)
+        target.qubits |= S
+        target.gates.append((gq, G))
+    return blocks
+
+
+class FusedBackend(GpuBackend):
+    """complex64 state; kronselect_dot is deferred and fused (module docstring)."""
+
+    fusion_width = 6
+
+    def __init__(self, n, handle, dtype):
+        super().__init__(n, handle, dtype)
+        self._gates = []
+        self.blocks_applied = 0
+
+    @classmethod
+    def make_state(cls, n, index_groups, feed_list, inputstartindex=None, inputendindex=None,
+                   outputstartindex=None, outputendindex=None, statetype=numpy.complex64) -> "FusedBackend":
+        if numpy.dtype(statetype) != numpy.complex64:
+            statetype = numpy.complex64          # the precision form stores complex64
+        return super(FusedBackend, cls).make_state(n, index_groups, feed_list, inputstartindex, inputendindex,
+                                                   outputstartindex, outputendindex, statetype=statetype)
+
+    # -- recording ----------------------------------------------------------------------------
+    def kronselect_dot(self, mats, input_offset: int = 0, output_offset: int = 0) -> None:
+        check_windows(self.n, 2 ** self.n, input_offset, output_offset)
+        for call in planner.plan(mats):
+            gq, G = controlled_dense(call)
+            if len(gq) > self.fusion_width:
+                self.flush()
+                GpuBackend.apply_matrix(self, call.targets, call.mat, call.controls)
+            else:
+                self._gates.append((gq, G))
+
+    def apply_matrix(self, targets, mat, controls=()) -> None:
+        call = planner.MatrixCall(list(targets), list(controls), numpy.ascontiguousarray(mat, dtype=numpy.complex128))
+        gq, G = controlled_dense(call)
+        if len(gq) > self.fusion_width:
+            self.flush()
+            GpuBackend.apply_matrix(self, targets, mat, controls)
+        else:
+            self._gates.append((gq, G))
+
+    def reinit(self, index_groups=(), feed_list=()) -> None:
+        self._gates = []
+        super().reinit(index_groups, feed_list)
+
+    def pending_gates(self) -> int:
+        return len(self._gates)
+
+    def plan_blocks(self) -> List[Tuple[List[int], numpy.ndarray]]:
+        """Take the recorded gates and return their fused blocks as (qubits, matrix), in order."""
+        gates, self._gates = self._gates, []
+        return [b.matrix() for b in fuse(gates, self.fusion_width)]
+
+    def launch_blocks(self, blocks: Sequence[Tuple[List[int], numpy.ndarray]]) -> int:
+        """Apply planned blocks: MFMA kernel for n >= 12, the plain complex64 kernels below."""
+        for qs, M in blocks:
+            if self.n >= FUSED_MIN_QUBITS:
+                self.apply_fused(qs, M)
+            else:
+                GpuBackend.apply_matrix(self, qs, M)
+        self.blocks_applied += len(blocks)
+        return len(blocks)
+
+    def flush(self) -> int:
+        """Fuse and apply every recorded gate; returns the number of blocks launched."""
+        if not self._gates:
+            return 0
+        return self.launch_blocks(self.plan_blocks())
+
+    # -- everything else sees the flushed state -----------------------------------------------
+    def get_relative_range(self, start, end):
+        self.flush()
+        return super().get_relative_range(start, end)
+
+    def overwrite_relative_range(self, start, end, data):
+        self.flush()
+        super().overwrite_relative_range(start, end, data)
+
+    def addto_relative_range(self, start, end, data):
+        self.flush()
+        super().addto_relative_range(start, end, data)
+
+    def func_apply(self, reg1_indices, reg2_indices, func, input_offset=0, output_offset=0):
+        self.flush()
+        super().func_apply(reg1_indices, reg2_indices, func, input_offset, output_offset)
+
+    def total_prob(self):
+        self.flush()
+        return super().total_prob()
+
+    def measure(self, indices, measured=None, measured_prob=None, input_offset=0, output_offset=0):
+        self.flush()
+        return super().measure(indices, measured, measured_prob, input_offset, output_offset)
+
+    def reduce_measure(self, indices, measured=None, measured_prob=None, input_offset=0, output_offset=0):
+        self.flush()
+        return super().reduce_measure(indices, measured, measured_prob, input_offset, output_offset)
+
+    def soft_measure(self, indices, measured=None, input_offset=0):
+        self.flush()
+        return super().soft_measure(indices, measured, input_offset)
+
+    def measure_probabilities(self, indices, top_k=0):
+        self.flush()
+        return super().measure_probabilities(indices, top_k)
+
+    def marginals(self, indices):
+        self.flush()
+        return super().marginals(indices)
+
+    def synchronize(self):
+        self.flush()
+        super().synchronize()

@@ -1,0 +1,59 @@
+"""CPU tests of the precision form's host logic: greedy fusion keeps program semantics (the
+ordered product of block matrices equals the ordered product of the gates, both expanded to the
+full 2^n operator independently), respects the width, and gives the expected block counts."""
+import numpy
+import pytest
+
+from qip_amd import planner
+from qip_amd.fused import controlled_dense, fuse
+from qip_amd.workloads import algorithmic_bytes_plain, gate_counts, layered_random_circuit
+
+
+def full_operator(gq, G, n):
+    """2^n operator of gate G on qubits gq (first = MSB of G's index) by tensor contraction."""
+    dim = 2 ** n
+    L = len(gq)
+    eye = numpy.eye(dim, dtype=complex).reshape([2] * n + [dim])
+    out = numpy.tensordot(G.reshape([2] * (2 * L)), eye, axes=(list(range(L, 2 * L)), list(gq)))
+    return numpy.moveaxis(out, list(range(L)), list(gq)).reshape(dim, dim)
+
+
+def gates_of(ops):
+    gates = []
+    for d in ops:
+        for call in planner.plan(d):
+            gates.append(controlled_dense(call))
+    return gates
+
+
+@pytest.mark.parametrize("width", [2, 3, 4, 6])
+def test_fusion_preserves_operator(width):
+    n, depth = 7, 4
+    gates = gates_of(layered_random_circuit(n, depth, seed=5))
+    want = numpy.eye(2 ** n, dtype=complex)
+    for gq, G in gates:
+        want = full_operator(gq, G, n) @ want
+    blocks = fuse(gates, width)
+    got = numpy.eye(2 ** n, dtype=complex)
+    for b in blocks:
+        assert len(b.qubits) <= width
+        qs, M = b.matrix()
+        got = full_operator(qs, M, n) @ got
+    assert numpy.max(numpy.abs(got - want)) < 1e-12
+    assert sum(len(b.gates) for b in blocks) == len(gates)
+
+
+def test_block_counts_brickwork():
+    gates = gates_of(layered_random_circuit(30, 40, seed=1))
+    assert len(gates) == 1780
+    assert len(fuse(gates, 6)) == 128
+
+
+def test_circuit_shape():
+    ops = layered_random_circuit(30, 40, seed=2024)
+    assert gate_counts(ops) == (1200, 580)
+    assert algorithmic_bytes_plain(ops, 30, 16) == (1200 * 32 + 300 * 16 + 280 * 8) * 2 ** 30
+    ops2 = layered_random_circuit(28, 20, seed=1)
+    assert gate_counts(ops2) == (560, 270)
+    U = ops[0][0]
+    assert numpy.allclose(U.conj().T @ U, numpy.eye(2), atol=1e-15)

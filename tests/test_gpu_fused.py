@@ -93,3 +93,36 @@ def test_fused_rejects():
     b = GpuBackend.make_state(8, [], [], statetype=numpy.complex64)
     with pytest.raises(NotImplementedError):
         b.apply_fused([0], numpy.eye(2))          # n < 12
+
+
+def test_fused_backend_layered_circuit():
+    """The precision-form backend on a layered random circuit vs the plain complex128 backend."""
+    from qip_amd.fused import FusedBackend
+    from qip_amd.workloads import layered_random_circuit
+    n = 16
+    ops = layered_random_circuit(n, 12, seed=3)
+    pl = GpuBackend.make_state(n, [], [])
+    fz = FusedBackend.make_state(n, [], [])
+    for d in ops:
+        pl.kronselect_dot(d)
+        fz.kronselect_dot(d)
+    assert fz.pending_gates() == len(ops)
+    got = fz.get_state()                  # flushes
+    assert fz.blocks_applied < len(ops) // 4
+    assert infidelity(pl, fz) < 1e-8
+    assert numpy.max(numpy.abs(got - pl.get_state())) < 1e-5
+    assert abs(fz.total_prob() - 1.0) < 1e-6
+
+
+def test_fused_backend_graph_qft():
+    """A qip graph run through FusedBackend: the 14-qubit QFT equals ifft(norm='ortho')."""
+    from qip_amd import QFFT, Qubit, run
+    from qip_amd.fused import FusedBackend
+    n = 14
+    psi = rand_state(n)
+    q = Qubit(n=n)
+    o, _ = run(QFFT(q), feed={q: psi}, backend_constructor=FusedBackend.make_state)
+    assert o.dtype == numpy.complex64
+    expect = numpy.fft.ifft(psi, norm="ortho")
+    ov = numpy.vdot(expect, o)
+    assert 1 - abs(ov) ** 2 / numpy.vdot(o, o).real < 1e-8
