@@ -90,7 +90,7 @@ def plain_dominant_roofline(be, n, ev):
     ev.record(b, be.stream)
     ms = ev.elapsed_ms(a, b) / len(ops)
     bytes_per = 2 * 16 * 2 ** n
-    return {"kernel": "k_dense<double,1,4> (single-qubit dense, complex128)", "avg_ms": ms,
+    return {"kernel": "k_dense<double,1,1> (single-qubit dense, complex128)", "avg_ms": ms,
             "bytes_per_launch": bytes_per, "achieved_gbs": bytes_per / (ms * 1e-3) / 1e9}
 
 
@@ -130,6 +130,20 @@ def cpu_baseline(ops, n, seconds: float):
     }
 
 
+def copy_calibration(ev, nbytes: int) -> None:
+    hip = ev.hip
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), nbytes) or hip.hipMalloc(ctypes.byref(b), nbytes):
+        raise MemoryError("calibration buffers")
+    if hip.hipMemcpy(b, a, nbytes, 3):          # hipMemcpyDeviceToDevice
+        raise RuntimeError("calibration copy failed")
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +155,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="leave out the plain complex128 comparison run")
+    ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
+                    help="after the benchmark, one device-to-device copy of this many GiB (a kernel of known "
+                         "traffic for calibrating hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the engine holds the whole state on a single MI355X; "
@@ -242,6 +259,9 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline: sampling")
         result["cpu_baseline"] = cpu_baseline(ops, n, args.cpu_seconds)
+
+    if args.copy_calibration_gib > 0:
+        copy_calibration(ev, int(args.copy_calibration_gib * 2 ** 30))
 
     print(json.dumps(result), flush=True)
 

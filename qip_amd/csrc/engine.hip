@@ -588,7 +588,7 @@ static int launch_dense_reg(qipgpu_state *s, const std::vector<int> &tbits, uint
         a.U[i].re = (R)U[i].real();
         a.U[i].im = (R)U[i].imag();
     }
-    constexpr int ITEMS = K == 1 ? 4 : (K == 2 ? 2 : 1);
+    constexpr int ITEMS = 1;  // one base per thread measured fastest (large grids keep HBM busy)
     const uint32_t nb = blocks_for(a.nbase, 256 * ITEMS);
     hipLaunchKernelGGL((k_dense<R, K, ITEMS>), dim3(nb), dim3(256), 0, s->stream, a);
     QIP_LAUNCHED();
