@@ -144,6 +144,165 @@ def copy_calibration(ev, nbytes: int) -> None:
     hip.hipFree(b)
 
 
+def _line(metric, value, unit, steps, warmup, ms, dtype, config, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic", "config": config}
+    out.update(extra)
+    return out
+
+
+def bench_circuit28(args):
+    """28-qubit, 20-layer random circuit: plain complex128 vs plain complex64 (unfused), one sweep
+    per gate each; 1 - F of complex64 against complex128."""
+    import numpy
+    from qip_amd import GpuBackend, infidelity, max_abs_diff
+    from qip_amd.workloads import algorithmic_bytes_plain, layered_random_circuit
+    n, depth = 28, 20
+    ops = layered_random_circuit(n, depth, args.seed)
+    res = {}
+    states = {}
+    for name, st, amp in (("c128", numpy.complex128, 16), ("c64", numpy.complex64, 8)):
+        be = GpuBackend.make_state(n, [], [], statetype=st)
+        for d in ops[:45]:                     # warm-up: first layer
+            be.kronselect_dot(d)
+        be.reinit()
+        be.synchronize()
+        t = time.perf_counter()
+        for d in ops:
+            be.kronselect_dot(d)
+        be.synchronize()
+        dt = time.perf_counter() - t
+        gb = algorithmic_bytes_plain(ops, n, amp) / 1e9
+        res[name] = {"gates_per_s": len(ops) / dt, "s_per_circuit": dt, "algorithmic_gbs": gb / dt,
+                     "hbm_frac": gb / dt / HBM_PEAK_GBS}
+        states[name] = be
+    res["infidelity_c64_vs_c128"] = infidelity(states["c128"], states["c64"])
+    res["max_abs_err_c64_vs_c128"] = max_abs_diff(states["c128"], states["c64"])
+    cfg = {"workload": "28-qubit depth-20 layered random circuit, plain complex128 vs plain complex64 (one sweep "
+                       "per gate)", "n_qubits": n, "depth": depth, "gates": len(ops), "seed": args.seed}
+    out = _line("28-qubit gates/s, plain c128 vs unfused c64", res["c128"]["gates_per_s"], "gates/s", 1, 1,
+                res["c128"]["s_per_circuit"] * 1e3, "c128", cfg, plain_c128=res["c128"], plain_c64=res["c64"],
+                infidelity_c64_vs_c128=res["infidelity_c64_vs_c128"],
+                max_abs_err_c64_vs_c128=res["max_abs_err_c64_vs_c128"])
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(ops, n, args.cpu_seconds)
+    return out
+
+
+def bench_grover24(args):
+    """24-qubit iterative Grover (23 search qubits + ancilla, 2274 iterations): plain complex128
+    with H/X/multi-controlled-Z diffusion, and the fused complex64 backend with F diffusion; then
+    a seeded measurement of the register. Amplitude error against the closed form."""
+    import random
+    import numpy
+    from qip_amd import GpuBackend
+    from qip_amd.fused import FusedBackend
+    from qip_amd.workloads import (apply_ops, grover_closed_form, grover_initial, grover_iteration_ops,
+                                   grover_iterations)
+    ns = 23
+    x_star = int(numpy.random.default_rng(args.seed).integers(0, 2 ** ns))
+    t_iter = grover_iterations(ns)
+    expect = grover_closed_form(ns, x_star, t_iter)
+    groups, feeds = grover_initial(ns)
+    res = {}
+    for name, make, diff in (("plain_c128", GpuBackend.make_state, "mcz"),
+                             ("fused_c64", FusedBackend.make_state, "f")):
+        be = make(ns + 1, groups, feeds)
+        ops = grover_iteration_ops(ns, x_star, diff)
+        apply_ops(be, ops)                      # warm-up iteration (tables, kernels)
+        be.reinit(groups, feeds)
+        be.synchronize()
+        t = time.perf_counter()
+        for _ in range(t_iter):
+            apply_ops(be, ops)
+        be.synchronize()
+        dt = time.perf_counter() - t
+        err = float(numpy.max(numpy.abs(be.get_state() - expect)))
+        random.seed(args.seed)
+        m, p = be.measure(list(range(ns)))
+        res[name] = {"iterations_per_s": t_iter / dt, "s_total": dt, "diffusion": diff,
+                     "max_abs_err_vs_closed_form": err, "measured": m, "measured_prob": p,
+                     "found_marked": m == x_star}
+        be.close()
+    cfg = {"workload": "24-qubit iterative Grover: 23 search qubits + |-> ancilla, F(x==x*) oracle, {} iterations, "
+                       "seeded measurement".format(t_iter), "n_qubits": ns + 1, "iterations": t_iter,
+           "x_star": x_star, "seed": args.seed}
+    out = _line("24-qubit Grover iterations/s (plain c128, H/X/MCZ diffusion)", res["plain_c128"]["iterations_per_s"],
+                "iterations/s", 1, 1, res["plain_c128"]["s_total"] * 1e3, "c128", cfg, **res)
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_grover(ns, x_star, args.cpu_seconds)
+    return out
+
+
+def cpu_baseline_grover(ns, x_star, seconds):
+    """Oracle restatement of the reference on Grover iterations at full size with the operations
+    the reference can run (per-qubit H descriptors and F-based diffusion: one H descriptor on the
+    whole register would cost 2^(n+23) in cdot_loop); per-iteration rate."""
+    import numpy
+    from oracle import oracle
+    n = ns + 1
+    cores = len(os.sched_getaffinity(0))
+    from qip_amd.workloads import H, grover_initial
+    groups, feeds = grover_initial(ns)
+    vec = oracle.make_state_vector(n, groups, feeds)
+    out = numpy.zeros_like(vec)
+    xs = numpy.arange(2 ** ns)
+    t_oracle = (xs == x_star).astype(numpy.int32)
+    t_zero = (xs == 0).astype(numpy.int32)
+    search, anc = list(range(ns)), [ns]
+    done, start = 0, time.perf_counter()
+    while time.perf_counter() - start < seconds or done == 0:
+        for table in (t_oracle, None, t_zero, None):
+            if table is None:
+                for q in search:
+                    oracle.kronselect_dot({q: H}, vec, n, out, fast=True, nthreads=cores)
+                    vec, out = out, vec
+            else:
+                oracle.func_apply(search, anc, None, vec, n, out, table=table, fast=True, nthreads=cores)
+                vec, out = out, vec
+        done += 1
+    dt = time.perf_counter() - start
+    return {"value": done / dt, "unit": "iterations/s", "cores": cores, "kind": "port",
+            "sample": "{} Grover iteration(s) at full size (24 qubits) on the oracle restatement: func_apply oracle, "
+                      "23 per-qubit H descriptors, func_apply F(x==0), 23 per-qubit H, {:.1f} s".format(done, dt)}
+
+
+def bench_qft16(args):
+    """16-qubit QFT (qip/qfft.py circuit, 144 gates) through the graph frontend on a seeded random
+    state: plain complex128; error against ifft(norm='ortho') and the probabilities."""
+    import numpy
+    from qip_amd import GpuBackend, QFFT, Qubit, StochasticMeasure, run
+    n = 16
+    rng = numpy.random.default_rng(args.seed)
+    psi = rng.standard_normal(2 ** n) + 1j * rng.standard_normal(2 ** n)
+    psi /= numpy.linalg.norm(psi)
+    q = Qubit(n=n)
+    sm = StochasticMeasure(QFFT(q))
+    run(sm, feed={q: psi}, backend_constructor=GpuBackend.make_state)      # warm-up
+    reps = 20
+    t = time.perf_counter()
+    for _ in range(reps):
+        state, classic = run(sm, feed={q: psi}, backend_constructor=GpuBackend.make_state)
+    dt = (time.perf_counter() - t) / reps
+    expect = numpy.fft.ifft(psi, norm="ortho")
+    err = float(numpy.max(numpy.abs(state - expect)))
+    perm = [int(format(i, "016b")[::-1], 2) for i in range(2 ** n)]
+    perr = float(numpy.max(numpy.abs(classic[sm] - (numpy.abs(expect) ** 2)[perm])))
+    cfg = {"workload": "16-qubit QFT (qip/qfft.py: 16 H, 120 controlled phases, 8 swaps) + StochasticMeasure, graph "
+                       "frontend, seeded random complex128 state", "n_qubits": n, "gates": 144, "seed": args.seed}
+    out = _line("16-qubit QFT circuits/s (graph run incl. state upload and read-back)", 1.0 / dt, "circuits/s", reps,
+                1, dt * 1e3, "c128", cfg, max_abs_err_vs_ifft=err, max_abs_err_probabilities=perr)
+    if not args.no_cpu_baseline:
+        from oracle.oracle_backend import OracleBackend
+        t = time.perf_counter()
+        run(sm, feed={q: psi}, backend_constructor=OracleBackend.make_state)
+        cdt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": 1.0 / cdt, "unit": "circuits/s", "cores": len(os.sched_getaffinity(0)),
+                               "kind": "port", "sample": "one full run of the same graph on the oracle backend"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,11 +317,18 @@ def main():
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="after the benchmark, one device-to-device copy of this many GiB (a kernel of known "
                          "traffic for calibrating hardware byte counters)")
+    ap.add_argument("--workload", default="circuit30", choices=["circuit30", "circuit28", "grover24", "qft16"],
+                    help="circuit30 (default, the headline) or one of the other configurations, each printed as "
+                         "its own JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the engine holds the whole state on a single MI355X; "
               "a multi-GPU run is not part of this build (--gpus {} requested)".format(args.gpus), file=sys.stderr)
         sys.exit(2)
+    if args.workload != "circuit30":
+        result = {"circuit28": bench_circuit28, "grover24": bench_grover24, "qft16": bench_qft16}[args.workload](args)
+        print(json.dumps(result), flush=True)
+        return
 
     import numpy
     from qip_amd import GpuBackend, infidelity, max_abs_diff

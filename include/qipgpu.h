@@ -80,6 +80,14 @@ int qipgpu_add_range(qipgpu_state *s, int64_t start, int64_t end, const void *ho
 int qipgpu_apply_matrix(qipgpu_state *s, int32_t k, const int32_t *targets, const double *mat,
                         int32_t n_ctrl, const int32_t *controls);
 
+/* Product of single-qubit operators, mats[4j..4j+3] (row-major 2x2 complex128) on targets[j], for
+ * k <= QIPGPU_MAX_PRODUCT distinct qubits, in one sweep: the uncontrolled single-qubit factors of
+ * one kronselect_dot descriptor (e.g. H on a register, qip/operators.py:97-99), which cdot_loop
+ * also applies in one sweep. Computed in FP64 on a tile held in LDS, rounded once per amplitude;
+ * needs n >= 12. */
+#define QIPGPU_MAX_PRODUCT 12
+int qipgpu_apply_product(qipgpu_state *s, int32_t k, const int32_t *targets, const double *mats);
+
 /* Diagonal operator diag(d) on `targets`, conditioned on `controls` (k <= QIPGPU_MAX_DENSE). */
 int qipgpu_apply_diagonal(qipgpu_state *s, int32_t k, const int32_t *targets, const double *diag,
                           int32_t n_ctrl, const int32_t *controls);

@@ -200,7 +200,12 @@ def kronselect_dot(mats, vec: numpy.ndarray, n: int, outputarray: numpy.ndarray,
 # ---------------------------------------------------------------------------------------------
 
 def func_table(func: Callable[[int], int], n_reg1: int) -> numpy.ndarray:
-    """func_apply.pyx:66-69: f tabulated into C ints (OverflowError past int32, as Cython)."""
+    """func_apply.pyx:66-69: f tabulated into C ints (OverflowError past int32, as Cython). An
+    integer array is taken as the table itself."""
+    if isinstance(func, numpy.ndarray):
+        return numpy.ascontiguousarray(func, dtype=numpy.int32)
+    if isinstance(getattr(func, "values", None), numpy.ndarray):
+        return numpy.ascontiguousarray(func.values, dtype=numpy.int32)
     table = numpy.zeros(2 ** n_reg1, dtype=numpy.int32)
     for x in range(2 ** n_reg1):
         v = int(func(x))

@@ -11,7 +11,7 @@ build the same graphs the reference does and run them on it:
 Backends: GpuBackend (plain complex128 / complex64 drop-in), FusedBackend (complex64 state,
 gates fused into <= 6-qubit blocks applied with split-BF16 MFMA).
 """
-from .backend import GpuBackend, StateType, infidelity, max_abs_diff, overlap
+from .backend import FuncTable, GpuBackend, StateType, infidelity, max_abs_diff, overlap
 from .operators import (C, CMat, COp, F, FOp, H, HOp, MatrixOp, Not, NotOp, R, Rm, RmOp, ROp, Swap, SwapMat,
                         SwapOp, X, XOp, Y, YOp, Z, ZOp)
 from .pipeline import PipelineObject, get_deps, make_circuit_mat, run, run_graph
@@ -20,7 +20,7 @@ from .qip import Measure, OpConstructor, Q, Qubit, SplitQubit, StochasticMeasure
 from .qubit_util import QubitFuncWrapper, QubitOpWrapper, QubitWrapperContext
 
 __all__ = [
-    "GpuBackend", "StateType", "infidelity", "max_abs_diff", "overlap",
+    "FuncTable", "GpuBackend", "StateType", "infidelity", "max_abs_diff", "overlap",
     "C", "CMat", "COp", "F", "FOp", "H", "HOp", "MatrixOp", "Not", "NotOp", "R", "Rm", "RmOp", "ROp",
     "Swap", "SwapMat", "SwapOp", "X", "XOp", "Y", "YOp", "Z", "ZOp",
     "PipelineObject", "get_deps", "make_circuit_mat", "run", "run_graph", "QFFT",

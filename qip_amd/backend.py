@@ -144,6 +144,34 @@ class _FuncTable:
 _func_tables = weakref.WeakKeyDictionary()
 
 
+class FuncTable:
+    """A classical function given by its table of values, f(x) = values[x].
+
+    It is callable like any f, so every backend accepts it, and GpuBackend keeps its device copy
+    for the object's lifetime: pass the same FuncTable to every func_apply of a loop (a Grover
+    oracle, say) and the table crosses PCIe once."""
+
+    def __init__(self, values):
+        v = numpy.asarray(values)
+        if v.ndim != 1 or (v.size & (v.size - 1)) or v.size == 0:
+            raise ValueError("a function table needs 2**k entries")
+        vi = v.astype(numpy.int64)
+        if vi.min() < -2 ** 31 or vi.max() >= 2 ** 31:
+            raise OverflowError("value too large to convert to int")
+        self.values = numpy.ascontiguousarray(vi.astype(numpy.int32))
+        self._device = None
+
+    def __call__(self, x: int) -> int:
+        return int(self.values[x])
+
+    def device_table(self, n_reg1: int) -> "_FuncTable":
+        if self.values.shape[0] != 2 ** n_reg1:
+            raise ValueError("function table has {} entries for a {}-qubit register".format(self.values.shape[0], n_reg1))
+        if self._device is None:
+            self._device = _FuncTable(n_reg1, self.values)
+        return self._device
+
+
 def tabulate(func, n_reg1: int) -> numpy.ndarray:
     """f(x) for every x of the register as C int, like func_apply.pyx:66-69 (OverflowError past
     int32, as Cython's conversion raises). An integer ndarray is taken as the table itself."""
@@ -164,6 +192,8 @@ def tabulate(func, n_reg1: int) -> numpy.ndarray:
 
 
 def func_table(func, n_reg1: int, cache: bool = True) -> _FuncTable:
+    if isinstance(func, FuncTable):
+        return func.device_table(n_reg1)
     if not cache or isinstance(func, numpy.ndarray):
         return _FuncTable(n_reg1, tabulate(func, n_reg1))
     try:
@@ -180,6 +210,10 @@ def func_table(func, n_reg1: int, cache: bool = True) -> _FuncTable:
         except TypeError:
             pass
     return t
+
+
+PRODUCT_MAX = 12            # single-qubit factors per product sweep (QIPGPU_MAX_PRODUCT)
+PRODUCT_MIN_QUBITS = 12     # the product sweep tiles 12 state bits
 
 
 class GpuBackend(StateType):
@@ -294,7 +328,24 @@ class GpuBackend(StateType):
                        input_offset: int = 0, output_offset: int = 0) -> None:
         check_windows(self.n, 2 ** self.n, input_offset, output_offset)
         for call in planner.plan(mats):
-            self.apply_matrix(call.targets, call.mat, call.controls)
+            if isinstance(call, planner.ProductCall):
+                self.apply_product(call.targets, call.mats)
+            else:
+                self.apply_matrix(call.targets, call.mat, call.controls)
+
+    def apply_product(self, targets: Sequence[int], mats: Sequence[numpy.ndarray]) -> None:
+        """Single-qubit operators mats[j] on distinct qubits targets[j], up to 12 per sweep
+        (n >= 12; smaller states take Kronecker blocks of 3 through apply_matrix)."""
+        if self.n < PRODUCT_MIN_QUBITS:
+            for call in planner.ProductCall(targets, mats).matrix_calls():
+                self.apply_matrix(call.targets, call.mat)
+            return
+        for i in range(0, len(targets), PRODUCT_MAX):
+            t = _capi.i32(targets[i:i + PRODUCT_MAX])
+            m = numpy.ascontiguousarray(numpy.stack([numpy.asarray(x, dtype=numpy.complex128).reshape(2, 2)
+                                                     for x in mats[i:i + PRODUCT_MAX]]))
+            _capi.check(self._lib.qipgpu_apply_product(self._h, len(t), _capi.ptr(t), _capi.ptr(m.view(numpy.float64))),
+                        "apply_product")
 
     def apply_matrix(self, targets: Sequence[int], mat: numpy.ndarray, controls: Sequence[int] = ()) -> None:
         """One engine call: `mat` on `targets` (first = MSB) where all `controls` are 1."""

@@ -90,6 +90,25 @@ using namespace qipgpu;
 // small device helpers
 // ------------------------------------------------------------------------------------------------
 
+// Gate arithmetic runs in FP64 for both state precisions: a complex64 state is widened on load
+// and rounded once on store, so its only error per sweep is the unbiased storage rounding (an FP32
+// matrix would not be exactly unitary and would drift the state coherently over many sweeps).
+template <typename R>
+__device__ __forceinline__ cx<double> widen(cx<R> a) {
+    cx<double> r;
+    r.re = (double)a.re;
+    r.im = (double)a.im;
+    return r;
+}
+
+template <typename R>
+__device__ __forceinline__ cx<R> narrow(cx<double> a) {
+    cx<R> r;
+    r.re = (R)a.re;
+    r.im = (R)a.im;
+    return r;
+}
+
 template <typename R>
 __device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) {
     cx<R> r;
@@ -399,7 +418,7 @@ struct DenseArgs {
     uint64_t ctrl_mask;
     PosList pl;
     uint64_t off[1 << K];
-    cx<R> U[(1 << K) * (1 << K)];
+    cx<double> U[(1 << K) * (1 << K)];
 };
 
 template <typename R, int K, int ITEMS>
@@ -422,12 +441,12 @@ __global__ __launch_bounds__(256) void k_dense(const DenseArgs<R, K> a) {
         if (g0 + (uint64_t)it * 256 < a.nbase) {
 #pragma unroll
             for (int i = 0; i < D; ++i) {
-                cx<R> acc;
+                cx<double> acc;
                 acc.re = 0;
                 acc.im = 0;
 #pragma unroll
-                for (int j = 0; j < D; ++j) cfma(acc, a.U[i * D + j], v[it][j]);
-                a.amps[idx[it] + a.off[i]] = acc;
+                for (int j = 0; j < D; ++j) cfma(acc, a.U[i * D + j], widen(v[it][j]));
+                a.amps[idx[it] + a.off[i]] = narrow<R>(acc);
             }
         }
     }
@@ -438,19 +457,19 @@ __global__ __launch_bounds__(256) void k_dense(const DenseArgs<R, K> a) {
 // device with wave-uniform addresses.
 template <typename R, int K>
 __global__ __launch_bounds__(64) void k_dense_lds(cx<R> *amps, uint64_t nbase, uint64_t ctrl_mask, PosList pl,
-                                                  const uint64_t *__restrict__ off, const cx<R> *__restrict__ U) {
+                                                  const uint64_t *__restrict__ off, const cx<double> *__restrict__ U) {
     constexpr int D = 1 << K;
-    __shared__ cx<R> v[D][64];
+    __shared__ cx<double> v[D][64];
     const uint64_t g = (uint64_t)blockIdx.x * 64 + threadIdx.x;
     if (g >= nbase) return;
     const uint64_t base = insert_zeros(g, pl) | ctrl_mask;
-    for (int j = 0; j < D; ++j) v[j][threadIdx.x] = amps[base + off[j]];
+    for (int j = 0; j < D; ++j) v[j][threadIdx.x] = widen(amps[base + off[j]]);
     for (int i = 0; i < D; ++i) {
-        cx<R> acc;
+        cx<double> acc;
         acc.re = 0;
         acc.im = 0;
         for (int j = 0; j < D; ++j) cfma(acc, U[i * D + j], v[j][threadIdx.x]);
-        amps[base + off[i]] = acc;
+        amps[base + off[i]] = narrow<R>(acc);
     }
 }
 
@@ -463,7 +482,7 @@ struct DiagArgs {
     uint64_t ctrl_mask;
     PosList pl;          // control positions only
     uint8_t tpos[8];     // state bit of target j (j = 0 is the MSB of the diagonal index)
-    cx<R> d[1 << K];
+    cx<double> d[1 << K];
 };
 
 template <typename R, int K, int ITEMS>
@@ -483,7 +502,7 @@ __global__ __launch_bounds__(256) void k_diag(const DiagArgs<R, K> a) {
             uint32_t j = 0;
 #pragma unroll
             for (int b = 0; b < K; ++b) j = (j << 1) | (uint32_t)((idx[it] >> a.tpos[b]) & 1ull);
-            a.amps[idx[it]] = cmul(v[it], a.d[j]);
+            a.amps[idx[it]] = narrow<R>(cmul(widen(v[it]), a.d[j]));
         }
     }
 }
@@ -585,8 +604,8 @@ static int launch_dense_reg(qipgpu_state *s, const std::vector<int> &tbits, uint
         a.off[j] = o;
     }
     for (int i = 0; i < D * D; ++i) {
-        a.U[i].re = (R)U[i].real();
-        a.U[i].im = (R)U[i].imag();
+        a.U[i].re = U[i].real();
+        a.U[i].im = U[i].imag();
     }
     constexpr int ITEMS = 1;  // one base per thread measured fastest (large grids keep HBM busy)
     const uint32_t nb = blocks_for(a.nbase, 256 * ITEMS);
@@ -600,9 +619,9 @@ static int launch_dense_lds(qipgpu_state *s, const std::vector<int> &tbits, uint
                             const std::vector<int> &allpos, const std::vector<cd> &U) {
     constexpr int D = 1 << K;
     // upload [off (D uint64) | U (D*D cx<R>)] through the staging ring
-    std::vector<char> buf(sizeof(uint64_t) * D + sizeof(cx<R>) * D * D);
+    std::vector<char> buf(sizeof(uint64_t) * D + sizeof(cx<double>) * D * D);
     uint64_t *off = reinterpret_cast<uint64_t *>(buf.data());
-    cx<R> *Uh = reinterpret_cast<cx<R> *>(buf.data() + sizeof(uint64_t) * D);
+    cx<double> *Uh = reinterpret_cast<cx<double> *>(buf.data() + sizeof(uint64_t) * D);
     for (int j = 0; j < D; ++j) {
         uint64_t o = 0;
         for (int t = 0; t < K; ++t)
@@ -610,8 +629,8 @@ static int launch_dense_lds(qipgpu_state *s, const std::vector<int> &tbits, uint
         off[j] = o;
     }
     for (int i = 0; i < D * D; ++i) {
-        Uh[i].re = (R)U[i].real();
-        Uh[i].im = (R)U[i].imag();
+        Uh[i].re = U[i].real();
+        Uh[i].im = U[i].imag();
     }
     void *d = nullptr;
     int rc = ring_upload(s, buf.data(), buf.size(), &d);
@@ -621,7 +640,7 @@ static int launch_dense_lds(qipgpu_state *s, const std::vector<int> &tbits, uint
     const uint64_t nbase = s->len >> allpos.size();
     const uint32_t nb = blocks_for(nbase, 64);
     hipLaunchKernelGGL((k_dense_lds<R, K>), dim3(nb), dim3(64), 0, s->stream, (cx<R> *)s->amps, nbase, ctrl_mask, pl,
-                       (const uint64_t *)d, (const cx<R> *)((char *)d + sizeof(uint64_t) * D));
+                       (const uint64_t *)d, (const cx<double> *)((char *)d + sizeof(uint64_t) * D));
     QIP_LAUNCHED();
     return QIPGPU_OK;
 }
@@ -636,8 +655,8 @@ static int launch_diag(qipgpu_state *s, const std::vector<int> &tbits, uint64_t 
     fill_poslist(a.pl, cpos);
     for (int b = 0; b < K; ++b) a.tpos[b] = (uint8_t)tbits[b];
     for (int j = 0; j < (1 << K); ++j) {
-        a.d[j].re = (R)d[j].real();
-        a.d[j].im = (R)d[j].imag();
+        a.d[j].re = d[j].real();
+        a.d[j].im = d[j].imag();
     }
     constexpr int ITEMS = 4;
     const uint32_t nb = blocks_for(a.count, 256 * ITEMS);

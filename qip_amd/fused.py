@@ -100,6 +100,10 @@ class FusedBackend(GpuBackend):
     def kronselect_dot(self, mats, input_offset: int = 0, output_offset: int = 0) -> None:
         check_windows(self.n, 2 ** self.n, input_offset, output_offset)
         for call in planner.plan(mats):
+            if isinstance(call, planner.ProductCall):
+                for q, m in zip(call.targets, call.mats):
+                    self._gates.append(((q,), numpy.ascontiguousarray(m, dtype=numpy.complex128)))
+                continue
             gq, G = controlled_dense(call)
             if len(gq) > self.fusion_width:
                 self.flush()
@@ -115,6 +119,10 @@ class FusedBackend(GpuBackend):
             GpuBackend.apply_matrix(self, targets, mat, controls)
         else:
             self._gates.append((gq, G))
+
+    def apply_product(self, targets, mats) -> None:
+        for q, m in zip(targets, mats):
+            self._gates.append(((q,), numpy.ascontiguousarray(m, dtype=numpy.complex128).reshape(2, 2)))
 
     def reinit(self, index_groups=(), feed_list=()) -> None:
         self._gates = []

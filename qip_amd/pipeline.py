@@ -50,7 +50,11 @@ class PipelineObject(object):
         return self.inputs
 
     def __hash__(self):
-        return hash(repr(self))
+        # Nodes compare by identity, so an identity hash is consistent with equality. (The
+        # reference hashes repr(self), qip/pipeline.py:67-68, which recurses through every input
+        # and makes each set operation of a run O(graph depth); its string hash is randomised per
+        # process, so no result can depend on the order it gives.)
+        return object.__hash__(self)
 
 
 def run(*args: PipelineObject,

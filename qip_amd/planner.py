@@ -25,6 +25,7 @@ import numpy
 
 KRON_NUMPY, KRON_CMAT, KRON_SWAP = 1, 2, 3          # qip/ext/kronprod.pyx:18-19
 MAX_EXPANDED_QUBITS = 10                             # dense expansion of shared-qubit components
+KRON_MERGE_WIDTH = 3                                 # single-qubit factors merged per sweep
 
 _SWAP4 = numpy.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], dtype=numpy.complex128)
 
@@ -187,4 +188,39 @@ def plan(mats: Mapping) -> List[MatrixCall]:
         else:
             union, dense = _expand_component(groups)
             calls.append(MatrixCall(union, [], dense))
-    return calls
+    return merge_single_qubit(calls)
+
+
+class ProductCall:
+    """The uncontrolled single-qubit factors of one descriptor (distinct qubits, so they commute),
+    applied together: mats[j] (2x2) on targets[j]."""
+    __slots__ = ("targets", "mats")
+    controls = ()
+
+    def __init__(self, targets: Sequence[int], mats: Sequence[numpy.ndarray]):
+        self.targets = list(targets)
+        self.mats = list(mats)
+
+    def matrix_calls(self, width: int = KRON_MERGE_WIDTH) -> List[MatrixCall]:
+        """The same product as Kronecker blocks on up to `width` qubits each."""
+        out = []
+        for i in range(0, len(self.targets), width):
+            mat = self.mats[i]
+            for m in self.mats[i + 1:i + width]:
+                mat = numpy.kron(mat, m)
+            out.append(MatrixCall(self.targets[i:i + width], [], mat))
+        return out
+
+    def __repr__(self):
+        return "ProductCall(t={})".format(self.targets)
+
+
+def merge_single_qubit(calls: List[MatrixCall]) -> list:
+    """Collect the uncontrolled single-qubit factors of one descriptor into one ProductCall (one
+    sweep for all of them; the reference applies the whole descriptor in one sweep too,
+    kronprod.pyx:157-197)."""
+    singles = [c for c in calls if len(c.targets) == 1 and not c.controls]
+    if len(singles) < 2:
+        return calls
+    rest = [c for c in calls if not (len(c.targets) == 1 and not c.controls)]
+    return rest + [ProductCall([c.targets[0] for c in singles], [c.mat for c in singles])]

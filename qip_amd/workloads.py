@@ -15,6 +15,7 @@ from typing import Dict, List, Tuple
 
 import numpy
 
+from .backend import FuncTable
 from .operators import CMat
 
 X = numpy.array([[0.0, 1.0], [1.0, 0.0]], dtype=numpy.complex128)
@@ -76,3 +77,67 @@ def random_product_unitaries(n: int, seed: int) -> List[Dict]:
     """One Haar single-qubit unitary per qubit (a layer without the brickwork)."""
     rng = numpy.random.default_rng(seed)
     return [{q: haar_2x2(rng)} for q in range(n)]
+
+
+# ---- iterative Grover search --------------------------------------------------------------------
+
+def grover_iterations(n_search: int) -> int:
+    """floor(pi/4 * sqrt(2^n_search))."""
+    return int(numpy.floor(numpy.pi / 4 * numpy.sqrt(2.0 ** n_search)))
+
+
+def grover_initial(n_search: int):
+    """(index_groups, feeds) of the search register in uniform superposition and the ancilla in
+    |-> (the defaults of the reference's examples/grovers_iterative.py:33-35)."""
+    N = 2 ** n_search
+    return ([list(range(n_search)), [n_search]],
+            [numpy.ones(N) / numpy.sqrt(N), numpy.array([1.0, -1.0]) / numpy.sqrt(2.0)])
+
+
+def grover_iteration_ops(n_search: int, x_star: int, diffusion: str = "f") -> List[Tuple]:
+    """One Grover iteration on search qubits 0..n_search-1 and ancilla qubit n_search.
+
+    Oracle: F(x == x_star) into the |-> ancilla (phase kickback), as grovers_iterative.py:20-21.
+    Diffusion "f": H on the register, F(x == 0), H (grovers_iterative.py:24-26);
+    diffusion "mcz": H, X, the (n_search-1)-controlled Z, X, H. Both equal
+    -(2|s><s| - I) on the search register. Operations are ("kron", mats) for
+    backend.kronselect_dot and ("func", reg1, reg2, f) for backend.func_apply, with f a FuncTable
+    (callable, and kept on the device across iterations)."""
+    N = 2 ** n_search
+    search = list(range(n_search))
+    anc = [n_search]
+    xs = numpy.arange(N)
+    oracle = FuncTable(xs == x_star)
+    hs = {q: H for q in search}
+    ops = [("func", search, anc, oracle)]
+    if diffusion == "f":
+        ops += [("kron", hs), ("func", search, anc, FuncTable(xs == 0)), ("kron", hs)]
+    elif diffusion == "mcz":
+        m = Z
+        for _ in range(n_search - 1):
+            m = CMat(m)
+        xx = {q: X for q in search}
+        ops += [("kron", hs), ("kron", xx), ("kron", {tuple(search): m}), ("kron", xx), ("kron", hs)]
+    else:
+        raise ValueError("diffusion must be 'f' or 'mcz'")
+    return ops
+
+
+def apply_ops(backend, ops) -> None:
+    for op in ops:
+        if op[0] == "kron":
+            backend.kronselect_dot(op[1])
+        else:
+            backend.func_apply(op[1], op[2], op[3])
+
+
+def grover_closed_form(n_search: int, x_star: int, iterations: int) -> numpy.ndarray:
+    """State after `iterations` Grover iterations (either diffusion form): marked amplitude
+    (-1)^t sin((2t+1) theta), others (-1)^t cos((2t+1) theta) / sqrt(N-1), sin theta = 2^(-n/2),
+    times the |-> ancilla."""
+    N = 2 ** n_search
+    theta = numpy.arcsin(1.0 / numpy.sqrt(N))
+    sign = (-1.0) ** iterations
+    search = numpy.full(N, sign * numpy.cos((2 * iterations + 1) * theta) / numpy.sqrt(N - 1))
+    search[x_star] = sign * numpy.sin((2 * iterations + 1) * theta)
+    return numpy.kron(search, numpy.array([1.0, -1.0]) / numpy.sqrt(2.0))

@@ -37,7 +37,10 @@ def expand_calls(calls, n):
     """Dense 2^n operator of a list of planner calls, applied in order (numpy only)."""
     dim = 2 ** n
     total = numpy.eye(dim, dtype=complex)
+    expanded = []
     for call in calls:
+        expanded.extend(call.matrix_calls() if isinstance(call, planner.ProductCall) else [call])
+    for call in expanded:
         op = numpy.zeros((dim, dim), dtype=complex)
         t, c = call.targets, call.controls
         for col in range(dim):

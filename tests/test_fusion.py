@@ -22,7 +22,10 @@ def gates_of(ops):
     gates = []
     for d in ops:
         for call in planner.plan(d):
-            gates.append(controlled_dense(call))
+            if isinstance(call, planner.ProductCall):
+                gates.extend(((q,), m) for q, m in zip(call.targets, call.mats))
+            else:
+                gates.append(controlled_dense(call))
     return gates
 
 

@@ -436,3 +436,17 @@ def test_stochastic_measure_graph():
     _, c = run(sm, q1, feed={q1: [1, 0, 0, 0], q2: a}, backend_constructor=OracleBackend.make_state)
     rev = [int(format(i, "03b")[::-1], 2) for i in range(8)]
     assert numpy.allclose(c[sm], (numpy.abs(a) ** 2)[rev])
+
+
+@pytest.mark.parametrize("diffusion", ["f", "mcz"])
+def test_grover_workload_closed_form(diffusion):
+    """The Grover workload's operation list (both diffusion forms) on the oracle equals the closed
+    form after the full floor(pi/4 sqrt(N)) iterations."""
+    from qip_amd.workloads import apply_ops, grover_closed_form, grover_initial, grover_iteration_ops, grover_iterations
+    ns, xs = 7, 90
+    groups, feeds = grover_initial(ns)
+    be = OracleBackend.make_state(ns + 1, groups, feeds)
+    t = grover_iterations(ns)
+    for _ in range(t):
+        apply_ops(be, grover_iteration_ops(ns, xs, diffusion))
+    assert numpy.max(numpy.abs(be.get_state() - grover_closed_form(ns, xs, t))) < 1e-13
