@@ -31,6 +31,23 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, published
 
+# HBM traffic of one k_fused6 launch on a 30-qubit complex64 state (8 GiB), measured by two
+# rocprofv3 --pmc runs of `bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-plain
+# --copy-calibration-gib 1` (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch, mean over the 128
+# block launches). Each counter is scaled by what it reads for the 1 GiB device-to-device copy in
+# the same run: FETCH_SIZE reads 524315 KiB for 1048576 KiB moved (x2.0), WRITE_SIZE 1048576 KiB
+# (x1.0). Commands and per-kernel figures: profiles/k_fused6_circuit30_depth40.txt.
+FUSED6_PMC_30Q = {"fetch_kib": 4195726.3, "fetch_scale": 1048576.0 / 524315.0,
+                  "write_kib": 8388632.9, "write_scale": 1.0}
+
+
+def fused6_traffic_bytes(n: int):
+    """Measured HBM bytes per k_fused6 launch at n = 30 (None for other sizes)."""
+    if n != 30:
+        return None
+    p = FUSED6_PMC_30Q
+    return int((p["fetch_kib"] * p["fetch_scale"] + p["write_kib"] * p["write_scale"]) * 1024)
+
 
 class HipEvents:
     """hipEvent timing on an arbitrary stream, through libamdhip64."""
@@ -388,7 +405,7 @@ def main():
         "roofline": {
             "bound": "hbm", "kernel": "k_fused6 (six-qubit split-BF16 MFMA block, complex64)",
             "achieved": fused_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": fused_gbs / HBM_PEAK_GBS,
-            "traffic": None, "bytes_per_launch": block_bytes, "avg_launch_ms": avg_block_ms,
+            "traffic": fused6_traffic_bytes(n), "bytes_per_launch": block_bytes, "avg_launch_ms": avg_block_ms,
         },
         "fused_c64": {"gates_per_s": fused_gates_s, "s_per_circuit": elapsed / args.steps,
                       "gpu_ms_per_circuit": sum(gpu_ms) / len(gpu_ms), "blocks": nblocks,
