@@ -320,6 +320,127 @@ def bench_qft16(args):
     return out
 
 
+def bench_batch512(args):
+    """512 independent layered random circuits, n ~ U{10..22}, depth ~ U{8..32} (seeded), complex64:
+    the batching form (one captured graph per run, autotuned) against the plain complex64 path with
+    one engine call (one kernel launch) per gate, and per-circuit 1 - F against plain complex128.
+    A step is: reset all 512 states to |0...0>, run every circuit, synchronise."""
+    import numpy
+    from qip_amd import GpuBackend
+    from qip_amd.batch import GpuBatch
+    from qip_amd.workloads import batch_layered_circuits
+    log = lambda msg: print("[bench] " + msg, file=sys.stderr, flush=True)  # noqa: E731
+    count = 512
+    t = time.perf_counter()
+    ns, depths, seeds, packed = batch_layered_circuits(count, args.seed)
+    gen_s = time.perf_counter() - t
+    b = GpuBatch(ns)
+    t = time.perf_counter()
+    b.load(packed)
+    load_s = time.perf_counter() - t
+    t = time.perf_counter()
+    tuning, tuned_ms = b.autotune(reps=2)
+    tune_s = time.perf_counter() - t
+    info = b.info()
+    log("batch: {} gates, {} blocks in {} waves, tuning {} ({:.2f} ms)".format(
+        info["gates"], info["blocks"], info["waves"], tuning, tuned_ms))
+    ev = HipEvents()
+    e0, e1 = ev.create(), ev.create()
+    for _ in range(args.warmup):
+        b.reset()
+        b.run()
+        b.synchronize()
+    gpu_ms = []
+    start = time.perf_counter()
+    for _ in range(args.steps):
+        b.reset()
+        ev.record(e0, b.stream)
+        b.run()
+        ev.record(e1, b.stream)
+        gpu_ms.append(ev.elapsed_ms(e0, e1))
+    b.synchronize()
+    elapsed = time.perf_counter() - start
+    circuits_s = count * args.steps / elapsed
+    run_ms = sum(gpu_ms) / len(gpu_ms)
+    eff_gbs = info["algorithmic_bytes"] / (run_ms * 1e-3) / 1e9
+    log("batch: {} steps in {:.3f} s, run {:.2f} ms".format(args.steps, elapsed, run_ms))
+
+    # plain paths, one launch per gate: complex64 (timed) and complex128 (the fidelity reference)
+    gates = [packed.gates_of(i) for i in range(count)]
+    plain = {}
+    fids, errs = [], []
+    for name, st in (() if args.no_plain else (("c64", numpy.complex64), ("c128", numpy.complex128))):
+        warm = GpuBackend.make_state(ns[0], [], [], statetype=st)
+        for gq, G in gates[0]:
+            warm.apply_matrix(list(gq), G)
+        warm.close()
+        t = time.perf_counter()
+        for i in range(count):
+            be = GpuBackend.make_state(ns[i], [], [], statetype=st)
+            for gq, G in gates[i]:
+                be.apply_matrix(list(gq), G)
+            be.synchronize()
+            if name == "c128":
+                want = be.get_state()
+                got = b.get_state(i).astype(numpy.complex128)
+                ov = numpy.vdot(want, got)
+                fids.append(1.0 - abs(ov) ** 2 / (numpy.vdot(want, want).real * numpy.vdot(got, got).real))
+                errs.append(float(numpy.max(numpy.abs(want - got))))
+            be.close()
+        plain[name] = time.perf_counter() - t
+        log("plain {}: {:.3f} s".format(name, plain[name]))
+    b.close()
+    cfg = {"workload": "512 independent layered random circuits (Haar 1q + CNOT/CZ brickwork), n ~ U{10..22}, "
+                       "depth ~ U{8..32}, complex64, batched (LDS-resident small states + fused MFMA waves, one "
+                       "hipGraph per run)", "circuits": count, "seed": args.seed, "gates": info["gates"],
+           "qubits_total": int(sum(ns)), "lds_states": info["lds_states"], "tiled_states": info["tiled_states"],
+           "blocks": info["blocks"], "waves": info["waves"], "tuning": tuning}
+    out = _line("512 uneven circuits/s, batched c64 vs one launch per gate", circuits_s, "circuits/s", args.steps,
+                args.warmup, elapsed * 1e3 / args.steps, "c64", cfg,
+                batch={"circuits_per_s": circuits_s, "gpu_ms_per_run": run_ms, "algorithmic_bytes_per_run":
+                       info["algorithmic_bytes"], "effective_gbs": eff_gbs, "hbm_frac": eff_gbs / HBM_PEAK_GBS,
+                       "generate_s": gen_s, "load_plan_capture_s": load_s, "autotune_s": tune_s,
+                       "autotuned_run_ms": tuned_ms})
+    if plain:
+        out.update(plain_per_gate_c64={"circuits_per_s": count / plain["c64"], "s_per_batch": plain["c64"]},
+                   plain_per_gate_c128={"circuits_per_s": count / plain["c128"], "s_per_batch": plain["c128"]},
+                   speedup_vs_plain_c64=plain["c64"] / (elapsed / args.steps),
+                   infidelity_vs_c128={"max": float(max(fids)), "median": float(numpy.median(fids))},
+                   max_abs_err_vs_c128=float(max(errs)))
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_batch(ns, gates, args.cpu_seconds)
+    return out
+
+
+def cpu_baseline_batch(ns, gates, seconds: float):
+    """The oracle's cdot_loop restatement (complex128, one out-of-place pass per gate, all visible
+    cores) on whole circuits of the batch, in batch order, until `seconds` of work; the rate in
+    amplitude-gate updates (sum over gates of 2^n) is extrapolated to the whole batch."""
+    import numpy
+    from oracle import oracle
+    cores = len(os.sched_getaffinity(0))
+    total_work = sum(len(g) * 2 ** n for n, g in zip(ns, gates))
+    done_work, spent, circuits = 0, 0.0, 0
+    for n, gs in zip(ns, gates):
+        v = numpy.zeros(2 ** n, dtype=numpy.complex128)
+        v[0] = 1.0
+        out = numpy.zeros_like(v)
+        t = time.perf_counter()
+        for gq, G in gs:
+            oracle.kronselect_dot({tuple(gq): G}, v, n, out, fast=True, nthreads=cores)
+            v, out = out, v
+        spent += time.perf_counter() - t
+        done_work += len(gs) * 2 ** n
+        circuits += 1
+        if spent >= seconds:
+            break
+    full_s = spent * total_work / done_work
+    return {"value": len(ns) / full_s, "unit": "circuits/s", "cores": cores, "kind": "port",
+            "sample": "the first {} circuits of the batch run in full on the oracle's cdot_loop restatement "
+                      "(-O3 -ffast-math -fopenmp), {:.1f} s; rate per amplitude-gate update extrapolated to the "
+                      "whole batch".format(circuits, spent)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -334,7 +455,8 @@ def main():
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="after the benchmark, one device-to-device copy of this many GiB (a kernel of known "
                          "traffic for calibrating hardware byte counters)")
-    ap.add_argument("--workload", default="circuit30", choices=["circuit30", "circuit28", "grover24", "qft16"],
+    ap.add_argument("--workload", default="circuit30",
+                    choices=["circuit30", "circuit28", "grover24", "qft16", "batch512"],
                     help="circuit30 (default, the headline) or one of the other configurations, each printed as "
                          "its own JSON line")
     args = ap.parse_args()
@@ -343,7 +465,8 @@ def main():
               "a multi-GPU run is not part of this build (--gpus {} requested)".format(args.gpus), file=sys.stderr)
         sys.exit(2)
     if args.workload != "circuit30":
-        result = {"circuit28": bench_circuit28, "grover24": bench_grover24, "qft16": bench_qft16}[args.workload](args)
+        result = {"circuit28": bench_circuit28, "grover24": bench_grover24, "qft16": bench_qft16,
+                  "batch512": bench_batch512}[args.workload](args)
         print(json.dumps(result), flush=True)
         return
 

@@ -163,6 +163,66 @@ int qipgpu_synchronize(qipgpu_state *s);
  * their own work against it (events, copies). */
 int qipgpu_get_stream(qipgpu_state *s, void **stream);
 
+/* ---- batching form: many independent circuits of uneven qubit counts -------------------------
+ * One batch holds `count` complex64 states (each in its own slice of one device arena) and a
+ * gate list per state; a run applies every state's gates. It replaces `count` independent
+ * reference runs, each a CythonBackend.make_state (qip/backend.py:73-104) of |0...0> followed by
+ * one kronselect_dot (qip/backend.py:112-116 -> kronprod.pyx:41-200) per gate, as
+ * qip/pipeline.py:71-135 issues them; the reference has no batched entry point.
+ *
+ * Gates are given packed, in program order per state (states may interleave): gate g acts on
+ * state gate_state[g], on the gate_k[g] qubits gate_qubits[6 g .. 6 g + k - 1] (first = MSB of
+ * the matrix index, qubit q = state bit n-1-q), with the dense 2^k x 2^k row-major complex128
+ * matrix that follows the previous gates' matrices in `mats` (controls are part of the dense
+ * matrix). 1 <= k <= 6.
+ *
+ * States with n <= lds_max_qubits run their whole gate list in one workgroup with the state in
+ * LDS, gate by gate, in FP64 arithmetic. Larger states have their gates fused greedily into
+ * blocks of at most fusion_width qubits (the rule of the precision form), applied with the
+ * split-BF16 MFMA tile kernel; block j of every large state runs in wave j, and each block's
+ * output is renormalised by the FP64 norm of the previous block's output. Every state that has
+ * gates leaves a run renormalised to FP64 norm 1; a state without gates is left as it is. */
+typedef struct qipgpu_batch qipgpu_batch;
+
+typedef struct {
+    int32_t fusion_width;   /* 1..6: qubits per fused block of the large states (default 6) */
+    int32_t lds_max_qubits; /* 11..13: states with n <= this run whole in LDS (default 13) */
+    int32_t chunk_tiles;    /* 1..64: 4096-amplitude tiles per workgroup in a block pass (default 8) */
+    int32_t use_graph;      /* 1: a run replays one captured hipGraph (default); 0: eager launches */
+} qipgpu_batch_tuning;
+
+/* Host-only planning (no GPU call): validate a packed gate list and return the fusion the batch
+ * would use. gate_block[g] = index of gate g's block within its state, or -1 if the state runs in
+ * LDS; state_blocks[i] = blocks of state i (0 for LDS states). */
+int qipgpu_batch_plan(int32_t count, const int32_t *n_qubits, int64_t n_gates, const int32_t *gate_state,
+                      const int32_t *gate_k, const int32_t *gate_qubits, int32_t fusion_width,
+                      int32_t lds_max_qubits, int32_t *gate_block, int32_t *state_blocks);
+/* Allocate `count` states (1 <= n_qubits[i] <= 30), all |0...0>, with no gates. */
+int qipgpu_batch_create(int32_t count, const int32_t *n_qubits, qipgpu_batch **out);
+int qipgpu_batch_destroy(qipgpu_batch *b);
+/* Replace the gate lists (packed as above) and plan them with `tuning` (NULL: defaults). */
+int qipgpu_batch_load(qipgpu_batch *b, int64_t n_gates, const int32_t *gate_state, const int32_t *gate_k,
+                      const int32_t *gate_qubits, const double *mats, const qipgpu_batch_tuning *tuning);
+/* Re-plan the loaded gates with another tuning. */
+int qipgpu_batch_set_tuning(qipgpu_batch *b, const qipgpu_batch_tuning *tuning);
+int qipgpu_batch_get_tuning(qipgpu_batch *b, qipgpu_batch_tuning *out);
+/* Time runs from |0...0> under candidate tunings (chunk_tiles, then lds_max_qubits, then
+ * fusion_width, each kept at its best so far; `reps` timed runs per candidate after one warm-up)
+ * and keep the fastest. Leaves every state reset to |0...0>. */
+int qipgpu_batch_autotune(qipgpu_batch *b, int32_t reps, qipgpu_batch_tuning *chosen, double *best_ms);
+/* Set every state to |0...0> (enqueued). */
+int qipgpu_batch_reset(qipgpu_batch *b);
+/* Apply every state's gates to its current contents (enqueued). */
+int qipgpu_batch_run(qipgpu_batch *b);
+int qipgpu_batch_synchronize(qipgpu_batch *b);
+/* Copy state i (2^n complex64) to / from host memory (synchronous). */
+int qipgpu_batch_get_state(qipgpu_batch *b, int32_t i, void *out);
+int qipgpu_batch_set_state(qipgpu_batch *b, int32_t i, const void *in);
+/* Plan figures: out[0..len) of {states, LDS states, tiled states, gates, blocks, waves, chunks,
+ * largest wave (blocks), arena bytes, algorithmic HBM bytes of one run}. */
+int qipgpu_batch_info(qipgpu_batch *b, int64_t *out, int32_t len);
+int qipgpu_batch_get_stream(qipgpu_batch *b, void **stream);
+
 #ifdef __cplusplus
 }
 #endif

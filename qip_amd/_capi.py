@@ -56,7 +56,27 @@ SIGNATURES = {
     "qipgpu_max_abs_diff": (ctypes.c_int, [_P, _P, _PD]),
     "qipgpu_synchronize": (ctypes.c_int, [_P]),
     "qipgpu_get_stream": (ctypes.c_int, [_P, _PP]),
+    "qipgpu_batch_plan": (ctypes.c_int, [_I32, _P, _I64, _P, _P, _P, _I32, _I32, _P, _P]),
+    "qipgpu_batch_create": (ctypes.c_int, [_I32, _P, _PP]),
+    "qipgpu_batch_destroy": (ctypes.c_int, [_P]),
+    "qipgpu_batch_load": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P]),
+    "qipgpu_batch_set_tuning": (ctypes.c_int, [_P, _P]),
+    "qipgpu_batch_get_tuning": (ctypes.c_int, [_P, _P]),
+    "qipgpu_batch_autotune": (ctypes.c_int, [_P, _I32, _P, _PD]),
+    "qipgpu_batch_reset": (ctypes.c_int, [_P]),
+    "qipgpu_batch_run": (ctypes.c_int, [_P]),
+    "qipgpu_batch_synchronize": (ctypes.c_int, [_P]),
+    "qipgpu_batch_get_state": (ctypes.c_int, [_P, _I32, _P]),
+    "qipgpu_batch_set_state": (ctypes.c_int, [_P, _I32, _P]),
+    "qipgpu_batch_info": (ctypes.c_int, [_P, _P, _I32]),
+    "qipgpu_batch_get_stream": (ctypes.c_int, [_P, _PP]),
 }
+
+
+class BatchTuning(ctypes.Structure):
+    """qipgpu_batch_tuning"""
+    _fields_ = [("fusion_width", ctypes.c_int32), ("lds_max_qubits", ctypes.c_int32),
+                ("chunk_tiles", ctypes.c_int32), ("use_graph", ctypes.c_int32)]
 
 _lib = None
 

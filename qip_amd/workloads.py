@@ -141,3 +141,45 @@ def grover_closed_form(n_search: int, x_star: int, iterations: int) -> numpy.nda
     search = numpy.full(N, sign * numpy.cos((2 * iterations + 1) * theta) / numpy.sqrt(N - 1))
     search[x_star] = sign * numpy.sin((2 * iterations + 1) * theta)
     return numpy.kron(search, numpy.array([1.0, -1.0]) / numpy.sqrt(2.0))
+
+
+# ---- batches of uneven layered random circuits -----------------------------------------------
+
+CNOT4 = numpy.eye(4, dtype=numpy.complex128)[[0, 1, 3, 2]]        # control = first qubit (MSB)
+CZ4 = numpy.diag([1.0, 1.0, 1.0, -1.0]).astype(numpy.complex128)
+
+
+def layered_circuit_gates(n: int, depth: int, seed: int) -> List[Tuple[Tuple[int, ...], numpy.ndarray]]:
+    """layered_random_circuit(n, depth, seed) as dense gates (controls first), in program order,
+    with the Haar unitaries drawn in one call and factored by one stacked QR: the same random
+    stream and the same LAPACK factorisation per matrix as haar_2x2."""
+    rng = numpy.random.default_rng(seed)
+    z = rng.standard_normal((depth * n, 2, 2, 2))          # per gate: real 2x2, then imaginary 2x2
+    zc = (z[:, 0] + 1j * z[:, 1]) / numpy.sqrt(2.0)
+    q, r = numpy.linalg.qr(zc)
+    d = numpy.diagonal(r, axis1=1, axis2=2)
+    us = q * (d / numpy.abs(d))[:, None, :]
+    gates = []
+    for layer in range(depth):
+        for qb in range(n):
+            gates.append(((qb,), us[layer * n + qb]))
+        if layer % 2 == 0:
+            gates.extend(((j, j + 1), CNOT4) for j in range(0, n - 1, 2))
+        else:
+            gates.extend(((j, j + 1), CZ4) for j in range(1, n - 1, 2))
+    return gates
+
+
+def batch_layered_circuits(count: int, seed: int, n_range: Tuple[int, int] = (10, 22),
+                           depth_range: Tuple[int, int] = (8, 32)):
+    """`count` layered random circuits with qubit counts and depths drawn uniformly from the
+    inclusive ranges and one circuit seed each, all from numpy.random.default_rng(seed) (counts,
+    then depths, then seeds). Returns (n list, depth list, seed list, PackedCircuits)."""
+    from .batch import PackedCircuits
+    rng = numpy.random.default_rng(seed)
+    ns = rng.integers(n_range[0], n_range[1] + 1, size=count)
+    depths = rng.integers(depth_range[0], depth_range[1] + 1, size=count)
+    seeds = rng.integers(0, 2 ** 31 - 1, size=count)
+    circuits = [layered_circuit_gates(int(n), int(d), int(s)) for n, d, s in zip(ns, depths, seeds)]
+    packed = PackedCircuits.from_gates([int(n) for n in ns], circuits)
+    return [int(n) for n in ns], [int(d) for d in depths], [int(s) for s in seeds], packed
