@@ -174,7 +174,7 @@ def bench_circuit28(args):
     per gate each; 1 - F of complex64 against complex128."""
     import numpy
     from qip_amd import GpuBackend, infidelity, max_abs_diff
-    from qip_amd.workloads import algorithmic_bytes_plain, layered_random_circuit
+    from qip_amd.workloads import algorithmic_bytes_plain, layered_random_circuit, touched_bytes_plain
     n, depth = 28, 20
     ops = layered_random_circuit(n, depth, args.seed)
     res = {}
@@ -192,7 +192,8 @@ def bench_circuit28(args):
         dt = time.perf_counter() - t
         gb = algorithmic_bytes_plain(ops, n, amp) / 1e9
         res[name] = {"gates_per_s": len(ops) / dt, "s_per_circuit": dt, "algorithmic_gbs": gb / dt,
-                     "hbm_frac": gb / dt / HBM_PEAK_GBS}
+                     "hbm_frac": gb / dt / HBM_PEAK_GBS,
+                     "touched_gbs": touched_bytes_plain(ops, n, amp) / 1e9 / dt}
         states[name] = be
     res["infidelity_c64_vs_c128"] = infidelity(states["c128"], states["c64"])
     res["max_abs_err_c64_vs_c128"] = max_abs_diff(states["c128"], states["c64"])
@@ -473,7 +474,7 @@ def main():
     import numpy
     from qip_amd import GpuBackend, infidelity, max_abs_diff
     from qip_amd.fused import FusedBackend
-    from qip_amd.workloads import algorithmic_bytes_plain, gate_counts, layered_random_circuit
+    from qip_amd.workloads import algorithmic_bytes_plain, gate_counts, layered_random_circuit, touched_bytes_plain
 
     n, depth = args.n, args.depth
     ops = layered_random_circuit(n, depth, args.seed)
@@ -553,6 +554,7 @@ def main():
         result["plain_c128"] = {
             "gates_per_s": ngates / plain_s, "s_per_circuit": plain_s,
             "algorithmic_gbs": plain_bytes / plain_s / 1e9, "hbm_frac": plain_bytes / plain_s / 1e9 / HBM_PEAK_GBS,
+            "touched_gbs": touched_bytes_plain(ops, n, 16) / plain_s / 1e9,
             "dominant_kernel": dict(dom, frac=dom["achieved_gbs"] / HBM_PEAK_GBS),
         }
         result["speedup_fused_vs_plain"] = plain_s / (elapsed / args.steps)

@@ -57,9 +57,25 @@ def gate_counts(ops: List[Dict]) -> Tuple[int, int]:
 
 
 def algorithmic_bytes_plain(ops: List[Dict], n: int, amp_bytes: int) -> int:
-    """Bytes a plain sweep must move: read + write of every amplitude a gate can change. A
-    single-qubit gate changes all 2^n; CNOT (control 1) changes 2^(n-1); CZ changes only the
-    amplitudes with both qubits 1, 2^(n-2)."""
+    """Algorithmic bytes of a plain run: read + write of every amplitude a gate can change,
+    2 * S * 2^(n - c) for a gate with c control qubits as written (a single-qubit gate 2^n, CNOT and
+    CZ, each one control, 2^(n-1)), with no credit for diagonal or permutation structure."""
+    total = 0
+    for d in ops:
+        key = next(iter(d))
+        m = d[key]
+        controls = 0
+        while isinstance(m, CMat):
+            controls += 1
+            m = m.m
+        total += 2 * amp_bytes * 2 ** (n - controls)
+    return total
+
+
+def touched_bytes_plain(ops: List[Dict], n: int, amp_bytes: int) -> int:
+    """Bytes the plain engine's kernels actually read and write: as algorithmic_bytes_plain, except
+    that CZ, which the engine applies as a phase on the amplitudes with both qubits 1, touches
+    2^(n-2) amplitudes."""
     total = 0
     for d in ops:
         key = next(iter(d))

@@ -6,7 +6,7 @@ import pytest
 
 from qip_amd import planner
 from qip_amd.fused import controlled_dense, fuse
-from qip_amd.workloads import algorithmic_bytes_plain, gate_counts, layered_random_circuit
+from qip_amd.workloads import algorithmic_bytes_plain, gate_counts, layered_random_circuit, touched_bytes_plain
 
 
 def full_operator(gq, G, n):
@@ -55,8 +55,11 @@ def test_block_counts_brickwork():
 def test_circuit_shape():
     ops = layered_random_circuit(30, 40, seed=2024)
     assert gate_counts(ops) == (1200, 580)
-    assert algorithmic_bytes_plain(ops, 30, 16) == (1200 * 32 + 300 * 16 + 280 * 8) * 2 ** 30
+    # the per-gate definition 2 S 2^(n - c), c = controls as written: 47,680 GiB and 5,560 GiB
+    assert algorithmic_bytes_plain(ops, 30, 16) == (1200 * 32 + 580 * 16) * 2 ** 30 == 47680 * 2 ** 30
+    assert touched_bytes_plain(ops, 30, 16) == (1200 * 32 + 300 * 16 + 280 * 8) * 2 ** 30
     ops2 = layered_random_circuit(28, 20, seed=1)
     assert gate_counts(ops2) == (560, 270)
+    assert algorithmic_bytes_plain(ops2, 28, 16) == 5560 * 2 ** 30
     U = ops[0][0]
     assert numpy.allclose(U.conj().T @ U, numpy.eye(2), atol=1e-15)
