@@ -16,7 +16,7 @@ using namespace qipgpu::tile;
 
 namespace {
 
-constexpr int kFusedBlocksPerCU = 2;   // 69 KB of LDS per workgroup
+constexpr int kFusedBlocksPerCU = 2;   // 69 KB of LDS and 212 VGPRs per workgroup
 
 __global__ __launch_bounds__(256, 2) void k_fused6(cx<float> *amps, const bf16x8 *__restrict__ wfrag, FusedArgs a,
                                                    const double *__restrict__ fstate, double *partial) {

@@ -126,3 +126,31 @@ def test_fused_backend_graph_qft():
     expect = numpy.fft.ifft(psi, norm="ortho")
     ov = numpy.vdot(expect, o)
     assert 1 - abs(ov) ** 2 / numpy.vdot(o, o).real < 1e-8
+
+
+def test_fused_backend_24_qubits_against_oracle():
+    """The precision form at 24 qubits against the oracle's FP64 restatement of cdot_loop (one
+    out-of-place pass per descriptor on the CPU): layered random circuit, depth 4, 142 gates."""
+    import os
+    from oracle import oracle
+    from qip_amd.fused import FusedBackend
+    from qip_amd.workloads import layered_random_circuit
+    n = 24
+    ops = layered_random_circuit(n, 4, seed=5)
+    fz = FusedBackend.make_state(n, [], [])
+    for d in ops:
+        fz.kronselect_dot(d)
+    got = fz.get_state().astype(numpy.complex128)
+    fz.close()
+    v = numpy.zeros(2 ** n, dtype=numpy.complex128)
+    v[0] = 1.0
+    out = numpy.zeros_like(v)
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    for d in ops:
+        oracle.kronselect_dot(d, v, n, out, fast=True, nthreads=threads)
+        v, out = out, v
+    ov = numpy.vdot(v, got)
+    inf = 1.0 - abs(ov) ** 2 / (numpy.vdot(v, v).real * numpy.vdot(got, got).real)
+    print("24-qubit fused vs oracle: 1-F", inf, flush=True)
+    assert inf < 1e-8
+    assert numpy.max(numpy.abs(got - v)) < 1e-6
