@@ -3,15 +3,22 @@
 ``FusedBackend`` is a StateType (qip/backend.py:14-65) like GpuBackend, but kronselect_dot only
 records: each engine call of the plan (targets, controls, matrix) becomes a dense gate on its
 controls + targets. Recorded gates are fused greedily into blocks of at most ``fusion_width``
-qubits and the blocks are applied by the split-BF16 MFMA kernel (qipgpu_fused_apply) when the
-state is next read or changed any other way (func_apply, measurement, probabilities, get_state,
-ranges). Every fused block renormalises the previous block's output with its FP64 norm.
+qubits and the blocks are applied by the split-BF16 MFMA kernel (qipgpu_fused_apply). A block is
+launched as soon as no later gate can join it (streaming), and every remaining block when the state
+is next read or changed any other way (func_apply, measurement, probabilities, get_state, ranges).
+Every fused block renormalises the previous block's output with its FP64 norm.
 
 Greedy fusion. Gates are taken in program order. For gate g on qubit set S let j be the last
 block that touches S. g may join block j or any later block (those do not touch S, so g commutes
 past them) whose qubit set stays within the width; the first that fits is taken, else g opens a
 new block at the end. A block's matrix is the ordered product of its gates embedded in the
 block's qubits. On the 30-qubit depth-40 brickwork circuit this gives 128 blocks for 1780 gates.
+
+Streaming. The rule is online: a gate's placement depends only on the blocks before it. A later gate
+scans from the last block touching one of its qubits, so once every qubit of the state has been
+touched by a block after block i, no later gate can reach block i: it is final and can run while the
+host records the rest. Launching final blocks early gives the same blocks, in the same order, with
+the same results, as fusing the whole record at the end.
 """
 from typing import List, Sequence, Tuple
 
@@ -56,6 +63,53 @@ def controlled_dense(call: planner.MatrixCall) -> Tuple[Tuple[int, ...], numpy.n
     return tuple(call.controls) + tuple(call.targets), M
 
 
+class OnlineFusion:
+    """The greedy rule applied one gate at a time on an n-qubit state, with the leading blocks that
+    no later gate can join handed out as they become final."""
+
+    def __init__(self, n: int, width: int):
+        self.n = n
+        self.width = width
+        self.blocks: List[Block] = []   # open blocks; blocks[i] has global index base + i
+        self.base = 0
+        self.last = {}                  # qubit -> global index of the last block touching it
+        self.pending = 0                # gates in open blocks
+
+    def add(self, gq: Tuple[int, ...], G: numpy.ndarray) -> None:
+        S = set(gq)
+        j = max(self.last.get(q, -1) for q in S)
+        start = max(max(j, 0) - self.base, 0)
+        target = None
+        for i in range(start, len(self.blocks)):
+            if len(self.blocks[i].qubits | S) <= self.width:
+                target = i
+                break
+        if target is None:
+            self.blocks.append(Block())
+            target = len(self.blocks) - 1
+        blk = self.blocks[target]
+        blk.qubits |= S
+        blk.gates.append((gq, G))
+        g = self.base + target
+        for q in S:
+            self.last[q] = max(self.last.get(q, -1), g)
+        self.pending += 1
+
+    def final_count(self) -> int:
+        """Leading open blocks that no later gate can join."""
+        if len(self.last) < self.n:
+            return 0
+        return max(0, min(self.last.values()) - self.base)
+
+    def take(self, count: int = None) -> List[Block]:
+        """Remove and return the first `count` open blocks (all when None)."""
+        count = len(self.blocks) if count is None else count
+        out, self.blocks = self.blocks[:count], self.blocks[count:]
+        self.base += len(out)
+        self.pending -= sum(len(b.gates) for b in out)
+        return out
+
+
 def fuse(gates: Sequence[Tuple[Tuple[int, ...], numpy.ndarray]], width: int) -> List[Block]:
     blocks: List[Block] = []
     for gq, G in gates:
@@ -82,10 +136,11 @@ class FusedBackend(GpuBackend):
     """complex64 state; kronselect_dot is deferred and fused (module docstring)."""
 
     fusion_width = 6
+    streaming = True
 
     def __init__(self, n, handle, dtype):
         super().__init__(n, handle, dtype)
-        self._gates = []
+        self._fuser = OnlineFusion(n, self.fusion_width)
         self.blocks_applied = 0
 
     @classmethod
@@ -102,14 +157,14 @@ class FusedBackend(GpuBackend):
         for call in planner.plan(mats):
             if isinstance(call, planner.ProductCall):
                 for q, m in zip(call.targets, call.mats):
-                    self._gates.append(((q,), numpy.ascontiguousarray(m, dtype=numpy.complex128)))
+                    self._record((q,), numpy.ascontiguousarray(m, dtype=numpy.complex128))
                 continue
             gq, G = controlled_dense(call)
             if len(gq) > self.fusion_width:
                 self.flush()
                 GpuBackend.apply_matrix(self, call.targets, call.mat, call.controls)
             else:
-                self._gates.append((gq, G))
+                self._record(gq, G)
 
     def apply_matrix(self, targets, mat, controls=()) -> None:
         call = planner.MatrixCall(list(targets), list(controls), numpy.ascontiguousarray(mat, dtype=numpy.complex128))
@@ -118,23 +173,30 @@ class FusedBackend(GpuBackend):
             self.flush()
             GpuBackend.apply_matrix(self, targets, mat, controls)
         else:
-            self._gates.append((gq, G))
+            self._record(gq, G)
 
     def apply_product(self, targets, mats) -> None:
         for q, m in zip(targets, mats):
-            self._gates.append(((q,), numpy.ascontiguousarray(m, dtype=numpy.complex128).reshape(2, 2)))
+            self._record((q,), numpy.ascontiguousarray(m, dtype=numpy.complex128).reshape(2, 2))
+
+    def _record(self, gq, G) -> None:
+        self._fuser.add(gq, G)
+        if self.streaming:
+            k = self._fuser.final_count()
+            if k:
+                self.launch_blocks([b.matrix() for b in self._fuser.take(k)])
 
     def reinit(self, index_groups=(), feed_list=()) -> None:
-        self._gates = []
+        self._fuser = OnlineFusion(self.n, self.fusion_width)
         super().reinit(index_groups, feed_list)
 
     def pending_gates(self) -> int:
-        return len(self._gates)
+        """Recorded gates not launched yet."""
+        return self._fuser.pending
 
     def plan_blocks(self) -> List[Tuple[List[int], numpy.ndarray]]:
-        """Take the recorded gates and return their fused blocks as (qubits, matrix), in order."""
-        gates, self._gates = self._gates, []
-        return [b.matrix() for b in fuse(gates, self.fusion_width)]
+        """Take the recorded gates not launched yet and return their blocks as (qubits, matrix)."""
+        return [b.matrix() for b in self._fuser.take()]
 
     def launch_blocks(self, blocks: Sequence[Tuple[List[int], numpy.ndarray]]) -> int:
         """Apply planned blocks: MFMA kernel for n >= 12, the plain complex64 kernels below."""
@@ -147,8 +209,8 @@ class FusedBackend(GpuBackend):
         return len(blocks)
 
     def flush(self) -> int:
-        """Fuse and apply every recorded gate; returns the number of blocks launched."""
-        if not self._gates:
+        """Apply every recorded gate not launched yet; returns the number of blocks launched."""
+        if not self._fuser.pending:
             return 0
         return self.launch_blocks(self.plan_blocks())
 

@@ -63,3 +63,40 @@ def test_circuit_shape():
     assert algorithmic_bytes_plain(ops2, 28, 16) == 5560 * 2 ** 30
     U = ops[0][0]
     assert numpy.allclose(U.conj().T @ U, numpy.eye(2), atol=1e-15)
+
+
+@pytest.mark.parametrize("n,depth,seed", [(30, 40, 2024), (16, 12, 3), (13, 5, 8)])
+def test_online_fusion_equals_batch_fusion(n, depth, seed):
+    """Gate-by-gate fusion with final blocks taken out as soon as they are final gives the blocks of
+    fuse() on the whole record, in the same order; taken blocks really are final (no later gate
+    would have joined them)."""
+    from qip_amd.fused import OnlineFusion
+    gates = gates_of(layered_random_circuit(n, depth, seed=seed))
+    ref = fuse(gates, 6)
+    of = OnlineFusion(n, 6)
+    got = []
+    for gq, G in gates:
+        of.add(gq, G)
+        k = of.final_count()
+        if k:
+            got.extend(of.take(k))
+    assert of.pending == sum(len(b.gates) for b in of.blocks)
+    got.extend(of.take())
+    assert of.pending == 0
+    assert len(got) == len(ref)
+    for a, b in zip(got, ref):
+        assert a.qubits == b.qubits
+        assert [g[0] for g in a.gates] == [g[0] for g in b.gates]
+    if n == 30:
+        assert len(got) == 128
+
+
+def test_online_fusion_waits_for_untouched_qubits():
+    """While some qubit has not been touched, a later gate on it could join the first block, so
+    nothing is final."""
+    from qip_amd.fused import OnlineFusion
+    of = OnlineFusion(14, 6)
+    for q in range(13):
+        of.add((q,), numpy.eye(2))
+        of.add((q, q + 1) if q < 12 else (q,), numpy.eye(4) if q < 12 else numpy.eye(2))
+    assert of.final_count() == 0

@@ -103,12 +103,18 @@ def test_fused_backend_layered_circuit():
     ops = layered_random_circuit(n, 12, seed=3)
     pl = GpuBackend.make_state(n, [], [])
     fz = FusedBackend.make_state(n, [], [])
+    deferred = FusedBackend.make_state(n, [], [])
+    deferred.streaming = False
     for d in ops:
         pl.kronselect_dot(d)
         fz.kronselect_dot(d)
-    assert fz.pending_gates() == len(ops)
-    got = fz.get_state()                  # flushes
+        deferred.kronselect_dot(d)
+    assert deferred.pending_gates() == len(ops) and deferred.blocks_applied == 0
+    assert fz.blocks_applied > 0 and fz.pending_gates() < len(ops)     # streamed before any read
+    got = fz.get_state()                  # flushes the rest
     assert fz.blocks_applied < len(ops) // 4
+    assert numpy.array_equal(got, deferred.get_state())    # same blocks, same order, same kernels
+    assert deferred.blocks_applied == fz.blocks_applied
     assert infidelity(pl, fz) < 1e-8
     assert numpy.max(numpy.abs(got - pl.get_state())) < 1e-5
     assert abs(fz.total_prob() - 1.0) < 1e-6
