@@ -150,6 +150,11 @@ static inline uint32_t blocks_for(uint64_t work, uint64_t per_block) {
     return (uint32_t)std::max<uint64_t>(b, 1);
 }
 
+// A launch may hold fewer than 2^32 threads along x; per-element kernels loop over their blocks
+// when a state needs more (2^32 bases at n = 33).
+constexpr uint32_t kMaxGridBlocks = 1u << 22;   // 2^30 threads at 256 per workgroup
+static inline uint32_t capped_grid(uint32_t nblocks) { return std::min(nblocks, kMaxGridBlocks); }
+
 // Workgroup order of the gate sweeps. With S = 2^split_bits segments, workgroup b takes block
 // (b mod S) * (nblocks / S) + b / S: the workgroups in flight at any moment work on S regions of
 // the state far apart instead of one narrow window, which keeps more HBM channels and banks busy
@@ -450,12 +455,14 @@ struct DenseArgs {
     uint64_t off[1 << K];
     cx<double> U[(1 << K) * (1 << K)];
     BlockOrder order;
+    uint32_t nblocks;
 };
 
 template <typename R, int K, int ITEMS>
 __global__ __launch_bounds__(256) void k_dense(const DenseArgs<R, K> a) {
     constexpr int D = 1 << K;
-    const uint64_t g0 = ordered_block(blockIdx.x, a.order) * (256 * ITEMS) + threadIdx.x;
+    for (uint32_t blk = blockIdx.x; blk < a.nblocks; blk += gridDim.x) {
+    const uint64_t g0 = ordered_block(blk, a.order) * (256 * ITEMS) + threadIdx.x;
     cx<R> v[ITEMS][D];
     uint64_t idx[ITEMS];
 #pragma unroll
@@ -480,6 +487,7 @@ __global__ __launch_bounds__(256) void k_dense(const DenseArgs<R, K> a) {
                 a.amps[idx[it] + a.off[i]] = narrow<R>(acc);
             }
         }
+    }
     }
 }
 
@@ -515,11 +523,13 @@ struct DiagArgs {
     uint8_t tpos[8];     // state bit of target j (j = 0 is the MSB of the diagonal index)
     cx<double> d[1 << K];
     BlockOrder order;
+    uint32_t nblocks;
 };
 
 template <typename R, int K, int ITEMS>
 __global__ __launch_bounds__(256) void k_diag(const DiagArgs<R, K> a) {
-    const uint64_t g0 = ordered_block(blockIdx.x, a.order) * (256 * ITEMS) + threadIdx.x;
+    for (uint32_t blk = blockIdx.x; blk < a.nblocks; blk += gridDim.x) {
+    const uint64_t g0 = ordered_block(blk, a.order) * (256 * ITEMS) + threadIdx.x;
     cx<R> v[ITEMS];
     uint64_t idx[ITEMS];
 #pragma unroll
@@ -536,6 +546,7 @@ __global__ __launch_bounds__(256) void k_diag(const DiagArgs<R, K> a) {
             for (int b = 0; b < K; ++b) j = (j << 1) | (uint32_t)((idx[it] >> a.tpos[b]) & 1ull);
             a.amps[idx[it]] = narrow<R>(cmul(widen(v[it]), a.d[j]));
         }
+    }
     }
 }
 
@@ -642,7 +653,8 @@ static int launch_dense_reg(qipgpu_state *s, const std::vector<int> &tbits, uint
     constexpr int ITEMS = 1;  // one base per thread measured fastest (large grids keep HBM busy)
     const uint32_t nb = blocks_for(a.nbase, 256 * ITEMS);
     a.order = block_order(s->len * s->amp_bytes, s->amp_bytes, nb);
-    hipLaunchKernelGGL((k_dense<R, K, ITEMS>), dim3(nb), dim3(256), 0, s->stream, a);
+    a.nblocks = nb;
+    hipLaunchKernelGGL((k_dense<R, K, ITEMS>), dim3(capped_grid(nb)), dim3(256), 0, s->stream, a);
     QIP_LAUNCHED();
     return QIPGPU_OK;
 }
@@ -694,7 +706,8 @@ static int launch_diag(qipgpu_state *s, const std::vector<int> &tbits, uint64_t 
     constexpr int ITEMS = 4;
     const uint32_t nb = blocks_for(a.count, 256 * ITEMS);
     a.order = block_order(s->len * s->amp_bytes, s->amp_bytes, nb);
-    hipLaunchKernelGGL((k_diag<R, K, ITEMS>), dim3(nb), dim3(256), 0, s->stream, a);
+    a.nblocks = nb;
+    hipLaunchKernelGGL((k_diag<R, K, ITEMS>), dim3(capped_grid(nb)), dim3(256), 0, s->stream, a);
     QIP_LAUNCHED();
     return QIPGPU_OK;
 }
@@ -812,11 +825,11 @@ struct FuncArgs {
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_func_apply(cx<R> *amps, const int32_t *__restrict__ table, FuncArgs a) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nthreads) return;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < a.nthreads; t += stride) {
     const uint64_t x = t >> a.rbits;
     const uint32_t y = (uint32_t)table[x] & a.ymask;
-    if (y == 0) return;
+    if (y == 0) continue;
     const uint64_t rest = t & ((1ull << a.rbits) - 1ull);
     const int hb = 31 - __clz(y);
     const uint64_t qlow = rest & ((1ull << (a.n2 - 1)) - 1ull);
@@ -827,6 +840,7 @@ __global__ __launch_bounds__(256) void k_func_apply(cx<R> *amps, const int32_t *
     const cx<R> va = amps[ia], vb = amps[ib];
     amps[ia] = vb;
     amps[ib] = va;
+    }
 }
 
 namespace {
@@ -860,7 +874,7 @@ static int func_apply_dev(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, 
     for (int q = 0; q < n; ++q)
         if (!(used & (1ull << q))) other.push_back(q);
     for (int b = 0; b < a.nother; ++b) a.rpos[b] = (uint8_t)(n - 1 - other[a.nother - 1 - b]);
-    const uint32_t nb = blocks_for(a.nthreads, 256);
+    const uint32_t nb = capped_grid(blocks_for(a.nthreads, 256));
     if (s->dtype == QIPGPU_C128)
         hipLaunchKernelGGL(k_func_apply<double>, dim3(nb), dim3(256), 0, s->stream, (cx<double> *)s->amps, d_table, a);
     else
