@@ -8,7 +8,7 @@
 //   * "Tiled states": gates fused greedily into <= fusion_width-qubit blocks (the precision
 //     form's rule). Block j of every tiled state belongs to wave j. Per wave:
 //       k_batch_build -- one workgroup per block: the 64x64 operator (block qubits padded to six
-//         with the lowest free state bits) composed from its gates in FP64 in LDS, split into BF16
+//         with free state bits, tile6.h) composed from its gates in FP64 in LDS, split into BF16
 //         hi/lo MFMA fragments. Builds run on a second stream, one wave ahead, into two
 //         alternating fragment sets, so they overlap the previous wave's tile passes;
 //       k_batch_scales -- the renormalisation factor of each block from the FP64 norm partials of
@@ -607,22 +607,18 @@ int build_plan(const std::vector<int32_t> &ns, const std::vector<uint64_t> &offs
         for (int32_t i : tiled) {
             if (w >= bmask[i].size()) continue;
             const int n = ns[i];
-            // six target state bits: the block's qubits, padded with the lowest free state bits
+            // tile bits: the block's qubits padded to six target bits, and six column bits
             uint64_t tmask = 0;
             const uint64_t qm = bmask[i][w];
             for (int q = 0; q < n; ++q)
                 if (qm >> q & 1) tmask |= 1ull << (n - 1 - q);
-            for (int b = 0; __builtin_popcountll(tmask) < 6; ++b) tmask |= 1ull << b;
-            int ts[6], nt = 0, rank_of[64];
-            for (int b = 0; b < n; ++b)
-                if (tmask >> b & 1) {
-                    rank_of[b] = nt;
-                    ts[nt++] = b;
-                }
+            int ts[6], ls[6], rank_of[64];
+            choose_tile_bits(n, tmask, ts, ls);
+            for (int b = 0; b < 6; ++b) rank_of[ts[b]] = b;
             BBlock bk;
             std::memset(&bk, 0, sizeof(bk));
             bk.amp_off = offs[i];
-            bk.a = make_tile_args(n, ts);
+            bk.a = make_tile_args(n, ts, ls);
             bk.gate0 = (uint32_t)p.bgates.size();
             bk.ngates = (uint32_t)bgl[i][w].size();
             bk.frag = slot++;

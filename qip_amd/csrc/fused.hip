@@ -156,22 +156,20 @@ int qipgpu_fused_apply(qipgpu_state *s, int32_t k, const int32_t *targets, const
     int rc = check_qubits(s, k, targets, "fused block qubits");
     if (rc) return rc;
     const int n = s->n;
-    // pad the block to six target bits with the lowest free state bits
-    std::vector<int> tbits;  // state bits in the caller's matrix order (MSB first)
+    // tile bits: the targets padded to six, and six column bits (choose_tile_bits)
+    std::vector<int> tbits;  // state bits in the caller's matrix order (MSB first), then padding
     uint64_t tmask = 0;
     for (int j = 0; j < k; ++j) {
         tbits.push_back(n - 1 - targets[j]);
         tmask |= 1ull << (n - 1 - targets[j]);
     }
-    for (int b = 0; (int)tbits.size() < 6; ++b)
-        if (!(tmask >> b & 1)) {
-            tbits.push_back(b);
-            tmask |= 1ull << b;
-        }
+    int ts[6], ls[6];
+    choose_tile_bits(n, tmask, ts, ls);
+    for (int b = 0; b < 6; ++b)
+        if (!(tmask >> ts[b] & 1)) tbits.push_back(ts[b]);
     // row index r: bit b on the b-th lowest T bit. Map r -> index of the padded caller matrix
     // (caller bits MSB-first over tbits[0..k-1], padding bits are identity).
-    std::vector<int> tsorted(tbits);
-    std::sort(tsorted.begin(), tsorted.end());
+    std::vector<int> tsorted(ts, ts + 6);
     int rowbit_of_tbit[64];
     for (int b = 0; b < 6; ++b) rowbit_of_tbit[tsorted[b]] = b;
     auto caller_index = [&](int r, int *pad) {
@@ -200,9 +198,7 @@ int qipgpu_fused_apply(qipgpu_state *s, int32_t k, const int32_t *targets, const
     rc = ring_upload(s, frag.data(), kFragBytes, &dfrag);
     if (rc) return rc;
 
-    int ts[6];
-    for (int b = 0; b < 6; ++b) ts[b] = tsorted[b];
-    const FusedArgs a = make_tile_args(n, ts);
+    const FusedArgs a = make_tile_args(n, ts, ls);
 
     const uint32_t grid = fused_grid(a.ntiles);
     rc = ensure_fused_state(s, grid);

@@ -230,27 +230,48 @@ inline float bf16_value(uint16_t b) {
     return f;
 }
 
-// Tile arguments for six target state bits `tsorted` (ascending, distinct, < n, n >= 12): L is the
-// six lowest state bits outside T, P = T u L ascending.
-inline FusedArgs make_tile_args(int n, const int (&tsorted)[6]) {
+// Tile bits of a block whose real target state bits are `tmask` (1 to 6 bits, n >= 12). The 12 tile
+// bits P are the targets plus the first free bits in the order 0..5, 12..17, 6..11, 18..n-1; the
+// targets are padded to six with the first of those extra bits (identity rows of the operator) and
+// the other six are the column bits L. Keeping the six lowest state bits in P makes every run of
+// the tile 512 B; the next preference puts the remaining bits 2^12 amplitudes apart, the pattern
+// measured fastest at n = 30 (per block: P = 0..5 + 12..17 3.08 ms, P = 0..11 3.45 ms,
+// P = 0..5 + 24..29 3.58 ms; targets {0,1,20..23}: columns {2..7} 3.63 ms, {2..5,12,13} 3.25 ms).
+inline void choose_tile_bits(int n, uint64_t tmask, int (&T)[6], int (&L)[6]) {
+    int order[64], no = 0;
+    for (int b = 0; b < 6; ++b) order[no++] = b;
+    for (int b = 12; b < 18; ++b) order[no++] = b;
+    for (int b = 6; b < 12; ++b) order[no++] = b;
+    for (int b = 18; b < 64; ++b) order[no++] = b;
+    int nt = 0, nl = 0;
+    for (int b = 0; b < 64; ++b)
+        if (tmask >> b & 1) T[nt++] = b;
+    for (int i = 0; i < no && (nt < 6 || nl < 6); ++i) {
+        const int b = order[i];
+        if (b >= n || (tmask >> b & 1)) continue;
+        if (nt < 6) T[nt++] = b;
+        else L[nl++] = b;
+    }
+    std::sort(T, T + 6);
+    std::sort(L, L + 6);
+}
+
+// Tile arguments for target bits T and column bits L (each six distinct bits, ascending, disjoint,
+// below n): P = T u L ascending, u bit i <-> state bit P[i].
+inline FusedArgs make_tile_args(int n, const int (&T)[6], const int (&L)[6]) {
     FusedArgs a;
     std::memset(&a, 0, sizeof(a));
     a.ntiles = 1ull << (n - kTileBits);
-    uint64_t tmask = 0;
-    for (int b = 0; b < 6; ++b) tmask |= 1ull << tsorted[b];
-    int lbits[6], nl = 0;
-    for (int b = 0; nl < 6; ++b)
-        if (!(tmask >> b & 1)) lbits[nl++] = b;
     int pbits[12];
     for (int b = 0; b < 6; ++b) {
-        pbits[b] = tsorted[b];
-        pbits[6 + b] = lbits[b];
+        pbits[b] = T[b];
+        pbits[6 + b] = L[b];
     }
     std::sort(pbits, pbits + 12);
     for (int i = 0; i < 12; ++i) a.pbits[i] = (uint8_t)pbits[i];
     for (int b = 0; b < 6; ++b) {
-        a.tu[b] = (uint8_t)(std::find(pbits, pbits + 12, tsorted[b]) - pbits);
-        a.lu[b] = (uint8_t)(std::find(pbits, pbits + 12, lbits[b]) - pbits);
+        a.tu[b] = (uint8_t)(std::find(pbits, pbits + 12, T[b]) - pbits);
+        a.lu[b] = (uint8_t)(std::find(pbits, pbits + 12, L[b]) - pbits);
     }
     return a;
 }
