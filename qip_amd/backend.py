@@ -213,6 +213,7 @@ def func_table(func, n_reg1: int, cache: bool = True) -> _FuncTable:
 
 
 PRODUCT_MAX = 12            # single-qubit factors per product sweep (QIPGPU_MAX_PRODUCT)
+PRODUCT_LOW_BITS = 4        # lowest state bits every product sweep keeps in its tile
 PRODUCT_MIN_QUBITS = 12     # the product sweep tiles 12 state bits
 
 
@@ -340,10 +341,18 @@ class GpuBackend(StateType):
             for call in planner.ProductCall(targets, mats).matrix_calls():
                 self.apply_matrix(call.targets, call.mat)
             return
-        for i in range(0, len(targets), PRODUCT_MAX):
-            t = _capi.i32(targets[i:i + PRODUCT_MAX])
-            m = numpy.ascontiguousarray(numpy.stack([numpy.asarray(x, dtype=numpy.complex128).reshape(2, 2)
-                                                     for x in mats[i:i + PRODUCT_MAX]]))
+        # the factors commute; one sweep keeps the 4 lowest state bits in its tile, so it takes
+        # the factors on those bits and at most 8 others (the engine splits a call the same way)
+        low = [j for j, q in enumerate(targets) if self.n - 1 - q < PRODUCT_LOW_BITS]
+        high = [j for j, q in enumerate(targets) if self.n - 1 - q >= PRODUCT_LOW_BITS]
+        step = PRODUCT_MAX - PRODUCT_LOW_BITS
+        chunks = [low + high[:step]] + [high[i:i + step] for i in range(step, len(high), step)]
+        for chunk in chunks:
+            if not chunk:
+                continue
+            t = _capi.i32([targets[j] for j in chunk])
+            m = numpy.ascontiguousarray(numpy.stack([numpy.asarray(mats[j], dtype=numpy.complex128).reshape(2, 2)
+                                                     for j in chunk]))
             _capi.check(self._lib.qipgpu_apply_product(self._h, len(t), _capi.ptr(t), _capi.ptr(m.view(numpy.float64))),
                         "apply_product")
 

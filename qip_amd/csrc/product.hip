@@ -4,7 +4,9 @@
 // A descriptor such as H on a whole register (qip/operators.py:97-99) is one operator in the
 // reference, applied in one cdot_loop sweep. Here its single-qubit factors commute and are applied
 // tile by tile: the state is cut into tiles of 4096 amplitudes spanning the target bits plus the
-// lowest free bits (12 tile bits P, always including state bit 0). A workgroup loads a tile in
+// lowest free bits (12 tile bits P). Every pass keeps the four lowest state bits in P, so a tile is
+// read in runs of at least 16 amplitudes (256 B for complex128): a pass takes at most 8 targets
+// above bit 3, and a product with more is applied in several passes. A workgroup loads a tile in
 // address order (u = tile bits ascending) into LDS as FP64, applies the gates three target bits
 // per round (each thread takes groups of 8 amplitudes that differ only in the round's bits and
 // applies the round's 2x2 gates in registers), and writes the tile back once: one HBM read and one
@@ -156,16 +158,12 @@ uint32_t product_grid(uint64_t ntiles) {
     return (uint32_t)std::min<uint64_t>(ntiles, (uint64_t)cus * kBlocksPerCU * 4);
 }
 
-}  // namespace
+constexpr int kLowBits = 4;         // state bits every pass keeps in its tile
+constexpr int kHighPerPass = kTileBits - kLowBits;
 
-extern "C" int qipgpu_apply_product(qipgpu_state *s, int32_t k, const int32_t *targets, const double *mats) {
-    if (k < 1 || k > QIPGPU_MAX_PRODUCT) return set_error(QIPGPU_EINVAL, "product of %d single-qubit gates (1..%d)", k,
-                                                         QIPGPU_MAX_PRODUCT);
-    if (s->n < kTileBits) return set_error(QIPGPU_EUNSUP, "product sweep needs n >= %d (n=%d)", kTileBits, s->n);
-    int rc = check_qubits(s, k, targets, "product qubits");
-    if (rc) return rc;
-    rc = settle(s);
-    if (rc) return rc;
+// One pass: the k single-qubit gates (k <= 12, all targets of at most kHighPerPass above the low
+// bits) in one sweep.
+int product_pass(qipgpu_state *s, int32_t k, const int32_t *targets, const double *mats) {
     const int n = s->n;
     uint64_t tmask = 0;
     std::vector<int> tbits(k);
@@ -208,5 +206,37 @@ extern "C" int qipgpu_apply_product(qipgpu_state *s, int32_t k, const int32_t *t
     else
         hipLaunchKernelGGL(k_product<float>, dim3(grid), dim3(256), 0, s->stream, (cx<float> *)s->amps, a);
     QIP_LAUNCHED();
+    return QIPGPU_OK;
+}
+
+}  // namespace
+
+extern "C" int qipgpu_apply_product(qipgpu_state *s, int32_t k, const int32_t *targets, const double *mats) {
+    if (k < 1 || k > QIPGPU_MAX_PRODUCT) return set_error(QIPGPU_EINVAL, "product of %d single-qubit gates (1..%d)", k,
+                                                         QIPGPU_MAX_PRODUCT);
+    if (s->n < kTileBits) return set_error(QIPGPU_EUNSUP, "product sweep needs n >= %d (n=%d)", kTileBits, s->n);
+    int rc = check_qubits(s, k, targets, "product qubits");
+    if (rc) return rc;
+    rc = settle(s);
+    if (rc) return rc;
+    // the factors commute: the first pass takes every factor on the low bits and up to
+    // kHighPerPass others, each further pass up to kHighPerPass more
+    std::vector<int> low, high;
+    for (int j = 0; j < k; ++j) (s->n - 1 - targets[j] < kLowBits ? low : high).push_back(j);
+    size_t next = 0;
+    bool first = true;
+    while (first || next < high.size()) {
+        std::vector<int> pick(first ? low : std::vector<int>());
+        for (int c = 0; c < kHighPerPass && next < high.size(); ++c) pick.push_back(high[next++]);
+        first = false;
+        if (pick.empty()) continue;
+        std::vector<int32_t> t;
+        std::vector<double> m;
+        for (int j : pick) {
+            t.push_back(targets[j]);
+            m.insert(m.end(), mats + 8 * j, mats + 8 * j + 8);
+        }
+        if ((rc = product_pass(s, (int32_t)t.size(), t.data(), m.data()))) return rc;
+    }
     return QIPGPU_OK;
 }
