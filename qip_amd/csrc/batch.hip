@@ -859,7 +859,8 @@ int make_plan(qipgpu_batch *b, const Tuning &t) {
         b->plan_mem.push_back(f);
         b->d_frags = (bf16x8 *)f;
     }
-    if (t.use_graph) {
+    const bool work = !p.smalls.empty() || !p.blocks.empty();
+    if (t.use_graph && work) {
         QIP_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         rc = enqueue_run(b);
         hipGraph_t g = nullptr;
@@ -876,7 +877,10 @@ int make_plan(qipgpu_batch *b, const Tuning &t) {
 }
 
 int run_batch(qipgpu_batch *b) {
-    if (!b->loaded) return QIPGPU_OK;
+    if (!b->loaded) {
+        if (b->gl.state.empty()) return QIPGPU_OK;   // never given gates: nothing to run
+        return set_error(QIPGPU_EINVAL, "the batch has no usable plan (its last load or re-tune failed)");
+    }
     if (b->exec) {
         QIP_HIP(hipGraphLaunch(b->exec, b->stream));
         return QIPGPU_OK;

@@ -234,3 +234,24 @@ def test_batch_errors():
     with pytest.raises(ValueError):
         b.load(batch.PackedCircuits.from_gates([4, 12], [[], []]))
     b.close()
+
+
+def test_batch_without_gates_and_rejected_retune():
+    """A batch loaded with no gates runs as a no-op; a rejected tuning leaves the loaded plan in use."""
+    ns = [5, 14]
+    b = batch.GpuBatch(ns)
+    b.load(batch.PackedCircuits.from_gates(ns, [[], []]))
+    b.run()
+    b.synchronize()
+    e = numpy.zeros(2 ** 14, dtype=numpy.complex64)
+    e[0] = 1
+    assert numpy.array_equal(b.get_state(1), e)
+    gates = [[((0,), rand_u(2))], [((3, 7), rand_u(4)), ((13,), rand_u(2))]]
+    b.load(batch.PackedCircuits.from_gates(ns, gates))
+    with pytest.raises(ValueError):
+        b.set_tuning({"lds_max_qubits": 20})
+    b.reset()
+    b.run()
+    for i, n in enumerate(ns):
+        check_state(n, oracle_run(n, gates[i]), b.get_state(i), n > b.tuning["lds_max_qubits"])
+    b.close()
