@@ -217,6 +217,18 @@ PRODUCT_LOW_BITS = 4        # lowest state bits every product sweep keeps in its
 PRODUCT_MIN_QUBITS = 12     # the product sweep tiles 12 state bits
 
 
+def product_passes(n: int, targets: Sequence[int]):
+    """Split the factors of a product of single-qubit gates (on distinct qubits; they commute)
+    into sweeps: every sweep keeps the PRODUCT_LOW_BITS lowest state bits in its tile, so it takes
+    the factors on those bits and at most PRODUCT_MAX - PRODUCT_LOW_BITS others. Returns lists of
+    indices into `targets`, the first holding every low-bit factor."""
+    low = [j for j, q in enumerate(targets) if n - 1 - q < PRODUCT_LOW_BITS]
+    high = [j for j, q in enumerate(targets) if n - 1 - q >= PRODUCT_LOW_BITS]
+    step = PRODUCT_MAX - PRODUCT_LOW_BITS
+    chunks = [low + high[:step]] + [high[i:i + step] for i in range(step, len(high), step)]
+    return [c for c in chunks if c]
+
+
 class GpuBackend(StateType):
     """Plain MI355X backend: one in-place HIP sweep per connected component of each operator,
     complex128 (the drop-in) or complex64 state, FP64 reductions."""
@@ -341,15 +353,7 @@ class GpuBackend(StateType):
             for call in planner.ProductCall(targets, mats).matrix_calls():
                 self.apply_matrix(call.targets, call.mat)
             return
-        # the factors commute; one sweep keeps the 4 lowest state bits in its tile, so it takes
-        # the factors on those bits and at most 8 others (the engine splits a call the same way)
-        low = [j for j, q in enumerate(targets) if self.n - 1 - q < PRODUCT_LOW_BITS]
-        high = [j for j, q in enumerate(targets) if self.n - 1 - q >= PRODUCT_LOW_BITS]
-        step = PRODUCT_MAX - PRODUCT_LOW_BITS
-        chunks = [low + high[:step]] + [high[i:i + step] for i in range(step, len(high), step)]
-        for chunk in chunks:
-            if not chunk:
-                continue
+        for chunk in product_passes(self.n, targets):
             t = _capi.i32([targets[j] for j in chunk])
             m = numpy.ascontiguousarray(numpy.stack([numpy.asarray(mats[j], dtype=numpy.complex128).reshape(2, 2)
                                                      for j in chunk]))

@@ -128,3 +128,21 @@ def test_tabulate():
     with pytest.raises(OverflowError):
         tabulate(lambda x: 2 ** 33, 2)
     assert list(tabulate(numpy.arange(4)[::-1], 2)) == [3, 2, 1, 0]
+
+
+@pytest.mark.parametrize("n,targets", [(24, list(range(23))), (30, list(range(30))), (12, [0, 11, 5]),
+                                       (16, [15, 14, 13, 12]), (20, [0, 1, 2]), (13, list(range(13)))])
+def test_product_passes_keep_low_bits(n, targets):
+    """Every product sweep holds at most 8 factors above state bit 3 and at most 12 in all; the
+    passes cover each factor once; the low-bit factors share the first pass."""
+    from qip_amd.backend import PRODUCT_LOW_BITS, PRODUCT_MAX, product_passes
+    passes = product_passes(n, targets)
+    seen = sorted(j for p in passes for j in p)
+    assert seen == list(range(len(targets)))
+    low = {j for j, q in enumerate(targets) if n - 1 - q < PRODUCT_LOW_BITS}
+    assert low <= set(passes[0])
+    for p in passes:
+        assert len(p) <= PRODUCT_MAX
+        assert sum(1 for j in p if j not in low) <= PRODUCT_MAX - PRODUCT_LOW_BITS
+    high = len(targets) - len(low)
+    assert len(passes) == max(1, -(-high // (PRODUCT_MAX - PRODUCT_LOW_BITS)))
