@@ -145,6 +145,14 @@ int qipgpu_reduce_measure(qipgpu_state *s, int32_t k, const int32_t *indices, do
  * multiplied by the pending FP64 renormalisation factor left by the previous fused call; the
  * norm of the result is accumulated in FP64 and becomes the next pending factor. */
 int qipgpu_fused_apply(qipgpu_state *s, int32_t k, const int32_t *targets, const double *mat);
+/* Host only (no GPU call): the 2^k x 2^k matrix (row-major complex128) of an ordered list of gates
+ * on the k <= 6 distinct qubits `qubits` (first = MSB of the result's index), out = G_last ... G_1.
+ * Gate g acts on its gate_k[g] qubits gate_qubits[6 g ..] (all among `qubits`, first = MSB of its
+ * own matrix) with the 2^gate_k x 2^gate_k matrix that follows the previous gates' in `mats`: the
+ * operator of a fused block, i.e. the product of the consecutive cdot_loop operators
+ * (kronprod.pyx:41-200) it replaces. */
+int qipgpu_compose(int32_t k, const int32_t *qubits, int32_t n_gates, const int32_t *gate_k,
+                   const int32_t *gate_qubits, const double *mats, double *out);
 /* Apply the pending renormalisation factor (if any) with one scaling sweep. */
 int qipgpu_fused_finish(qipgpu_state *s);
 /* Norm of the state after the last fused call, before its renormalisation, in FP64. */

@@ -4,6 +4,7 @@
 // single-state kernel, the FP64 renormalisation and the C entry points.
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <vector>
 
@@ -213,6 +214,184 @@ int qipgpu_fused_apply(qipgpu_state *s, int32_t k, const int32_t *targets, const
     hipLaunchKernelGGL(k_fused_norm, dim3(1), dim3(256), 0, s->stream, (const double *)(s->d_fused + 4), grid, s->d_fused);
     QIP_LAUNCHED();
     s->fused_pending = true;
+    return QIPGPU_OK;
+}
+
+int qipgpu_compose(int32_t k, const int32_t *qubits, int32_t n_gates, const int32_t *gate_k,
+                   const int32_t *gate_qubits, const double *mats, double *out) {
+    if (k < 1 || k > 6) return set_error(QIPGPU_EINVAL, "block of %d qubits (1..6)", k);
+    if (n_gates < 0) return set_error(QIPGPU_EINVAL, "negative gate count");
+    if (!qubits || !out || (n_gates > 0 && (!gate_k || !gate_qubits || !mats)))
+        return set_error(QIPGPU_EINVAL, "null argument");
+    for (int i = 0; i < k; ++i)
+        for (int j = 0; j < i; ++j)
+            if (qubits[i] == qubits[j]) return set_error(QIPGPU_EINVAL, "block qubit %d repeated", qubits[i]);
+    // bit position of each gate qubit in the block index (block qubit i is bit k-1-i)
+    std::vector<int8_t> gbits((size_t)n_gates * 6);
+    for (int32_t g = 0; g < n_gates; ++g) {
+        const int kg = gate_k[g];
+        if (kg < 1 || kg > k) return set_error(QIPGPU_EINVAL, "gate %d on %d qubits of a %d-qubit block", g, kg, k);
+        uint32_t mask = 0;
+        for (int j = 0; j < kg; ++j) {
+            const int q = gate_qubits[6 * g + j];
+            int pos = -1;
+            for (int i = 0; i < k; ++i)
+                if (qubits[i] == q) pos = i;
+            if (pos < 0) return set_error(QIPGPU_EINVAL, "gate %d: qubit %d is not a block qubit", g, q);
+            if (mask >> (k - 1 - pos) & 1) return set_error(QIPGPU_EINVAL, "gate %d: qubit %d repeated", g, q);
+            mask |= 1u << (k - 1 - pos);
+            gbits[6 * g + j] = (int8_t)(k - 1 - pos);
+        }
+    }
+    // Single-qubit gates commute with every gate on other qubits, so each one waits on its bit (the
+    // product of the waiting ones kept as a 2x2) until a wider gate touches that bit, where it is
+    // folded into that gate's small matrix; the rest meet the product at the end. While the product is
+    // still the identity, a gate is written into it directly and the final single-qubit gates as
+    // their Kronecker product, so a block of single-qubit gates costs D^2 products and not k 2 D^2.
+    const int D = 1 << k;
+    std::vector<double> mre(D * D, 0.0), mim(D * D, 0.0), tre(D * D), tim(D * D);
+    bool ident = true;
+    double pw[6][8];      // waiting 2x2 per bit, interleaved complex row-major
+    bool has[6] = {false, false, false, false, false, false};
+    double gre[4096], gim[4096];   // a wider gate with its waiting single-qubit gates folded in
+    size_t moff = 0;
+    for (int32_t g = 0; g < n_gates; ++g) {
+        const int kg = gate_k[g], dg = 1 << kg;
+        const double *G = mats + 2 * moff;   // interleaved complex, row-major dg x dg
+        moff += (size_t)dg * dg;
+        const int8_t *bits = &gbits[6 * g];
+        if (kg == 1) {
+            double *P = pw[bits[0]];
+            if (!has[bits[0]]) {
+                std::memcpy(P, G, sizeof(double) * 8);
+                has[bits[0]] = true;
+            } else {   // P = G P
+                double r[8];
+                for (int i = 0; i < 2; ++i)
+                    for (int j = 0; j < 2; ++j) {
+                        double re = 0.0, im = 0.0;
+                        for (int e = 0; e < 2; ++e) {
+                            const double ar = G[2 * (2 * i + e)], ai = G[2 * (2 * i + e) + 1];
+                            const double br = P[2 * (2 * e + j)], bi = P[2 * (2 * e + j) + 1];
+                            re += ar * br - ai * bi;
+                            im += ar * bi + ai * br;
+                        }
+                        r[2 * (2 * i + j)] = re;
+                        r[2 * (2 * i + j) + 1] = im;
+                    }
+                std::memcpy(P, r, sizeof(r));
+            }
+            continue;
+        }
+        for (int i = 0; i < dg * dg; ++i) {
+            gre[i] = G[2 * i];
+            gim[i] = G[2 * i + 1];
+        }
+        uint32_t mask = 0;
+        for (int j = 0; j < kg; ++j) {
+            mask |= 1u << bits[j];
+            if (!has[bits[j]]) continue;
+            // G <- G (I x P x I) with P on gate index bit kg-1-j: mixes column pairs c0 / c1
+            const double *P = pw[bits[j]];
+            const int cb = 1 << (kg - 1 - j);
+            for (int r = 0; r < dg; ++r)
+                for (int c0 = 0; c0 < dg; ++c0) {
+                    if (c0 & cb) continue;
+                    const int c1 = c0 | cb;
+                    const double ar = gre[r * dg + c0], ai = gim[r * dg + c0];
+                    const double br = gre[r * dg + c1], bi = gim[r * dg + c1];
+                    gre[r * dg + c0] = ar * P[0] - ai * P[1] + br * P[4] - bi * P[5];
+                    gim[r * dg + c0] = ar * P[1] + ai * P[0] + br * P[5] + bi * P[4];
+                    gre[r * dg + c1] = ar * P[2] - ai * P[3] + br * P[6] - bi * P[7];
+                    gim[r * dg + c1] = ar * P[3] + ai * P[2] + br * P[7] + bi * P[6];
+                }
+            has[bits[j]] = false;
+        }
+        int dep[64];   // gate index e -> its bits placed in the block index
+        for (int e = 0; e < dg; ++e) {
+            int d = 0;
+            for (int j = 0; j < kg; ++j)
+                if ((e >> (kg - 1 - j)) & 1) d |= 1 << bits[j];
+            dep[e] = d;
+        }
+        auto sub = [&](int x) {
+            int v = 0;
+            for (int j = 0; j < kg; ++j) v = (v << 1) | ((x >> bits[j]) & 1);
+            return v;
+        };
+        if (ident) {   // product = this gate embedded in the block
+            for (int r = 0; r < D; ++r) {
+                const int sr = sub(r), base = r & ~(int)mask;
+                for (int e = 0; e < dg; ++e) {
+                    mre[r * D + (base | dep[e])] = gre[sr * dg + e];
+                    mim[r * D + (base | dep[e])] = gim[sr * dg + e];
+                }
+            }
+            ident = false;
+            continue;
+        }
+        // row r of the product takes gate row sub(r) against the rows base(r) | dep[e] of m
+        for (int r = 0; r < D; ++r) {
+            const int sr = sub(r), base = r & ~(int)mask;
+            double *__restrict__ ore = &tre[r * D], *__restrict__ oim = &tim[r * D];
+            for (int c = 0; c < D; ++c) ore[c] = oim[c] = 0.0;
+            for (int e = 0; e < dg; ++e) {
+                const double gr = gre[sr * dg + e], gi = gim[sr * dg + e];
+                if (gr == 0.0 && gi == 0.0) continue;
+                const double *__restrict__ are = &mre[(base | dep[e]) * D];
+                const double *__restrict__ aim = &mim[(base | dep[e]) * D];
+                for (int c = 0; c < D; ++c) {
+                    ore[c] += gr * are[c] - gi * aim[c];
+                    oim[c] += gr * aim[c] + gi * are[c];
+                }
+            }
+        }
+        mre.swap(tre);
+        mim.swap(tim);
+    }
+    if (ident) {   // Kronecker product of the waiting single-qubit gates, MSB first, by doubling
+        mre[0] = 1.0;
+        mim[0] = 0.0;
+        for (int b = k - 1, d = 1; b >= 0; --b, d *= 2) {
+            double P[8] = {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0};
+            if (has[b]) std::memcpy(P, pw[b], sizeof(P));
+            const int d2 = 2 * d;
+            for (int r = 0; r < d; ++r)
+                for (int c = 0; c < d; ++c) {
+                    const double xr = mre[r * d + c], xi = mim[r * d + c];
+                    for (int i = 0; i < 2; ++i)
+                        for (int j = 0; j < 2; ++j) {
+                            const double pr = P[2 * (2 * i + j)], pi = P[2 * (2 * i + j) + 1];
+                            tre[(2 * r + i) * d2 + 2 * c + j] = xr * pr - xi * pi;
+                            tim[(2 * r + i) * d2 + 2 * c + j] = xr * pi + xi * pr;
+                        }
+                }
+            mre.swap(tre);
+            mim.swap(tim);
+        }
+    } else {
+        for (int b = 0; b < k; ++b) {   // product <- P_b product: mixes row pairs r0 / r1
+            if (!has[b]) continue;
+            const double *P = pw[b];
+            for (int r0 = 0; r0 < D; ++r0) {
+                if (r0 >> b & 1) continue;
+                const int r1 = r0 | (1 << b);
+                double *__restrict__ a_re = &mre[r0 * D], *__restrict__ a_im = &mim[r0 * D];
+                double *__restrict__ b_re = &mre[r1 * D], *__restrict__ b_im = &mim[r1 * D];
+                for (int c = 0; c < D; ++c) {
+                    const double ar = a_re[c], ai = a_im[c], br = b_re[c], bi = b_im[c];
+                    a_re[c] = P[0] * ar - P[1] * ai + P[2] * br - P[3] * bi;
+                    a_im[c] = P[0] * ai + P[1] * ar + P[2] * bi + P[3] * br;
+                    b_re[c] = P[4] * ar - P[5] * ai + P[6] * br - P[7] * bi;
+                    b_im[c] = P[4] * ai + P[5] * ar + P[6] * bi + P[7] * br;
+                }
+            }
+        }
+    }
+    for (int i = 0; i < D * D; ++i) {
+        out[2 * i] = mre[i];
+        out[2 * i + 1] = mim[i];
+    }
     return QIPGPU_OK;
 }
 

@@ -24,7 +24,7 @@ from typing import List, Sequence, Tuple
 
 import numpy
 
-from . import planner
+from . import _capi, planner
 from .backend import GpuBackend, check_windows
 
 FUSED_MIN_QUBITS = 12       # the MFMA kernel tiles 12 state bits
@@ -38,18 +38,34 @@ class Block:
         self.gates = []          # [(qubits tuple MSB-first, dense matrix)]
 
     def matrix(self) -> Tuple[List[int], numpy.ndarray]:
-        """Ordered product of the block's gates on the block qubits (ascending, first = MSB)."""
+        """Ordered product of the block's gates on the block qubits (ascending, first = MSB),
+        composed in FP64 by the library's host routine qipgpu_compose."""
         qs = sorted(self.qubits)
-        k = len(qs)
-        pos = {q: i for i, q in enumerate(qs)}
-        M = numpy.eye(2 ** k, dtype=numpy.complex128).reshape([2] * k + [2 ** k])
-        for gq, G in self.gates:
-            L = len(gq)
-            axes = [pos[q] for q in gq]
-            Gt = G.reshape([2] * (2 * L))
-            M = numpy.tensordot(Gt, M, axes=(list(range(L, 2 * L)), axes))
-            M = numpy.moveaxis(M, list(range(L)), axes)
-        return qs, M.reshape(2 ** k, 2 ** k)
+        return qs, compose(qs, self.gates)
+
+
+def compose(qubits: Sequence[int], gates) -> numpy.ndarray:
+    """G_last ... G_1 as a 2^k x 2^k complex128 matrix on ``qubits`` (first = MSB); each gate is
+    (its qubits MSB-first, all among ``qubits``, its dense matrix)."""
+    k = len(qubits)
+    gk, gq6, mats = [], [], []
+    for g, (gq, G) in enumerate(gates):
+        L = len(gq)
+        if L > 6 or G.size != 1 << (2 * L):
+            raise ValueError("gate {} on qubits {} has a matrix of shape {}".format(g, gq, numpy.shape(G)))
+        gk.append(L)
+        gq6.extend(gq)
+        gq6.extend((0,) * (6 - L))
+        mats.append(G.ravel())
+    flat = numpy.concatenate(mats).astype(numpy.complex128, copy=False) if mats else numpy.zeros(1, numpy.complex128)
+    qs = numpy.array(qubits, dtype=numpy.int32)
+    gk = numpy.array(gk or [0], dtype=numpy.int32)
+    gq6 = numpy.array(gq6 or [0], dtype=numpy.int32)
+    out = numpy.empty((1 << k, 1 << k), dtype=numpy.complex128)
+    # the arrays stay referenced here for the whole call, so their raw addresses are safe to pass
+    _capi.check(_capi.load().qipgpu_compose(k, qs.ctypes.data, len(gates), gk.ctypes.data, gq6.ctypes.data,
+                                            flat.ctypes.data, out.ctypes.data), "compose")
+    return out
 
 
 def controlled_dense(call: planner.MatrixCall) -> Tuple[Tuple[int, ...], numpy.ndarray]:

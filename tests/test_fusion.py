@@ -100,3 +100,85 @@ def test_online_fusion_waits_for_untouched_qubits():
         of.add((q,), numpy.eye(2))
         of.add((q, q + 1) if q < 12 else (q,), numpy.eye(4) if q < 12 else numpy.eye(2))
     assert of.final_count() == 0
+
+
+def _compose_tensordot(qs, gates):
+    """The block product by tensor contraction (independent of the library's loop)."""
+    k = len(qs)
+    pos = {q: i for i, q in enumerate(qs)}
+    M = numpy.eye(2 ** k, dtype=numpy.complex128).reshape([2] * k + [2 ** k])
+    for gq, G in gates:
+        L = len(gq)
+        axes = [pos[q] for q in gq]
+        M = numpy.tensordot(G.reshape([2] * (2 * L)), M, axes=(list(range(L, 2 * L)), axes))
+        M = numpy.moveaxis(M, list(range(L)), axes)
+    return M.reshape(2 ** k, 2 ** k)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_compose_matches_full_operator(seed):
+    """qipgpu_compose (host C++) on random gates of 1..k qubits in any order, against the product of
+    the gates embedded element by element on the block qubits, and against tensor contraction."""
+    from qip_amd.fused import compose
+    rng = numpy.random.default_rng(seed)
+    k = int(rng.integers(1, 7))
+    qs = sorted(rng.choice(40, size=k, replace=False).tolist())
+    gates = []
+    for _ in range(int(rng.integers(0, 12))):
+        L = int(rng.integers(1, k + 1))
+        gq = tuple(int(q) for q in rng.permutation(qs)[:L])
+        G = rng.standard_normal((2 ** L, 2 ** L)) + 1j * rng.standard_normal((2 ** L, 2 ** L))
+        gates.append((gq, G))
+    got = compose(qs, gates)
+    # element by element: <r|G|c> = G[sub(r), sub(c)] when r and c agree off the gate's bits
+    # (block qubit qs[i] is bit k-1-i of the block index, gate qubit j bit L-1-j of G's index)
+    bit = {q: k - 1 - i for i, q in enumerate(qs)}
+    D = 2 ** k
+    want = numpy.eye(D, dtype=complex)
+    for gq, G in gates:
+        mask = sum(1 << bit[q] for q in gq)
+        sub = [sum(((x >> bit[q]) & 1) << (len(gq) - 1 - j) for j, q in enumerate(gq)) for x in range(D)]
+        E = numpy.zeros((D, D), dtype=complex)
+        for r in range(D):
+            for c in range(D):
+                if (r & ~mask) == (c & ~mask):
+                    E[r, c] = G[sub[r], sub[c]]
+        want = E @ want
+    scale = max(1.0, float(numpy.max(numpy.abs(want))))
+    assert numpy.max(numpy.abs(got - want)) < 1e-12 * scale
+    assert numpy.max(numpy.abs(got - _compose_tensordot(qs, gates))) < 1e-12 * scale
+
+
+def test_compose_rejects_bad_blocks():
+    from qip_amd.fused import compose
+    H = numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2)
+    with pytest.raises(ValueError):
+        compose([0, 1], [((2,), H)])              # gate qubit outside the block
+    with pytest.raises(ValueError):
+        compose([0, 0], [((0,), H)])              # repeated block qubit
+    with pytest.raises(ValueError):
+        compose(list(range(7)), [((0,), H)])      # wider than six
+    with pytest.raises(ValueError):
+        compose([0, 1], [((0, 0), numpy.eye(4))])  # repeated gate qubit
+    with pytest.raises(ValueError):
+        compose([0, 1], [((0, 1), H)])            # matrix of the wrong size
+    assert numpy.array_equal(compose([3], []), numpy.eye(2))
+
+
+@pytest.mark.parametrize("k", [1, 3, 6])
+def test_compose_single_qubit_blocks(k):
+    """Blocks of single-qubit gates only (several per qubit, some qubits without one) compose to the
+    Kronecker product of each qubit's ordered 2x2 product."""
+    from qip_amd.fused import compose
+    rng = numpy.random.default_rng(k)
+    qs = list(range(10, 10 + k))
+    gates, per = [], {q: numpy.eye(2, dtype=complex) for q in qs}
+    for _ in range(3 * k):
+        q = int(rng.choice(qs[:-1] if k > 1 else qs))      # the last qubit keeps no gate when k > 1
+        G = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+        gates.append(((q,), G))
+        per[q] = G @ per[q]
+    want = numpy.ones((1, 1), dtype=complex)
+    for q in qs:
+        want = numpy.kron(want, per[q])
+    assert numpy.max(numpy.abs(compose(qs, gates) - want)) < 1e-12 * max(1.0, numpy.max(numpy.abs(want)))
