@@ -60,7 +60,7 @@ struct qipgpu_state {
     bool ring_used[qipgpu::kRingSlots] = {};
     int ring_next = 0;
     // fused path: d_fused[0] = pending renormalisation factor, d_fused[1] = last norm,
-    // d_fused[2..] = per-block norm partials
+    // d_fused[2] = finished-workgroup count of the running block, d_fused[4..] = per-workgroup norm partials
     double *d_fused = nullptr;
     size_t fused_partials = 0;
     bool fused_pending = false;

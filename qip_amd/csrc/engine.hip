@@ -776,8 +776,7 @@ int qipgpu_apply_matrix(qipgpu_state *s, int32_t k, const int32_t *targets, cons
     if (n_ctrl < 0) return set_error(QIPGPU_EINVAL, "negative control count");
     int rc = check_disjoint(s, k, targets, n_ctrl, controls);
     if (rc) return rc;
-    rc = settle(s);
-    if (rc) return rc;
+    // linear in the state: a pending fused renormalisation (a scalar) stays pending
     Analysed an = analyse(k, targets, mat);
     std::vector<int32_t> ctl(controls, controls + n_ctrl);
     return s->dtype == QIPGPU_C128 ? dispatch<double>(s, an, ctl) : dispatch<float>(s, an, ctl);
@@ -788,8 +787,7 @@ int qipgpu_apply_diagonal(qipgpu_state *s, int32_t k, const int32_t *targets, co
     if (k < 0 || k > QIPGPU_MAX_DENSE) return set_error(QIPGPU_EINVAL, "diagonal on %d qubits", k);
     int rc = check_disjoint(s, k, targets, n_ctrl, controls);
     if (rc) return rc;
-    rc = settle(s);
-    if (rc) return rc;
+    // linear in the state: a pending fused renormalisation (a scalar) stays pending
     Analysed an;
     an.targets.assign(targets, targets + k);
     const int D = 1 << k;
@@ -854,8 +852,7 @@ static int func_apply_dev(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, 
     all.insert(all.end(), reg2, reg2 + n_reg2);
     int rc = check_qubits(s, (int32_t)all.size(), all.data(), "func_apply registers");
     if (rc) return rc;
-    rc = settle(s);
-    if (rc) return rc;
+    // linear in the state: a pending fused renormalisation (a scalar) stays pending
     FuncArgs a;
     std::memset(&a, 0, sizeof(a));
     a.n1 = n_reg1;

@@ -19,8 +19,12 @@ namespace {
 
 constexpr int kFusedBlocksPerCU = 2;   // 69 KB of LDS and 212 VGPRs per workgroup
 
+// fstate[0] = scale applied to every amplitude as it is loaded (the previous block's 1/sqrt(norm)),
+// fstate[1] = norm of this block's output, fstate[2] (low word) = count of finished workgroups,
+// partial[grid] = per-workgroup FP64 norm partials. The last workgroup to finish sums the partials in a
+// fixed order, publishes the norm and the next scale, and resets the count.
 __global__ __launch_bounds__(256, 2) void k_fused6(cx<float> *amps, const bf16x8 *__restrict__ wfrag, FusedArgs a,
-                                                   const double *__restrict__ fstate, double *partial) {
+                                                   double *fstate, double *partial) {
     typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f4;
     __shared__ TileLds lds;
     __shared__ double red[4];
@@ -76,28 +80,45 @@ __global__ __launch_bounds__(256, 2) void k_fused6(cx<float> *amps, const bf16x8
     for (int off = 32; off > 0; off >>= 1) nacc += __shfl_down(nacc, off, 64);
     if (lane == 0) red[wave] = nacc;
     __syncthreads();
-    if (tid == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
-}
-
-// Sum the partials in a fixed order; fstate[1] = norm, fstate[0] = 1 / sqrt(norm).
-__global__ __launch_bounds__(256) void k_fused_norm(const double *__restrict__ partial, uint32_t count, double *fstate) {
-    __shared__ double red[256];
+    __shared__ uint32_t is_last;
+    uint32_t *count = reinterpret_cast<uint32_t *>(fstate + 2);
+    if (tid == 0) {
+        __hip_atomic_store(&partial[blockIdx.x], (red[0] + red[1]) + (red[2] + red[3]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        // release the partial, then count this workgroup; acquire the others' if it is the last
+        is_last = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    double *fin = reinterpret_cast<double *>(&lds);   // the tile buffers are free now
     double acc = 0.0;
-    for (uint32_t i = threadIdx.x; i < count; i += 256) acc += partial[i];
-    red[threadIdx.x] = acc;
+    for (uint32_t i = tid; i < gridDim.x; i += 256)
+        acc += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    fin[tid] = acc;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        if (tid < o) fin[tid] += fin[tid + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        fstate[1] = red[0];
-        fstate[0] = 1.0 / sqrt(red[0]);
+    if (tid == 0) {
+        fstate[1] = fin[0];
+        fstate[0] = 1.0 / sqrt(fin[0]);
+        __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
 __global__ void k_set_scale(double *fstate, double v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) fstate[0] = v;
+}
+
+__global__ void k_fused_init(double *fstate) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        fstate[0] = 1.0;
+        fstate[1] = 0.0;
+        fstate[2] = 0.0;   // the finished-workgroup count lives in its low word
+        fstate[3] = 0.0;
+    }
 }
 
 template <typename R>
@@ -121,7 +142,7 @@ int ensure_fused_state(qipgpu_state *s, uint32_t nblocks) {
     }
     QIP_HIP(hipMalloc(&s->d_fused, sizeof(double) * (4 + nblocks)));
     s->fused_partials = nblocks;
-    hipLaunchKernelGGL(k_set_scale, dim3(1), dim3(64), 0, s->stream, s->d_fused, 1.0);
+    hipLaunchKernelGGL(k_fused_init, dim3(1), dim3(64), 0, s->stream, s->d_fused);
     QIP_LAUNCHED();
     return QIPGPU_OK;
 }
@@ -140,7 +161,9 @@ uint32_t fused_grid(uint64_t ntiles) {
 
 namespace qipgpu {
 
-// Apply a pending renormalisation factor before any non-fused use of the state.
+// Apply a pending renormalisation factor before the state is read or written from the host. Gate,
+// diagonal, product and func_apply sweeps are linear, so they commute with the scalar and leave it
+// pending for the next fused block (which applies it as it loads) or the next read.
 int settle(qipgpu_state *s) {
     if (!s->fused_pending) return QIPGPU_OK;
     return qipgpu_fused_finish(s);
@@ -209,9 +232,7 @@ int qipgpu_fused_apply(qipgpu_state *s, int32_t k, const int32_t *targets, const
         QIP_LAUNCHED();
     }
     hipLaunchKernelGGL(k_fused6, dim3(grid), dim3(256), 0, s->stream, (cx<float> *)s->amps, (const bf16x8 *)dfrag, a,
-                       (const double *)s->d_fused, s->d_fused + 4);
-    QIP_LAUNCHED();
-    hipLaunchKernelGGL(k_fused_norm, dim3(1), dim3(256), 0, s->stream, (const double *)(s->d_fused + 4), grid, s->d_fused);
+                       s->d_fused, s->d_fused + 4);
     QIP_LAUNCHED();
     s->fused_pending = true;
     return QIPGPU_OK;

@@ -86,6 +86,32 @@ def test_fused_then_plain_settles():
     assert infidelity(ref, fz) < 1e-9
 
 
+def test_pending_scale_through_linear_sweeps():
+    """Each fused block leaves its 1/sqrt(norm) pending. Gate, product, diagonal and func_apply sweeps
+    are linear and leave it pending; the next fused block applies it as it loads, a read applies it
+    to the whole state. Starting from 2 psi makes a dropped or doubled scale visible."""
+    n = 14
+    psi = rand_state(n)
+    ref = GpuBackend.make_state(n, [list(range(n))], [psi])
+    fz = GpuBackend.make_state(n, [list(range(n))], [2.0 * psi], statetype=numpy.complex64)
+    H = numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2)
+    S = numpy.diag([1.0, 1j])
+    CX = numpy.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 0, 1], [0, 0, 1, 0]], dtype=complex)
+    f = lambda x: (3 * x + 1) % 8
+    U1, U2 = rand_u(64), rand_u(8)
+    for be, fused in ((ref, False), (fz, True)):
+        (be.apply_fused if fused else be.apply_matrix)([0, 3, 5, 8, 9, 13], U1)
+        be.kronselect_dot({q: H for q in range(1, 11)})    # product passes
+        be.kronselect_dot({4: S})                          # diagonal
+        be.kronselect_dot({(2, 6): CX})                    # controlled gate
+        be.func_apply([7, 10, 11], [12, 1, 0], f)
+        (be.apply_fused if fused else be.apply_matrix)([13, 2, 6], U2)
+        be.kronselect_dot({12: H})
+    assert abs(fz.total_prob() - 1.0) < 1e-6
+    assert infidelity(ref, fz) < 1e-8
+    assert max_abs_diff(ref, fz) < 1e-4 * 2.0 ** (-n / 2) * 16
+
+
 def test_fused_rejects():
     a = GpuBackend.make_state(12, [], [])
     with pytest.raises(ValueError):
