@@ -14,7 +14,7 @@ Beside it, in the same run:
     algorithmic GB/s and the HBM roofline of its dominant kernel (single-qubit dense sweep);
   * 1 - F and max |amplitude difference| of the fused result against the complex128 result;
   * the HBM roofline of the fused kernel from HIP events on the state's stream, over one more run
-    of the same circuit recorded without streaming so that all blocks launch back to back;
+    of the same circuit's blocks, composed up front and launched back to back;
   * cpu_baseline: the oracle's restatement of the reference's cdot_loop (oracle/liboracle_fast.so,
     the reference's -O3 -ffast-math -fopenmp flags, all visible cores) on a time-bounded sample of
     the circuit's own gates at full size, extrapolated to the circuit's gate mix.
@@ -94,24 +94,23 @@ def run_fused_step(be, ops):
 
 
 def fused_kernel_ms(be, ops, ev):
-    """Average time per block pass of the fused kernel: the circuit recorded without streaming and
-    every block launched back to back between HIP events on the state's stream; the final scaling
-    sweep counts as one more pass of the same bytes. Returns (ms per pass, blocks)."""
-    be.streaming = False
-    try:
-        be.reinit()
-        for d in ops:
-            be.kronselect_dot(d)
-        blocks = be.plan_blocks()
-        t0, t1 = ev.create(), ev.create()
-        ev.record(t0, be.stream)
-        nblocks = be.launch_blocks(blocks)
-        be.fused_finish()
-        ev.record(t1, be.stream)
-        be.synchronize()
-    finally:
-        be.streaming = True
-    return ev.elapsed_ms(t0, t1) / (nblocks + 1), nblocks
+    """Average time per block pass of the fused kernel: the circuit's blocks (the recorder's greedy
+    rule restated by qip_amd.fused.fuse, composed up front) launched back to back between HIP events
+    on the state's stream; the final scaling sweep counts as one more pass of the same bytes.
+    Returns (ms per pass, blocks)."""
+    from qip_amd.fused import descriptor_gates, fuse
+    gates = [g for d in ops for g in descriptor_gates(d)]
+    blocks = [b.matrix() for b in fuse(gates, be.fusion_width)]
+    be.reinit()
+    be.synchronize()
+    t0, t1 = ev.create(), ev.create()
+    ev.record(t0, be.stream)
+    for qs, M in blocks:
+        be.apply_fused(qs, M)
+    be.fused_finish()
+    ev.record(t1, be.stream)
+    be.synchronize()
+    return ev.elapsed_ms(t0, t1) / (len(blocks) + 1), len(blocks)
 
 
 def plain_dominant_roofline(be, n, ev):
@@ -519,7 +518,7 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
     fused_gates_s = ngates * args.steps / elapsed
     block_bytes = 2 * 8 * 2 ** n
-    # per block pass: k_fused6 plus its one-block norm kernel, from two back-to-back runs
+    # per block pass (k_fused6, whose last workgroup also sums the norm), from two back-to-back runs
     timings = [fused_kernel_ms(fz, ops, ev) for _ in range(2)]
     avg_block_ms = sum(t[0] for t in timings) / len(timings)
     fused_gbs = block_bytes / (avg_block_ms * 1e-3) / 1e9

@@ -153,7 +153,32 @@ int qipgpu_fused_apply(qipgpu_state *s, int32_t k, const int32_t *targets, const
  * (kronprod.pyx:41-200) it replaces. */
 int qipgpu_compose(int32_t k, const int32_t *qubits, int32_t n_gates, const int32_t *gate_k,
                    const int32_t *gate_qubits, const double *mats, double *out);
-/* Apply the pending renormalisation factor (if any) with one scaling sweep. */
+/* Record gates on a complex64 state for online fusion: the precision form's replacement of the
+ * gate-by-gate kronselect_dot -> cdot_loop calls (kronprod.pyx:300-345, 41-200). Gate g has gate_k[g]
+ * (1..width) distinct qubits gate_qubits[6 g ..] (first = MSB of its matrix) and the next
+ * 4^gate_k[g] interleaved complex entries of `mats`. Gates are fused greedily, in program order, into
+ * blocks of at most `width` qubits: a gate joins the first block, from the last one touching its
+ * qubits on, that stays within the width, else opens a block. A block that no later gate can join is
+ * composed (qipgpu_compose) and launched at once when streaming (qipgpu_fused_apply for n >= 12, the
+ * plain dense sweep below). Every other entry point that touches the state, and qipgpu_synchronize,
+ * launches what is recorded first; qipgpu_init_product drops it. All gates are checked before any is
+ * recorded. */
+int qipgpu_fused_record(qipgpu_state *s, int32_t n_gates, const int32_t *gate_k, const int32_t *gate_qubits,
+                        const double *mats);
+/* Fusion width (1..6, default 6) and streaming (1, default) or deferred (0) launches; launches what
+ * is recorded first. */
+int qipgpu_fused_set_recording(qipgpu_state *s, int32_t width, int32_t streaming);
+/* Compose and launch every recorded gate (the blocks back to back). */
+int qipgpu_fused_flush(qipgpu_state *s);
+/* Recorded gates not launched yet, and blocks launched since the first recording on this state. */
+int qipgpu_fused_recorded(const qipgpu_state *s, int64_t *pending_gates, int64_t *blocks_launched);
+/* Host only (no GPU call): the recorder's streaming fusion plan for a gate list on n qubits (<= 64).
+ * gate_block[g] = global number of gate g's block (blocks numbered from 0 in launch order);
+ * final_after[g] = number of blocks final (launched when streaming) once gate g is recorded. */
+int qipgpu_fuse_online(int32_t n, int32_t width, int32_t n_gates, const int32_t *gate_k, const int32_t *gate_qubits,
+                       int64_t *gate_block, int64_t *final_after);
+/* Launch recorded gates, then apply the pending renormalisation factor (if any) with one scaling
+ * sweep. */
 int qipgpu_fused_finish(qipgpu_state *s);
 /* Norm of the state after the last fused call, before its renormalisation, in FP64. */
 int qipgpu_fused_last_norm(qipgpu_state *s, double *out);
@@ -165,7 +190,7 @@ int qipgpu_overlap(qipgpu_state *a, qipgpu_state *b, double *re, double *im, dou
                    double *norm_b);
 /* max_i |a_i - b_i| in FP64. */
 int qipgpu_max_abs_diff(qipgpu_state *a, qipgpu_state *b, double *out);
-/* Block until all work queued on the state's stream has finished. */
+/* Launch recorded gates, then block until all work queued on the state's stream has finished. */
 int qipgpu_synchronize(qipgpu_state *s);
 /* The hipStream_t every kernel on this state is launched on, for callers that time or order
  * their own work against it (events, copies). */

@@ -51,6 +51,11 @@ SIGNATURES = {
     "qipgpu_reduce_measure": (ctypes.c_int, [_P, _I32, _P, _D, _I64, _D, _PI64, _PD]),
     "qipgpu_fused_apply": (ctypes.c_int, [_P, _I32, _P, _P]),
     "qipgpu_compose": (ctypes.c_int, [_I32, _P, _I32, _P, _P, _P, _P]),
+    "qipgpu_fused_record": (ctypes.c_int, [_P, _I32, _P, _P, _P]),
+    "qipgpu_fused_set_recording": (ctypes.c_int, [_P, _I32, _I32]),
+    "qipgpu_fused_flush": (ctypes.c_int, [_P]),
+    "qipgpu_fused_recorded": (ctypes.c_int, [_P, _PI64, _PI64]),
+    "qipgpu_fuse_online": (ctypes.c_int, [_I32, _I32, _I32, _P, _P, _P, _P]),
     "qipgpu_fused_finish": (ctypes.c_int, [_P]),
     "qipgpu_fused_last_norm": (ctypes.c_int, [_P, _PD]),
     "qipgpu_overlap": (ctypes.c_int, [_P, _P, _PD, _PD, _PD, _PD]),

@@ -37,6 +37,8 @@ __device__ __forceinline__ uint64_t insert_zeros(uint64_t g, const PosList &pl) 
     return g;
 }
 
+struct Recorder;   // recorder.hip: gates recorded for online fusion
+
 constexpr int kRingSlots = 8;
 constexpr size_t kRingSlotBytes = 64 * 64 * 16 + 4096;  // a 64x64 complex128 matrix + offsets
 
@@ -55,18 +57,25 @@ struct qipgpu_state {
     size_t scratch_cap = 0;
     // device staging ring for operators too large for kernel arguments
     void *d_ring = nullptr;
-    void *h_ring = nullptr;  // pinned
+    void *h_ring = nullptr;      // pinned
+    void *h_ring_dev = nullptr;  // the pinned ring as the device addresses it
     hipEvent_t ring_ev[qipgpu::kRingSlots] = {};
     bool ring_used[qipgpu::kRingSlots] = {};
     int ring_next = 0;
-    // fused path: d_fused[0] = pending renormalisation factor, d_fused[1] = last norm,
-    // d_fused[2] = finished-workgroup count of the running block, d_fused[4..] = per-workgroup norm partials
+    // fused path: d_fused[0] = renormalisation factor and d_fused[1] = norm of the last block (written
+    // when the state is settled or the norm asked for), then two arrays of `fused_partials` per-workgroup
+    // norm partials used by consecutive blocks in turn. A block's output stays unnormalised until the
+    // next block or a settle applies 1/sqrt(sum of its partials): fused_prev_count > 0 marks that
+    // pending factor, with the partials in array fused_parity ^ 1.
     double *d_fused = nullptr;
     size_t fused_partials = 0;
-    bool fused_pending = false;
+    uint32_t fused_prev_count = 0;
+    int fused_parity = 0;
     // fused path operator staging (split BF16 planes), device
     void *d_fmat = nullptr;
     size_t d_fmat_bytes = 0;
+    // gates recorded for online fusion and not launched yet (complex64 states)
+    qipgpu::Recorder *recorder = nullptr;
 };
 
 namespace qipgpu {
@@ -95,7 +104,13 @@ int ensure_scratch(qipgpu_state *s, size_t bytes);
 int check_qubits(const qipgpu_state *s, int32_t k, const int32_t *q, const char *what);
 // Reserve a staging ring slot, copy `bytes` from `src` to the device, return device pointer.
 int ring_upload(qipgpu_state *s, const void *src, size_t bytes, void **dev);
-// Apply a fused path's pending renormalisation factor before any other use of the state.
+// Launch every recorded gate (recorder.hip); a no-op while the recorder itself is launching.
+int flush_recorded(qipgpu_state *s);
+// Drop recorded gates (the state is about to be overwritten) / free the recorder.
+void recorder_discard(qipgpu_state *s);
+void recorder_free(qipgpu_state *s);
+// Launch recorded gates, then apply a fused path's pending renormalisation factor: before the state
+// is read or written from the host.
 int settle(qipgpu_state *s);
 // Sum of |psi|^2 in FP64, result left on the device at `dst` (enqueued on s->stream).
 int norm_to_device(qipgpu_state *s, double *dst);

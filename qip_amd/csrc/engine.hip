@@ -62,20 +62,35 @@ int check_qubits(const qipgpu_state *s, int32_t k, const int32_t *q, const char 
     return QIPGPU_OK;
 }
 
+// Copy a staging slot out of pinned host memory on the state's own queue. A small kernel reads the
+// slot over the host link (system-scope loads, so no cache holds an older copy of a reused slot).
+// A copy-engine transfer would instead add a cross-queue wait before every kernel that uses the slot.
+__global__ __launch_bounds__(256) void k_stage(const uint64_t *src, uint64_t *__restrict__ dst, uint32_t words) {
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < words; i += gridDim.x * 256)
+        dst[i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 int ring_upload(qipgpu_state *s, const void *src, size_t bytes, void **dev) {
     if (bytes > kRingSlotBytes) return set_error(QIPGPU_EINVAL, "operator of %zu bytes exceeds staging slot", bytes);
     if (!s->d_ring) {
         QIP_HIP(hipMalloc(&s->d_ring, kRingSlots * kRingSlotBytes));
         QIP_HIP(hipHostMalloc(&s->h_ring, kRingSlots * kRingSlotBytes, hipHostMallocDefault));
+        QIP_HIP(hipHostGetDevicePointer(&s->h_ring_dev, s->h_ring, 0));
         for (int i = 0; i < kRingSlots; ++i) QIP_HIP(hipEventCreateWithFlags(&s->ring_ev[i], hipEventDisableTiming));
     }
     const int slot = s->ring_next;
     s->ring_next = (slot + 1) % kRingSlots;
-    if (s->ring_used[slot]) QIP_HIP(hipEventSynchronize(s->ring_ev[slot]));
+    if (s->ring_used[slot]) QIP_HIP(hipEventSynchronize(s->ring_ev[slot]));   // its last stage has run
     char *h = static_cast<char *>(s->h_ring) + slot * kRingSlotBytes;
     char *d = static_cast<char *>(s->d_ring) + slot * kRingSlotBytes;
     std::memcpy(h, src, bytes);
-    QIP_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s->stream));
+    // whole 8-byte words; the slot size is a multiple of 8, so the last word stays inside the slot
+    const uint32_t words = (uint32_t)((bytes + 7) / 8);
+    const uint32_t grid = std::min<uint32_t>((words + 255) / 256, 32u);
+    hipLaunchKernelGGL(k_stage, dim3(grid), dim3(256), 0, s->stream,
+                       (const uint64_t *)(static_cast<char *>(s->h_ring_dev) + slot * kRingSlotBytes), (uint64_t *)d,
+                       words);
+    QIP_LAUNCHED();
     QIP_HIP(hipEventRecord(s->ring_ev[slot], s->stream));
     s->ring_used[slot] = true;
     *dev = d;
@@ -295,6 +310,7 @@ int qipgpu_destroy(qipgpu_state *s) {
         if (s->ring_ev[i]) (void)hipEventDestroy(s->ring_ev[i]);
     if (s->d_fused) (void)hipFree(s->d_fused);
     if (s->d_fmat) (void)hipFree(s->d_fmat);
+    recorder_free(s);
     (void)hipStreamDestroy(s->stream);
     delete s;
     return QIPGPU_OK;
@@ -307,6 +323,8 @@ int qipgpu_info(const qipgpu_state *s, int32_t *n, int32_t *dtype) {
 }
 
 int qipgpu_synchronize(qipgpu_state *s) {
+    const int rc = flush_recorded(s);
+    if (rc) return rc;
     QIP_HIP(hipStreamSynchronize(s->stream));
     return QIPGPU_OK;
 }
@@ -318,7 +336,8 @@ int qipgpu_get_stream(qipgpu_state *s, void **stream) {
 
 int qipgpu_init_product(qipgpu_state *s, int32_t n_groups, const int32_t *group_len,
                         const int32_t *group_idx, const double *feeds) {
-    s->fused_pending = false;  // every amplitude is rewritten
+    s->fused_prev_count = 0;  // every amplitude is rewritten, so recorded gates and the factor are moot
+    recorder_discard(s);
     if (n_groups < 0) return set_error(QIPGPU_EINVAL, "negative group count");
     uint64_t fed = 0;
     int64_t total_idx = 0, total_feed = 0;
@@ -776,7 +795,10 @@ int qipgpu_apply_matrix(qipgpu_state *s, int32_t k, const int32_t *targets, cons
     if (n_ctrl < 0) return set_error(QIPGPU_EINVAL, "negative control count");
     int rc = check_disjoint(s, k, targets, n_ctrl, controls);
     if (rc) return rc;
-    // linear in the state: a pending fused renormalisation (a scalar) stays pending
+    // linear in the state: a pending fused renormalisation (a scalar) stays pending; recorded gates
+    // go first
+    rc = flush_recorded(s);
+    if (rc) return rc;
     Analysed an = analyse(k, targets, mat);
     std::vector<int32_t> ctl(controls, controls + n_ctrl);
     return s->dtype == QIPGPU_C128 ? dispatch<double>(s, an, ctl) : dispatch<float>(s, an, ctl);
@@ -787,7 +809,10 @@ int qipgpu_apply_diagonal(qipgpu_state *s, int32_t k, const int32_t *targets, co
     if (k < 0 || k > QIPGPU_MAX_DENSE) return set_error(QIPGPU_EINVAL, "diagonal on %d qubits", k);
     int rc = check_disjoint(s, k, targets, n_ctrl, controls);
     if (rc) return rc;
-    // linear in the state: a pending fused renormalisation (a scalar) stays pending
+    // linear in the state: a pending fused renormalisation (a scalar) stays pending; recorded gates
+    // go first
+    rc = flush_recorded(s);
+    if (rc) return rc;
     Analysed an;
     an.targets.assign(targets, targets + k);
     const int D = 1 << k;
@@ -852,7 +877,10 @@ static int func_apply_dev(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, 
     all.insert(all.end(), reg2, reg2 + n_reg2);
     int rc = check_qubits(s, (int32_t)all.size(), all.data(), "func_apply registers");
     if (rc) return rc;
-    // linear in the state: a pending fused renormalisation (a scalar) stays pending
+    // linear in the state: a pending fused renormalisation (a scalar) stays pending; recorded gates
+    // go first
+    rc = flush_recorded(s);
+    if (rc) return rc;
     FuncArgs a;
     std::memset(&a, 0, sizeof(a));
     a.n1 = n_reg1;

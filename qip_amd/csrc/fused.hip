@@ -19,12 +19,23 @@ namespace {
 
 constexpr int kFusedBlocksPerCU = 2;   // 69 KB of LDS and 212 VGPRs per workgroup
 
-// fstate[0] = scale applied to every amplitude as it is loaded (the previous block's 1/sqrt(norm)),
-// fstate[1] = norm of this block's output, fstate[2] (low word) = count of finished workgroups,
-// partial[grid] = per-workgroup FP64 norm partials. The last workgroup to finish sums the partials in a
-// fixed order, publishes the norm and the next scale, and resets the count.
+// Sum `count` FP64 partials with one wavefront: lane l adds partials l, l + 64, ... in order, then a
+// butterfly over the lanes. Each butterfly step adds the same two values on both lanes of a pair, so
+// every lane, every wavefront and every workgroup that sums the same partials gets the same bits.
+__device__ __forceinline__ double wave_sum_partials(const double *__restrict__ partial, uint32_t count, int lane) {
+    double acc = 0.0;
+    for (uint32_t i = lane; i < count; i += 64) acc += partial[i];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    return acc;
+}
+
+// One fused block. Every amplitude is scaled as it is loaded by 1/sqrt(norm of the previous block's
+// output): each workgroup sums the previous block's `prev_count` partials itself (the same sum in all
+// of them; prev_count = 0 means no pending factor), so a block is one launch with no cross-workgroup
+// synchronisation. partial[grid] receives this block's per-workgroup FP64 norm partials.
 __global__ __launch_bounds__(256, 2) void k_fused6(cx<float> *amps, const bf16x8 *__restrict__ wfrag, FusedArgs a,
-                                                   double *fstate, double *partial) {
+                                                   const double *__restrict__ prev_partial, uint32_t prev_count,
+                                                   double *partial) {
     typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f4;
     __shared__ TileLds lds;
     __shared__ double red[4];
@@ -38,7 +49,6 @@ __global__ __launch_bounds__(256, 2) void k_fused6(cx<float> *amps, const bf16x8
         ah[ks] = wfrag[((wave * 8 + ks) * 2 + 0) * 64 + lane];
         al[ks] = wfrag[((wave * 8 + ks) * 2 + 1) * 64 + lane];
     }
-    const double scale = fstate[0];
     const uint64_t off_t = u_offset<0, 9>(2u * (uint32_t)tid, a);     // this thread's u-pair offset
     const uint32_t ucol_gather = u_of_col(lane, a);
     const uint32_t ucol_out0 = u_of_col(lane & 31, a), ucol_out1 = u_of_col(32 + (lane & 31), a);
@@ -53,7 +63,6 @@ __global__ __launch_bounds__(256, 2) void k_fused6(cx<float> *amps, const bf16x8
     ctx.ucol_out0 = ucol_out0;
     ctx.ucol_out1 = ucol_out1;
     ctx.urow_half = urow_half;
-    ctx.scale = scale;
 
     // Each workgroup takes a contiguous range of tiles (neighbouring tiles share DRAM pages more
     // often than tiles a grid apart). Two register buffers alternate: while one tile is processed
@@ -63,10 +72,12 @@ __global__ __launch_bounds__(256, 2) void k_fused6(cx<float> *amps, const bf16x8
     const uint64_t per = (a.ntiles + gridDim.x - 1) / gridDim.x;
     uint64_t tile = (uint64_t)blockIdx.x * per;
     const uint64_t end = min(tile + per, a.ntiles);
+    cx<float> va[16], vb[16];
+    if (tile < end) load_tile(va, amps, tile_base(tile, a) + off_t, a);
+    // the scale, per wavefront, while the first tile's loads are in flight
+    ctx.scale = prev_count ? 1.0 / sqrt(wave_sum_partials(prev_partial, prev_count, lane)) : 1.0;
     if (tile < end) {
         const uint64_t last = end - 1;
-        cx<float> va[16], vb[16];
-        load_tile(va, amps, tile_base(tile, a) + off_t, a);
         for (;;) {
             load_tile(vb, amps, tile_base(min(tile + 1, last), a) + off_t, a);
             process_tile(va, tile_base(tile, a) + off_t, amps, lds, ah, al, ctx, a, nacc);
@@ -80,44 +91,16 @@ __global__ __launch_bounds__(256, 2) void k_fused6(cx<float> *amps, const bf16x8
     for (int off = 32; off > 0; off >>= 1) nacc += __shfl_down(nacc, off, 64);
     if (lane == 0) red[wave] = nacc;
     __syncthreads();
-    __shared__ uint32_t is_last;
-    uint32_t *count = reinterpret_cast<uint32_t *>(fstate + 2);
-    if (tid == 0) {
-        __hip_atomic_store(&partial[blockIdx.x], (red[0] + red[1]) + (red[2] + red[3]), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        // release the partial, then count this workgroup; acquire the others' if it is the last
-        is_last = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    double *fin = reinterpret_cast<double *>(&lds);   // the tile buffers are free now
-    double acc = 0.0;
-    for (uint32_t i = tid; i < gridDim.x; i += 256)
-        acc += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    fin[tid] = acc;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (tid < o) fin[tid] += fin[tid + o];
-        __syncthreads();
-    }
-    if (tid == 0) {
-        fstate[1] = fin[0];
-        fstate[0] = 1.0 / sqrt(fin[0]);
-        __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (tid == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-__global__ void k_set_scale(double *fstate, double v) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) fstate[0] = v;
-}
-
-__global__ void k_fused_init(double *fstate) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        fstate[0] = 1.0;
-        fstate[1] = 0.0;
-        fstate[2] = 0.0;   // the finished-workgroup count lives in its low word
-        fstate[3] = 0.0;
+// fstate[1] = norm of the last block's output (its partials in the same fixed order as k_fused6),
+// fstate[0] = 1 / sqrt(norm).
+__global__ __launch_bounds__(64) void k_fused_norm(const double *__restrict__ partial, uint32_t count, double *fstate) {
+    const double nrm = wave_sum_partials(partial, count, threadIdx.x);
+    if (threadIdx.x == 0) {
+        fstate[1] = nrm;
+        fstate[0] = 1.0 / sqrt(nrm);
     }
 }
 
@@ -133,17 +116,33 @@ __global__ __launch_bounds__(256) void k_scale(cx<R> *amps, uint64_t len, const 
     }
 }
 
+double *fused_partials_of(qipgpu_state *s, int parity) {
+    return s->d_fused + 4 + (size_t)parity * s->fused_partials;
+}
+
+// d_fused: [0] scale and [1] norm written by k_fused_norm, then two partial arrays of `cap` doubles
+// that alternate between consecutive blocks (a block reads the other array while it writes its own).
 int ensure_fused_state(qipgpu_state *s, uint32_t nblocks) {
     if (s->d_fused && s->fused_partials >= nblocks) return QIPGPU_OK;
     if (s->d_fused) {
         QIP_HIP(hipStreamSynchronize(s->stream));
+        if (s->fused_prev_count) {   // keep the pending block's partials
+            std::vector<double> keep(s->fused_prev_count);
+            QIP_HIP(hipMemcpy(keep.data(), fused_partials_of(s, s->fused_parity ^ 1), keep.size() * sizeof(double),
+                              hipMemcpyDeviceToHost));
+            QIP_HIP(hipFree(s->d_fused));
+            s->d_fused = nullptr;
+            QIP_HIP(hipMalloc(&s->d_fused, sizeof(double) * (4 + 2 * (size_t)nblocks)));
+            s->fused_partials = nblocks;
+            QIP_HIP(hipMemcpy(fused_partials_of(s, s->fused_parity ^ 1), keep.data(), keep.size() * sizeof(double),
+                              hipMemcpyHostToDevice));
+            return QIPGPU_OK;
+        }
         QIP_HIP(hipFree(s->d_fused));
         s->d_fused = nullptr;
     }
-    QIP_HIP(hipMalloc(&s->d_fused, sizeof(double) * (4 + nblocks)));
+    QIP_HIP(hipMalloc(&s->d_fused, sizeof(double) * (4 + 2 * (size_t)nblocks)));
     s->fused_partials = nblocks;
-    hipLaunchKernelGGL(k_fused_init, dim3(1), dim3(64), 0, s->stream, s->d_fused);
-    QIP_LAUNCHED();
     return QIPGPU_OK;
 }
 
@@ -161,11 +160,11 @@ uint32_t fused_grid(uint64_t ntiles) {
 
 namespace qipgpu {
 
-// Apply a pending renormalisation factor before the state is read or written from the host. Gate,
+// Launch recorded gates and apply a pending renormalisation factor before the state is read or written
+// from the host. Gate,
 // diagonal, product and func_apply sweeps are linear, so they commute with the scalar and leave it
 // pending for the next fused block (which applies it as it loads) or the next read.
 int settle(qipgpu_state *s) {
-    if (!s->fused_pending) return QIPGPU_OK;
     return qipgpu_fused_finish(s);
 }
 
@@ -178,6 +177,8 @@ int qipgpu_fused_apply(qipgpu_state *s, int32_t k, const int32_t *targets, const
     if (s->n < kTileBits) return set_error(QIPGPU_EUNSUP, "fused blocks need n >= %d (n=%d)", kTileBits, s->n);
     if (k < 1 || k > 6) return set_error(QIPGPU_EINVAL, "fused block on %d qubits (1..6)", k);
     int rc = check_qubits(s, k, targets, "fused block qubits");
+    if (rc) return rc;
+    rc = flush_recorded(s);   // recorded gates go first (a no-op when the recorder is launching)
     if (rc) return rc;
     const int n = s->n;
     // tile bits: the targets padded to six, and six column bits (choose_tile_bits)
@@ -227,14 +228,12 @@ int qipgpu_fused_apply(qipgpu_state *s, int32_t k, const int32_t *targets, const
     const uint32_t grid = fused_grid(a.ntiles);
     rc = ensure_fused_state(s, grid);
     if (rc) return rc;
-    if (!s->fused_pending) {
-        hipLaunchKernelGGL(k_set_scale, dim3(1), dim3(64), 0, s->stream, s->d_fused, 1.0);
-        QIP_LAUNCHED();
-    }
     hipLaunchKernelGGL(k_fused6, dim3(grid), dim3(256), 0, s->stream, (cx<float> *)s->amps, (const bf16x8 *)dfrag, a,
-                       s->d_fused, s->d_fused + 4);
+                       (const double *)fused_partials_of(s, s->fused_parity ^ 1), s->fused_prev_count,
+                       fused_partials_of(s, s->fused_parity));
     QIP_LAUNCHED();
-    s->fused_pending = true;
+    s->fused_prev_count = grid;
+    s->fused_parity ^= 1;
     return QIPGPU_OK;
 }
 
@@ -417,21 +416,27 @@ int qipgpu_compose(int32_t k, const int32_t *qubits, int32_t n_gates, const int3
 }
 
 int qipgpu_fused_finish(qipgpu_state *s) {
-    if (!s->fused_pending) return QIPGPU_OK;
-    s->fused_pending = false;
+    const int rc = flush_recorded(s);
+    if (rc) return rc;
+    if (!s->fused_prev_count) return QIPGPU_OK;
+    hipLaunchKernelGGL(k_fused_norm, dim3(1), dim3(64), 0, s->stream,
+                       (const double *)fused_partials_of(s, s->fused_parity ^ 1), s->fused_prev_count, s->d_fused);
+    QIP_LAUNCHED();
+    s->fused_prev_count = 0;
     const uint32_t nb = (uint32_t)std::min<uint64_t>((s->len + 1023) / 1024, 1u << 20);
-    if (s->dtype == QIPGPU_C128)
-        hipLaunchKernelGGL(k_scale<double>, dim3(nb), dim3(256), 0, s->stream, (cx<double> *)s->amps, s->len,
-                           (const double *)s->d_fused);
-    else
-        hipLaunchKernelGGL(k_scale<float>, dim3(nb), dim3(256), 0, s->stream, (cx<float> *)s->amps, s->len,
-                           (const double *)s->d_fused);
+    hipLaunchKernelGGL(k_scale<float>, dim3(nb), dim3(256), 0, s->stream, (cx<float> *)s->amps, s->len,
+                       (const double *)s->d_fused);
     QIP_LAUNCHED();
     return QIPGPU_OK;
 }
 
 int qipgpu_fused_last_norm(qipgpu_state *s, double *out) {
     if (!s->d_fused) return set_error(QIPGPU_EINVAL, "no fused block has been applied to this state");
+    if (s->fused_prev_count) {   // still pending: sum the last block's partials now
+        hipLaunchKernelGGL(k_fused_norm, dim3(1), dim3(64), 0, s->stream,
+                           (const double *)fused_partials_of(s, s->fused_parity ^ 1), s->fused_prev_count, s->d_fused);
+        QIP_LAUNCHED();
+    }
     QIP_HIP(hipMemcpyAsync(out, s->d_fused + 1, sizeof(double), hipMemcpyDeviceToHost, s->stream));
     QIP_HIP(hipStreamSynchronize(s->stream));
     return QIPGPU_OK;

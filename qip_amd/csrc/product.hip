@@ -217,7 +217,10 @@ extern "C" int qipgpu_apply_product(qipgpu_state *s, int32_t k, const int32_t *t
     if (s->n < kTileBits) return set_error(QIPGPU_EUNSUP, "product sweep needs n >= %d (n=%d)", kTileBits, s->n);
     int rc = check_qubits(s, k, targets, "product qubits");
     if (rc) return rc;
-    // linear in the state: a pending fused renormalisation (a scalar) stays pending
+    // linear in the state: a pending fused renormalisation (a scalar) stays pending; recorded gates
+    // go first
+    rc = flush_recorded(s);
+    if (rc) return rc;
     // the factors commute: the first pass takes every factor on the low bits and up to
     // kHighPerPass others, each further pass up to kHighPerPass more
     std::vector<int> low, high;

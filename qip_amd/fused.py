@@ -20,6 +20,7 @@ touched by a block after block i, no later gate can reach block i: it is final a
 host records the rest. Launching final blocks early gives the same blocks, in the same order, with
 the same results, as fusing the whole record at the end.
 """
+import ctypes
 from typing import List, Sequence, Tuple
 
 import numpy
@@ -77,6 +78,20 @@ def controlled_dense(call: planner.MatrixCall) -> Tuple[Tuple[int, ...], numpy.n
         C[d:, d:] = M
         M = C
     return tuple(call.controls) + tuple(call.targets), M
+
+
+def descriptor_gates(mats) -> List[Tuple[Tuple[int, ...], numpy.ndarray]]:
+    """The dense gates, in order, that FusedBackend.kronselect_dot(mats) records (no call may span
+    more than six qubits)."""
+    out = []
+    for call in planner.plan(mats):
+        if isinstance(call, planner.ProductCall):
+            out.extend(((q,), m) for q, m in zip(call.targets, call.mats))
+        elif len(call.controls) + len(call.targets) > 6:
+            raise ValueError("a call on {} qubits is applied unfused".format(len(call.controls) + len(call.targets)))
+        else:
+            out.append(controlled_dense(call))
+    return out
 
 
 class OnlineFusion:
@@ -148,16 +163,41 @@ def fuse(gates: Sequence[Tuple[Tuple[int, ...], numpy.ndarray]], width: int) -> 
     return blocks
 
 
+class _GateBuffer:
+    """Gates of one call packed for qipgpu_fused_record: widths, qubits in six slots, matrices."""
+    __slots__ = ("k", "q", "mats")
+
+    def __init__(self):
+        self.k, self.q, self.mats = [], [], []
+
+    def add(self, gq, G) -> None:
+        L = len(gq)
+        self.k.append(L)
+        self.q.extend(gq)
+        self.q.extend((0,) * (6 - L))
+        self.mats.append(numpy.ravel(G))
+
+    def record(self, be) -> None:
+        if not self.k:
+            return
+        k = numpy.array(self.k, dtype=numpy.int32)
+        q = numpy.array(self.q, dtype=numpy.int32)
+        m = numpy.concatenate(self.mats).astype(numpy.complex128, copy=False)
+        self.k, self.q, self.mats = [], [], []
+        # the arrays stay referenced here for the whole call, so their raw addresses are safe to pass
+        _capi.check(be._lib.qipgpu_fused_record(be._h, len(k), k.ctypes.data, q.ctypes.data, m.ctypes.data),
+                    "record")
+
+
 class FusedBackend(GpuBackend):
-    """complex64 state; kronselect_dot is deferred and fused (module docstring)."""
+    """complex64 state; kronselect_dot records into the library's online fusion (module docstring)."""
 
     fusion_width = 6
-    streaming = True
 
     def __init__(self, n, handle, dtype):
         super().__init__(n, handle, dtype)
-        self._fuser = OnlineFusion(n, self.fusion_width)
-        self.blocks_applied = 0
+        self._streaming = True
+        _capi.check(self._lib.qipgpu_fused_set_recording(self._h, self.fusion_width, 1), "set_recording")
 
     @classmethod
     def make_state(cls, n, index_groups, feed_list, inputstartindex=None, inputendindex=None,
@@ -167,110 +207,62 @@ class FusedBackend(GpuBackend):
         return super(FusedBackend, cls).make_state(n, index_groups, feed_list, inputstartindex, inputendindex,
                                                    outputstartindex, outputendindex, statetype=statetype)
 
+    @property
+    def streaming(self) -> bool:
+        """Launch each block as soon as it is final (True), or every block at the next read (False)."""
+        return self._streaming
+
+    @streaming.setter
+    def streaming(self, on: bool) -> None:
+        _capi.check(self._lib.qipgpu_fused_set_recording(self._h, self.fusion_width, int(bool(on))), "set_recording")
+        self._streaming = bool(on)
+
     # -- recording ----------------------------------------------------------------------------
     def kronselect_dot(self, mats, input_offset: int = 0, output_offset: int = 0) -> None:
         check_windows(self.n, 2 ** self.n, input_offset, output_offset)
+        rec = _GateBuffer()
         for call in planner.plan(mats):
             if isinstance(call, planner.ProductCall):
                 for q, m in zip(call.targets, call.mats):
-                    self._record((q,), numpy.ascontiguousarray(m, dtype=numpy.complex128))
-                continue
-            gq, G = controlled_dense(call)
-            if len(gq) > self.fusion_width:
-                self.flush()
-                GpuBackend.apply_matrix(self, call.targets, call.mat, call.controls)
+                    rec.add((q,), m)
+            elif len(call.controls) + len(call.targets) > self.fusion_width:
+                rec.record(self)
+                GpuBackend.apply_matrix(self, call.targets, call.mat, call.controls)   # launches the record first
             else:
-                self._record(gq, G)
+                rec.add(*controlled_dense(call))
+        rec.record(self)
 
     def apply_matrix(self, targets, mat, controls=()) -> None:
-        call = planner.MatrixCall(list(targets), list(controls), numpy.ascontiguousarray(mat, dtype=numpy.complex128))
-        gq, G = controlled_dense(call)
-        if len(gq) > self.fusion_width:
-            self.flush()
+        if len(targets) + len(controls) > self.fusion_width:
             GpuBackend.apply_matrix(self, targets, mat, controls)
-        else:
-            self._record(gq, G)
+            return
+        call = planner.MatrixCall(list(targets), list(controls), numpy.ascontiguousarray(mat, dtype=numpy.complex128))
+        rec = _GateBuffer()
+        rec.add(*controlled_dense(call))
+        rec.record(self)
 
     def apply_product(self, targets, mats) -> None:
+        rec = _GateBuffer()
         for q, m in zip(targets, mats):
-            self._record((q,), numpy.ascontiguousarray(m, dtype=numpy.complex128).reshape(2, 2))
+            rec.add((q,), numpy.asarray(m).reshape(2, 2))
+        rec.record(self)
 
-    def _record(self, gq, G) -> None:
-        self._fuser.add(gq, G)
-        if self.streaming:
-            k = self._fuser.final_count()
-            if k:
-                self.launch_blocks([b.matrix() for b in self._fuser.take(k)])
-
-    def reinit(self, index_groups=(), feed_list=()) -> None:
-        self._fuser = OnlineFusion(self.n, self.fusion_width)
-        super().reinit(index_groups, feed_list)
+    def _recorded(self) -> Tuple[int, int]:
+        pending, launched = ctypes.c_int64(0), ctypes.c_int64(0)
+        _capi.check(self._lib.qipgpu_fused_recorded(self._h, ctypes.byref(pending), ctypes.byref(launched)), "recorded")
+        return int(pending.value), int(launched.value)
 
     def pending_gates(self) -> int:
         """Recorded gates not launched yet."""
-        return self._fuser.pending
+        return self._recorded()[0]
 
-    def plan_blocks(self) -> List[Tuple[List[int], numpy.ndarray]]:
-        """Take the recorded gates not launched yet and return their blocks as (qubits, matrix)."""
-        return [b.matrix() for b in self._fuser.take()]
-
-    def launch_blocks(self, blocks: Sequence[Tuple[List[int], numpy.ndarray]]) -> int:
-        """Apply planned blocks: MFMA kernel for n >= 12, the plain complex64 kernels below."""
-        for qs, M in blocks:
-            if self.n >= FUSED_MIN_QUBITS:
-                self.apply_fused(qs, M)
-            else:
-                GpuBackend.apply_matrix(self, qs, M)
-        self.blocks_applied += len(blocks)
-        return len(blocks)
+    @property
+    def blocks_applied(self) -> int:
+        """Fused blocks launched on this state so far."""
+        return self._recorded()[1]
 
     def flush(self) -> int:
-        """Apply every recorded gate not launched yet; returns the number of blocks launched."""
-        if not self._fuser.pending:
-            return 0
-        return self.launch_blocks(self.plan_blocks())
-
-    # -- everything else sees the flushed state -----------------------------------------------
-    def get_relative_range(self, start, end):
-        self.flush()
-        return super().get_relative_range(start, end)
-
-    def overwrite_relative_range(self, start, end, data):
-        self.flush()
-        super().overwrite_relative_range(start, end, data)
-
-    def addto_relative_range(self, start, end, data):
-        self.flush()
-        super().addto_relative_range(start, end, data)
-
-    def func_apply(self, reg1_indices, reg2_indices, func, input_offset=0, output_offset=0):
-        self.flush()
-        super().func_apply(reg1_indices, reg2_indices, func, input_offset, output_offset)
-
-    def total_prob(self):
-        self.flush()
-        return super().total_prob()
-
-    def measure(self, indices, measured=None, measured_prob=None, input_offset=0, output_offset=0):
-        self.flush()
-        return super().measure(indices, measured, measured_prob, input_offset, output_offset)
-
-    def reduce_measure(self, indices, measured=None, measured_prob=None, input_offset=0, output_offset=0):
-        self.flush()
-        return super().reduce_measure(indices, measured, measured_prob, input_offset, output_offset)
-
-    def soft_measure(self, indices, measured=None, input_offset=0):
-        self.flush()
-        return super().soft_measure(indices, measured, input_offset)
-
-    def measure_probabilities(self, indices, top_k=0):
-        self.flush()
-        return super().measure_probabilities(indices, top_k)
-
-    def marginals(self, indices):
-        self.flush()
-        return super().marginals(indices)
-
-    def synchronize(self):
-        self.flush()
-        super().synchronize()
+        """Launch every recorded gate not launched yet; returns the number of blocks launched."""
+        before = self.blocks_applied
+        _capi.check(self._lib.qipgpu_fused_flush(self._h), "flush")
+        return self.blocks_applied - before

@@ -182,3 +182,53 @@ def test_compose_single_qubit_blocks(k):
     for q in qs:
         want = numpy.kron(want, per[q])
     assert numpy.max(numpy.abs(compose(qs, gates) - want)) < 1e-12 * max(1.0, numpy.max(numpy.abs(want)))
+
+
+def _native_plan(n, width, gates):
+    from qip_amd import _capi
+    k = numpy.array([len(gq) for gq, _ in gates], dtype=numpy.int32)
+    q = numpy.zeros((max(len(gates), 1), 6), dtype=numpy.int32)
+    for i, (gq, _) in enumerate(gates):
+        q[i, :len(gq)] = gq
+    blk = numpy.zeros(max(len(gates), 1), dtype=numpy.int64)
+    fin = numpy.zeros(max(len(gates), 1), dtype=numpy.int64)
+    _capi.check(_capi.load().qipgpu_fuse_online(n, width, len(gates), _capi.ptr(k), _capi.ptr(q), _capi.ptr(blk),
+                                                _capi.ptr(fin)), "fuse_online")
+    return blk[:len(gates)], fin[:len(gates)]
+
+
+@pytest.mark.parametrize("n,depth,seed,width", [(30, 40, 2024, 6), (16, 12, 3, 6), (13, 5, 8, 4), (9, 6, 1, 2)])
+def test_native_recorder_plan_equals_online_fusion(n, depth, seed, width):
+    """The library's recorder (qipgpu_fuse_online, the routine qipgpu_fused_record runs) places every
+    gate in the block the Python statement of the rule does, and has the same blocks final after
+    every gate."""
+    from qip_amd.fused import OnlineFusion
+    gates = gates_of(layered_random_circuit(n, depth, seed=seed))
+    blk, fin = _native_plan(n, width, gates)
+    of = OnlineFusion(n, width)
+    for i, (gq, G) in enumerate(gates):
+        G = G.view()                      # a fresh object, so the gate is found by identity
+        of.add(gq, G)
+        # the block of gate i: the open block whose last gate is this one
+        for bi, b in enumerate(of.blocks):
+            if b.gates and b.gates[-1][1] is G:
+                assert blk[i] == of.base + bi, (i, blk[i], of.base + bi)
+                break
+        else:
+            raise AssertionError("gate {} not found".format(i))
+        of.take(of.final_count())
+        assert fin[i] == of.base, (i, fin[i], of.base)
+    assert int(fin[-1]) + len(of.blocks) == int(blk.max()) + 1
+
+
+def test_native_recorder_plan_rejects():
+    from qip_amd import _capi
+    H = numpy.eye(2)
+    with pytest.raises(ValueError):
+        _native_plan(4, 6, [((4,), H)])             # qubit outside the state
+    with pytest.raises(ValueError):
+        _native_plan(4, 2, [((0, 1, 2), numpy.eye(8))])   # wider than the fusion width
+    with pytest.raises(ValueError):
+        _native_plan(4, 6, [((1, 1), numpy.eye(4))])    # repeated qubit
+    blk, fin = _native_plan(3, 6, [])
+    assert len(blk) == 0

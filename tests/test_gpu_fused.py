@@ -146,6 +146,56 @@ def test_fused_backend_layered_circuit():
     assert abs(fz.total_prob() - 1.0) < 1e-6
 
 
+@pytest.mark.parametrize("n", [16, 10])
+def test_recorder_blocks_equal_python_plan(n):
+    """The library's recorder launches exactly the blocks of the Python statement of the greedy
+    rule: the same blocks composed by qipgpu_compose and applied one by one (the MFMA kernel at
+    n >= 12, the plain dense sweep below) give a bitwise equal state."""
+    from qip_amd.fused import FusedBackend, descriptor_gates, fuse
+    from qip_amd.workloads import layered_random_circuit
+    ops = layered_random_circuit(n, 10, seed=n)
+    fz = FusedBackend.make_state(n, [], [])
+    for d in ops:
+        fz.kronselect_dot(d)
+    got = fz.get_state()
+    blocks = fuse([g for d in ops for g in descriptor_gates(d)], 6)
+    assert fz.blocks_applied == len(blocks)
+    ref = GpuBackend.make_state(n, [], [], statetype=numpy.complex64)
+    for b in blocks:
+        qs, M = b.matrix()
+        if n >= 12:
+            ref.apply_fused(qs, M)
+        else:
+            ref.apply_matrix(qs, M)
+    assert numpy.array_equal(got, ref.get_state())
+
+
+def test_recorder_orders_wide_and_direct_calls():
+    """Gates wider than the fusion width (here a 7-qubit multi-controlled Z, built from CMat, never
+    expanded to a dense matrix) and direct calls between recorded gates act in program order."""
+    from qip_amd.fused import FusedBackend
+    from qip_amd.operators import CMat
+    n = 13
+    psi = rand_state(n)
+    pl = GpuBackend.make_state(n, [list(range(n))], [psi])
+    fz = FusedBackend.make_state(n, [list(range(n))], [psi])
+    H = numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2)
+    m = numpy.diag([1.0, -1.0])
+    for _ in range(6):
+        m = CMat(m)
+    U = rand_u(4)
+    for be in (pl, fz):
+        be.kronselect_dot({q: H for q in range(n)})
+        be.kronselect_dot({tuple(range(7)): m})           # 6 controls + Z: applied unfused
+        be.kronselect_dot({(3, 9): U})
+        be.apply_product([0, 5], [H, H])
+        be.func_apply([1, 2], [11, 12], lambda x: x ^ 1)  # direct call between recorded gates
+        be.kronselect_dot({12: H, 4: H})
+    assert fz.pending_gates() > 0 or fz.blocks_applied > 0
+    assert infidelity(pl, fz) < 1e-8
+    assert fz.pending_gates() == 0
+
+
 def test_fused_backend_graph_qft():
     """A qip graph run through FusedBackend: the 14-qubit QFT equals ifft(norm='ortho')."""
     from qip_amd import QFFT, Qubit, run
