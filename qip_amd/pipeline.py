@@ -49,12 +49,11 @@ class PipelineObject(object):
     def get_inputs(self) -> Sequence["PipelineObject"]:
         return self.inputs
 
-    def __hash__(self):
-        # Nodes compare by identity, so an identity hash is consistent with equality. (The
-        # reference hashes repr(self), qip/pipeline.py:67-68, which recurses through every input
-        # and makes each set operation of a run O(graph depth); its string hash is randomised per
-        # process, so no result can depend on the order it gives.)
-        return object.__hash__(self)
+    # Nodes compare by identity, so an identity hash is consistent with equality. (The reference
+    # hashes repr(self), qip/pipeline.py:67-68, which recurses through every input and makes each
+    # set operation of a run O(graph depth); its string hash is randomised per process, so no
+    # result can depend on the order it gives.)
+    __hash__ = object.__hash__
 
 
 def run(*args: PipelineObject,
@@ -129,7 +128,9 @@ def get_deps(*args: PipelineObject, feed: Mapping = None) -> Tuple[Sequence[Pipe
         if node in feed or len(node.inputs) == 0:
             deps.add(node)
         else:
-            todo.update(node.inputs)
+            # a node reached again through another path has been walked already (same sets as
+            # walking it again, without the repeated walks of shared sub-graphs)
+            todo.update(i for i in node.inputs if i not in seen)
         seen.add(node)
     return list(deps), seen
 
