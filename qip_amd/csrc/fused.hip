@@ -22,9 +22,23 @@ constexpr int kFusedBlocksPerCU = 2;   // 69 KB of LDS and 212 VGPRs per workgro
 // Sum `count` FP64 partials with one wavefront: lane l adds partials l, l + 64, ... in order, then a
 // butterfly over the lanes. Each butterfly step adds the same two values on both lanes of a pair, so
 // every lane, every wavefront and every workgroup that sums the same partials gets the same bits.
-__device__ __forceinline__ double wave_sum_partials(const double *__restrict__ partial, uint32_t count, int lane) {
+// The first 512 partials are loaded up front (partials_first) so a caller can issue other loads
+// before the sum waits for these; partials_sum adds any beyond 512 and reduces.
+__device__ __forceinline__ void partials_first(const double *__restrict__ partial, uint32_t count, int lane,
+                                               double (&pv)[8]) {
+    // unconditional loads (index clamped into the array, which holds at least one partial), so
+    // none of them waits for another; partials_sum drops the clamped ones
+    const uint32_t last = count ? count - 1 : 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) pv[j] = partial[min((uint32_t)lane + 64u * j, last)];
+}
+
+__device__ __forceinline__ double partials_sum(const double (&pv)[8], const double *__restrict__ partial,
+                                               uint32_t count, int lane) {
     double acc = 0.0;
-    for (uint32_t i = lane; i < count; i += 64) acc += partial[i];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc += (uint32_t)lane + 64u * j < count ? pv[j] : 0.0;
+    for (uint32_t i = (uint32_t)lane + 512u; i < count; i += 64) acc += partial[i];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
     return acc;
 }
@@ -73,9 +87,14 @@ __global__ __launch_bounds__(256, 2) void k_fused6(cx<float> *amps, const bf16x8
     uint64_t tile = (uint64_t)blockIdx.x * per;
     const uint64_t end = min(tile + per, a.ntiles);
     cx<float> va[16], vb[16];
-    if (tile < end) load_tile(va, amps, tile_base(tile, a) + off_t, a);
-    // the scale, per wavefront, while the first tile's loads are in flight
-    ctx.scale = prev_count ? 1.0 / sqrt(wave_sum_partials(prev_partial, prev_count, lane)) : 1.0;
+    // the scale, per wavefront: the partials' loads go out first, so the sum waits for them while
+    // the first tile's loads stay in flight
+    double pv[8];
+    partials_first(prev_partial, prev_count, lane, pv);
+    // unconditional (a workgroup without tiles reads the last tile and never uses it), so the wait
+    // before the sum covers the partials only on every path
+    load_tile(va, amps, tile_base(min(tile, a.ntiles - 1), a) + off_t, a);
+    ctx.scale = prev_count ? 1.0 / sqrt(partials_sum(pv, prev_partial, prev_count, lane)) : 1.0;
     if (tile < end) {
         const uint64_t last = end - 1;
         for (;;) {
@@ -97,7 +116,9 @@ __global__ __launch_bounds__(256, 2) void k_fused6(cx<float> *amps, const bf16x8
 // fstate[1] = norm of the last block's output (its partials in the same fixed order as k_fused6),
 // fstate[0] = 1 / sqrt(norm).
 __global__ __launch_bounds__(64) void k_fused_norm(const double *__restrict__ partial, uint32_t count, double *fstate) {
-    const double nrm = wave_sum_partials(partial, count, threadIdx.x);
+    double pv[8];
+    partials_first(partial, count, threadIdx.x, pv);
+    const double nrm = partials_sum(pv, partial, count, threadIdx.x);
     if (threadIdx.x == 0) {
         fstate[1] = nrm;
         fstate[0] = 1.0 / sqrt(nrm);
