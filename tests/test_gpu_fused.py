@@ -196,6 +196,69 @@ def test_recorder_orders_wide_and_direct_calls():
     assert fz.pending_gates() == 0
 
 
+def test_recorder_edges():
+    """Recorded gates and host calls: reinit drops the record; range writes, measurement and a
+    change of streaming launch it first; closing with gates recorded is fine; a bad gate is
+    rejected with nothing recorded; a complex128 state cannot record."""
+    import ctypes
+    import random
+    from qip_amd import _capi
+    from qip_amd.fused import FusedBackend
+    n = 12
+    H = numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2)
+    psi = rand_state(n)
+    # reinit drops what was recorded
+    fz = FusedBackend.make_state(n, [], [])
+    fz.streaming = False
+    fz.kronselect_dot({q: H for q in range(n)})
+    assert fz.pending_gates() == n
+    fz.reinit([list(range(n))], [psi])
+    assert fz.pending_gates() == 0
+    assert numpy.max(numpy.abs(fz.get_state() - psi)) < 1e-7
+    # a range write after recorded gates sees them applied first
+    ref = GpuBackend.make_state(n, [list(range(n))], [psi], statetype=numpy.complex64)
+    U37 = rand_u(4)
+    for be in (ref, fz):
+        be.kronselect_dot({q: H for q in range(0, n, 2)})
+        be.overwrite_relative_range(5, 9, numpy.arange(4) * 0.01)
+        be.addto_relative_range(100, 102, numpy.array([0.5, -0.5]))
+        be.kronselect_dot({(3, 7): U37})
+    want = ref.get_state().astype(numpy.complex128)
+    want /= numpy.linalg.norm(want)       # every fused block leaves the state normalised
+    assert numpy.max(numpy.abs(fz.get_state() - want)) < 1e-5
+    # measurement launches the record first: the outcome the plain state gives for the same uniform
+    # (the outcome depends only on the relative probabilities)
+    for be in (ref, fz):
+        be.kronselect_dot({1: H, 2: H})
+    random.seed(11)
+    m_ref = ref.measure([1, 2])
+    random.seed(11)
+    m_fz = fz.measure([1, 2])
+    assert m_ref[0] == m_fz[0]
+    # switching streaming back on launches the record
+    fz.kronselect_dot({0: H})
+    assert fz.pending_gates() == 1
+    fz.streaming = True
+    assert fz.pending_gates() == 0
+    fz.kronselect_dot({4: H})
+    fz.close()                              # gates still recorded: dropped with the state
+    # all or nothing: the second gate is wider than the width, the first is not recorded either
+    fz = FusedBackend.make_state(n, [], [])
+    fz.streaming = False
+    k = numpy.array([1, 3], dtype=numpy.int32)
+    q = numpy.zeros(12, dtype=numpy.int32)
+    q[6:9] = [0, 1, 2]
+    m = numpy.zeros(4 + 64, dtype=numpy.complex128)
+    lib = _capi.load()
+    assert lib.qipgpu_fused_set_recording(fz.handle, 2, 0) == 0
+    assert lib.qipgpu_fused_record(fz.handle, 2, _capi.ptr(k), _capi.ptr(q), _capi.ptr(m)) == _capi.EINVAL
+    assert fz.pending_gates() == 0
+    fz.close()
+    c128 = GpuBackend.make_state(n, [], [])
+    assert lib.qipgpu_fused_record(c128.handle, 0, None, None, None) == _capi.EINVAL
+    c128.close()
+
+
 def test_fused_backend_graph_qft():
     """A qip graph run through FusedBackend: the 14-qubit QFT equals ifft(norm='ortho')."""
     from qip_amd import QFFT, Qubit, run
