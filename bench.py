@@ -15,6 +15,8 @@ Beside it, in the same run:
   * 1 - F and max |amplitude difference| of the fused result against the complex128 result;
   * the HBM roofline of the fused kernel from HIP events on the state's stream, over one more run
     of the same circuit's blocks, composed up front and launched back to back;
+  * the memory-pattern floor of those blocks: the same tile patterns swept in place with no
+    arithmetic (qipgpu_fused_pattern_sweep), and the kernel's time over that floor;
   * cpu_baseline: the oracle's restatement of the reference's cdot_loop (oracle/liboracle_fast.so,
     the reference's -O3 -ffast-math -fopenmp flags, all visible cores) on a time-bounded sample of
     the circuit's own gates at full size, extrapolated to the circuit's gate mix.
@@ -111,6 +113,22 @@ def fused_kernel_ms(be, ops, ev):
     ev.record(t1, be.stream)
     be.synchronize()
     return ev.elapsed_ms(t0, t1) / (len(blocks) + 1), len(blocks)
+
+
+def fused_pattern_floor_ms(be, ops, ev):
+    """Average time per block of the same blocks' memory pattern alone: every amplitude loaded and
+    stored in place with each block's tile pattern and k_fused6's grid, no arithmetic
+    (qipgpu_fused_pattern_sweep), launched back to back between HIP events."""
+    from qip_amd.fused import descriptor_gates, fuse
+    blocks = fuse([g for d in ops for g in descriptor_gates(d)], be.fusion_width)
+    be.synchronize()
+    t0, t1 = ev.create(), ev.create()
+    ev.record(t0, be.stream)
+    for b in blocks:
+        be.fused_pattern_sweep(sorted(b.qubits))
+    ev.record(t1, be.stream)
+    be.synchronize()
+    return ev.elapsed_ms(t0, t1) / len(blocks)
 
 
 def plain_dominant_roofline(be, n, ev):
@@ -522,6 +540,7 @@ def main():
     timings = [fused_kernel_ms(fz, ops, ev) for _ in range(2)]
     avg_block_ms = sum(t[0] for t in timings) / len(timings)
     fused_gbs = block_bytes / (avg_block_ms * 1e-3) / 1e9
+    floor_ms = fused_pattern_floor_ms(fz, ops, ev)
 
     result = {
         "metric": "30-qubit gates/s + HBM GB/s: fused c64 vs plain c128 at 1-F<=1e-8",
@@ -550,7 +569,11 @@ def main():
         },
         "fused_c64": {"gates_per_s": fused_gates_s, "s_per_circuit": elapsed / args.steps,
                       "gpu_ms_per_circuit": avg_block_ms * (nblocks + 1), "blocks": nblocks,
-                      "algorithmic_gb_per_circuit": block_bytes * (nblocks + 1) / 1e9},
+                      "algorithmic_gb_per_circuit": block_bytes * (nblocks + 1) / 1e9,
+                      "pattern_floor": {"what": "the same blocks' tile patterns swept in place with no arithmetic",
+                                        "avg_launch_ms": floor_ms,
+                                        "gbs": block_bytes / (floor_ms * 1e-3) / 1e9,
+                                        "kernel_over_floor": avg_block_ms / floor_ms}},
     }
 
     # ---- plain complex128 on the same circuit ----------------------------------------------------

@@ -145,6 +145,10 @@ int qipgpu_reduce_measure(qipgpu_state *s, int32_t k, const int32_t *indices, do
  * multiplied by the pending FP64 renormalisation factor left by the previous fused call; the
  * norm of the result is accumulated in FP64 and becomes the next pending factor. */
 int qipgpu_fused_apply(qipgpu_state *s, int32_t k, const int32_t *targets, const double *mat);
+/* Diagnostic, with no counterpart in the reference: load and store every amplitude in place with
+ * exactly the tile pattern, grid and double buffering qipgpu_fused_apply uses for these targets, and no
+ * arithmetic. The state is unchanged; the launch time is the memory-pattern floor of such a block. */
+int qipgpu_fused_pattern_sweep(qipgpu_state *s, int32_t k, const int32_t *targets);
 /* Host only (no GPU call): the 2^k x 2^k matrix (row-major complex128) of an ordered list of gates
  * on the k <= 6 distinct qubits `qubits` (first = MSB of the result's index), out = G_last ... G_1.
  * Gate g acts on its gate_k[g] qubits gate_qubits[6 g ..] (all among `qubits`, first = MSB of its

@@ -381,6 +381,12 @@ class GpuBackend(StateType):
         _capi.check(self._lib.qipgpu_fused_apply(self._h, len(t), _capi.ptr(t), _capi.ptr(m.view(numpy.float64))),
                     "fused_apply")
 
+    def fused_pattern_sweep(self, targets: Sequence[int]) -> None:
+        """Diagnostic: load and store the state in place with the tile pattern apply_fused uses for
+        these targets, without arithmetic (the state is unchanged)."""
+        t = _capi.i32(targets)
+        _capi.check(self._lib.qipgpu_fused_pattern_sweep(self._h, len(t), _capi.ptr(t)), "fused_pattern_sweep")
+
     def fused_finish(self) -> None:
         """Apply the pending renormalisation factor of the fused path now."""
         _capi.check(self._lib.qipgpu_fused_finish(self._h), "fused_finish")

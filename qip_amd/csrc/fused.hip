@@ -125,6 +125,42 @@ __global__ __launch_bounds__(64) void k_fused_norm(const double *__restrict__ pa
     }
 }
 
+// The memory-pattern floor of a fused block: every amplitude is loaded and stored back in place
+// with exactly k_fused6's tile pattern, grid, tile ranges and double buffering, and no arithmetic, no
+// LDS. Its time against k_fused6's on the same targets separates what the tile's DRAM pattern costs
+// from what the kernel's own work adds.
+__global__ __launch_bounds__(256, 2) void k_tile_sweep(cx<float> *amps, FusedArgs a) {
+    typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f4;
+    const int tid = threadIdx.x;
+    const uint64_t off_t = u_offset<0, 9>(2u * (uint32_t)tid, a);
+    const uint64_t per = (a.ntiles + gridDim.x - 1) / gridDim.x;
+    uint64_t tile = (uint64_t)blockIdx.x * per;
+    const uint64_t end = min(tile + per, a.ntiles);
+    if (tile >= end) return;
+    const uint64_t last = end - 1;
+    auto store = [&](const cx<float> (&v)[16], uint64_t base) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            f4 p;
+            p[0] = v[2 * i].re;
+            p[1] = v[2 * i].im;
+            p[2] = v[2 * i + 1].re;
+            p[3] = v[2 * i + 1].im;
+            *reinterpret_cast<f4 *>(amps + base + u_offset<9, 12>(512u * i, a)) = p;
+        }
+    };
+    cx<float> va[16], vb[16];
+    load_tile(va, amps, tile_base(tile, a) + off_t, a);
+    for (;;) {
+        load_tile(vb, amps, tile_base(min(tile + 1, last), a) + off_t, a);
+        store(va, tile_base(tile, a) + off_t);
+        if (++tile >= end) break;
+        load_tile(va, amps, tile_base(min(tile + 1, last), a) + off_t, a);
+        store(vb, tile_base(tile, a) + off_t);
+        if (++tile >= end) break;
+    }
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_scale(cx<R> *amps, uint64_t len, const double *__restrict__ fstate) {
     const double s = fstate[0];
@@ -255,6 +291,24 @@ int qipgpu_fused_apply(qipgpu_state *s, int32_t k, const int32_t *targets, const
     QIP_LAUNCHED();
     s->fused_prev_count = grid;
     s->fused_parity ^= 1;
+    return QIPGPU_OK;
+}
+
+int qipgpu_fused_pattern_sweep(qipgpu_state *s, int32_t k, const int32_t *targets) {
+    if (s->dtype != QIPGPU_C64) return set_error(QIPGPU_EINVAL, "fused blocks apply to complex64 states");
+    if (s->n < kTileBits) return set_error(QIPGPU_EUNSUP, "fused blocks need n >= %d (n=%d)", kTileBits, s->n);
+    if (k < 1 || k > 6) return set_error(QIPGPU_EINVAL, "fused block on %d qubits (1..6)", k);
+    int rc = check_qubits(s, k, targets, "fused block qubits");
+    if (rc) return rc;
+    rc = flush_recorded(s);
+    if (rc) return rc;
+    uint64_t tmask = 0;
+    for (int j = 0; j < k; ++j) tmask |= 1ull << (s->n - 1 - targets[j]);
+    int ts[6], ls[6];
+    choose_tile_bits(s->n, tmask, ts, ls);
+    const FusedArgs a = make_tile_args(s->n, ts, ls);
+    hipLaunchKernelGGL(k_tile_sweep, dim3(fused_grid(a.ntiles)), dim3(256), 0, s->stream, (cx<float> *)s->amps, a);
+    QIP_LAUNCHED();
     return QIPGPU_OK;
 }
 

@@ -268,6 +268,7 @@ def test_fused_backend_graph_qft():
     q = Qubit(n=n)
     o, _ = run(QFFT(q), feed={q: psi}, backend_constructor=FusedBackend.make_state)
     assert o.dtype == numpy.complex64
+    o = o.astype(numpy.complex128)         # norms in FP64: a complex64 vdot is good to ~1e-7 only
     expect = numpy.fft.ifft(psi, norm="ortho")
     ov = numpy.vdot(expect, o)
     assert 1 - abs(ov) ** 2 / numpy.vdot(o, o).real < 1e-8
@@ -299,3 +300,18 @@ def test_fused_backend_24_qubits_against_oracle():
     print("24-qubit fused vs oracle: 1-F", inf, flush=True)
     assert inf < 1e-8
     assert numpy.max(numpy.abs(got - v)) < 1e-6
+
+
+def test_pattern_sweep_leaves_state_unchanged():
+    """The pattern-floor diagnostic moves every amplitude in place without changing it, for target
+    sets of every kind (low, high, mixed, fewer than six)."""
+    n = 14
+    psi = rand_state(n)
+    fz = GpuBackend.make_state(n, [list(range(n))], [psi], statetype=numpy.complex64)
+    before = fz.get_state()
+    for t in ([0, 1, 2, 3, 4, 5], [13, 12, 11, 10, 9, 8], [0, 13, 3, 9, 6, 11], [7], [2, 11]):
+        fz.fused_pattern_sweep(t)
+    assert numpy.array_equal(fz.get_state(), before)
+    with pytest.raises(ValueError):
+        fz.fused_pattern_sweep([0, 0])
+    fz.close()
