@@ -164,7 +164,10 @@ int qipgpu_compose(int32_t k, const int32_t *qubits, int32_t n_gates, const int3
  * blocks of at most `width` qubits: a gate joins the first block, from the last one touching its
  * qubits on, that stays within the width, else opens a block. A block that no later gate can join is
  * composed (qipgpu_compose) and launched at once when streaming (qipgpu_fused_apply for n >= 12, the
- * plain dense sweep below). Every other entry point that touches the state, and qipgpu_synchronize,
+ * plain dense sweep below). For n >= 12 a run of consecutive blocks holding single-qubit gates only
+ * is one product of per-qubit 2x2s on the product sweep (qipgpu_apply_product, FP64 arithmetic); a
+ * trailing run waits until the block after it is final, so runs launch whole. Every other entry
+ * point that touches the state, and qipgpu_synchronize,
  * launches what is recorded first; qipgpu_init_product drops it. All gates are checked before any is
  * recorded. */
 int qipgpu_fused_record(qipgpu_state *s, int32_t n_gates, const int32_t *gate_k, const int32_t *gate_qubits,
@@ -178,7 +181,8 @@ int qipgpu_fused_flush(qipgpu_state *s);
 int qipgpu_fused_recorded(const qipgpu_state *s, int64_t *pending_gates, int64_t *blocks_launched);
 /* Host only (no GPU call): the recorder's streaming fusion plan for a gate list on n qubits (<= 64).
  * gate_block[g] = global number of gate g's block (blocks numbered from 0 in launch order);
- * final_after[g] = number of blocks final (launched when streaming) once gate g is recorded. */
+ * final_after[g] = number of blocks final once gate g is recorded (launched when streaming, but for
+ * a trailing run of single-qubit-only blocks). */
 int qipgpu_fuse_online(int32_t n, int32_t width, int32_t n_gates, const int32_t *gate_k, const int32_t *gate_qubits,
                        int64_t *gate_block, int64_t *final_after);
 /* Launch recorded gates, then apply the pending renormalisation factor (if any) with one scaling

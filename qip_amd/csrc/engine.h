@@ -39,6 +39,11 @@ __device__ __forceinline__ uint64_t insert_zeros(uint64_t g, const PosList &pl) 
 
 struct Recorder;   // recorder.hip: gates recorded for online fusion
 
+// Product sweeps (product.hip): every pass keeps the kProductLowBits lowest state bits in its tile and
+// takes at most kProductHighPerPass factors above them.
+constexpr int kProductLowBits = 4;
+constexpr int kProductHighPerPass = 8;
+
 constexpr int kRingSlots = 8;
 constexpr size_t kRingSlotBytes = 64 * 64 * 16 + 4096;  // a 64x64 complex128 matrix + offsets
 

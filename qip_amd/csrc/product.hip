@@ -4,15 +4,19 @@
 // A descriptor such as H on a whole register (qip/operators.py:97-99) is one operator in the
 // reference, applied in one cdot_loop sweep. Here its single-qubit factors commute and are applied
 // tile by tile: the state is cut into tiles of 4096 amplitudes spanning the target bits plus the
-// lowest free bits (12 tile bits P). Every pass keeps the four lowest state bits in P, so a tile is
-// read in runs of at least 16 amplitudes (256 B for complex128): a pass takes at most 8 targets
-// above bit 3, and a product with more is applied in several passes. A workgroup loads a tile in
-// address order (u = tile bits ascending) into LDS as FP64, applies the gates three target bits
-// per round (each thread takes groups of 8 amplitudes that differ only in the round's bits and
-// applies the round's 2x2 gates in registers), and writes the tile back once: one HBM read and one
-// write per amplitude for up to 12 gates, and for a complex64 state one rounding instead of one
-// per gate. LDS rows of 8 amplitudes are padded by one slot so that every access pattern of a
-// round spreads over the banks.
+// lowest free bits (12 tile bits P, u = tile bits ascending). Every pass keeps the four lowest
+// state bits in P, so a tile is read in runs of at least 16 amplitudes: a pass takes at most 8
+// targets above bit 3, and a product with more is applied in several passes.
+//
+// Each thread holds 16 amplitudes of the tile in FP64 registers, and a factor on u bit b is applied
+// where b is one of the four bits that vary across those 16:
+//   layout A (as loaded and stored, address order): u = tid + 256 e, u bits 8..11 = e;
+//   layout B: u bits 4..7 = e;  layout C: u bits 0..3 = e.
+// The tile moves A -> B -> C -> A through LDS only as far as the factors need: factors on u bits
+// 8..11 only cost no LDS traffic at all, factors on 4..11 two transposes. One HBM read and one write
+// per amplitude for up to 12 gates, and for a complex64 state one rounding instead of one per gate.
+// LDS rows of 16 amplitudes are padded by one slot, so that the 16-amplitude-stride reads of layout
+// C spread over the banks.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -26,20 +30,18 @@ namespace {
 
 constexpr int kTileBits = 12;
 constexpr int kTile = 1 << kTileBits;
-constexpr int kPadded = kTile + kTile / 8;
-constexpr int kMaxRounds = 4;
-constexpr int kBlocksPerCU = 2;   // 72 KB of LDS per workgroup
+constexpr int kPadded = kTile + kTile / 16;
+constexpr int kBlocksPerCU = 2;   // 68 KB of LDS per workgroup
 
 struct ProductArgs {
     uint64_t ntiles;
     uint8_t pbits[12];                 // u bit i <-> state bit pbits[i], ascending
-    int32_t nrounds;
-    int32_t rcount[kMaxRounds];        // target bits in each round (1..3)
-    uint8_t rbits[kMaxRounds][3];      // their u bit positions, ascending
-    cx<double> g[kMaxRounds][3][4];    // the gate on rbits[r][j]: row-major 2x2
+    uint32_t fmask;                    // u bits that carry a factor
+    uint32_t rmask;                    // u bits whose factor is a real matrix (H, X, ...)
+    cx<double> g[12][4];               // the factor on u bit i: row-major 2x2
 };
 
-__device__ __forceinline__ int ppos(uint32_t u) { return (int)(u + (u >> 3)); }
+__device__ __forceinline__ int ppos(uint32_t u) { return (int)(u + (u >> 4)); }
 
 __device__ __forceinline__ uint64_t tile_base(uint64_t t, const ProductArgs &a) {
 #pragma unroll
@@ -66,85 +68,115 @@ __device__ __forceinline__ cx<double> mac2(cx<double> m0, cx<double> x0, cx<doub
     return r;
 }
 
-// One round over the tile in LDS: CNT target bits, 8-amplitude (or 4, 2) groups per thread.
-template <int CNT>
-__device__ __forceinline__ void product_round(cx<double> *t, const ProductArgs &a, int r) {
-    constexpr int G = 1 << CNT;                 // amplitudes per group
-    constexpr int PER = (kTile / G) / 256;      // groups per thread
-    uint32_t b[3];
+// The factors on u bits B..B+3 applied to 16 registers whose index e is u bits B..B+3.
+template <int B>
+__device__ __forceinline__ void apply4(cx<double> (&x)[16], const ProductArgs &a) {
 #pragma unroll
-    for (int j = 0; j < CNT; ++j) b[j] = a.rbits[r][j];
+    for (int j = 0; j < 4; ++j) {
+        if (!((a.fmask >> (B + j)) & 1u)) continue;
+        const cx<double> m00 = a.g[B + j][0], m01 = a.g[B + j][1], m10 = a.g[B + j][2], m11 = a.g[B + j][3];
+        if ((a.rmask >> (B + j)) & 1u) {   // real matrix: half the arithmetic
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        uint32_t gi = threadIdx.x + 256u * q;
-#pragma unroll
-        for (int j = 0; j < CNT; ++j) {           // insert zeros at the round's bits, ascending
-            const uint32_t lo = gi & ((1u << b[j]) - 1u);
-            gi = ((gi ^ lo) << 1) | lo;
-        }
-        cx<double> x[G];
-#pragma unroll
-        for (int e = 0; e < G; ++e) {
-            uint32_t u = gi;
-#pragma unroll
-            for (int j = 0; j < CNT; ++j) u |= (uint32_t)((e >> j) & 1) << b[j];
-            x[e] = t[ppos(u)];
-        }
-#pragma unroll
-        for (int j = 0; j < CNT; ++j) {
-            const cx<double> m00 = a.g[r][j][0], m01 = a.g[r][j][1], m10 = a.g[r][j][2], m11 = a.g[r][j][3];
-#pragma unroll
-            for (int e = 0; e < G; ++e) {
+            for (int e = 0; e < 16; ++e) {
                 if ((e >> j) & 1) continue;
                 const int f = e | (1 << j);
                 const cx<double> x0 = x[e], x1 = x[f];
-                x[e] = mac2(m00, x0, m01, x1);
-                x[f] = mac2(m10, x0, m11, x1);
+                x[e].re = m00.re * x0.re + m01.re * x1.re;
+                x[e].im = m00.re * x0.im + m01.re * x1.im;
+                x[f].re = m10.re * x0.re + m11.re * x1.re;
+                x[f].im = m10.re * x0.im + m11.re * x1.im;
             }
+            continue;
         }
 #pragma unroll
-        for (int e = 0; e < G; ++e) {
-            uint32_t u = gi;
-#pragma unroll
-            for (int j = 0; j < CNT; ++j) u |= (uint32_t)((e >> j) & 1) << b[j];
-            t[ppos(u)] = x[e];
+        for (int e = 0; e < 16; ++e) {
+            if ((e >> j) & 1) continue;
+            const int f = e | (1 << j);
+            const cx<double> x0 = x[e], x1 = x[f];
+            x[e] = mac2(m00, x0, m01, x1);
+            x[f] = mac2(m10, x0, m11, x1);
         }
     }
 }
 
+// Layouts B and C: the u of register e of thread tid.
+__device__ __forceinline__ uint32_t u_b(uint32_t tid, int e) { return (tid & 15u) | ((uint32_t)e << 4) | ((tid >> 4) << 8); }
+__device__ __forceinline__ uint32_t u_c(uint32_t tid, int e) { return (uint32_t)e | (tid << 4); }
+
+template <typename R>
+__device__ __forceinline__ void load_raw(cx<R> (&v)[16], const cx<R> *amps, uint64_t base, const ProductArgs &a) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = amps[base + u_offset<8, 12>(256u * i, a)];
+}
+
+// One tile: its 16 raw amplitudes per thread (layout A) in, widened, the factors applied, rounded
+// and stored back.
+template <typename R>
+__device__ __forceinline__ void product_tile(const cx<R> (&v)[16], cx<R> *amps, uint64_t base, cx<double> *t,
+                                             const ProductArgs &a, bool use_b, bool use_c, uint32_t tid) {
+    cx<double> x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        x[i].re = (double)v[i].re;
+        x[i].im = (double)v[i].im;
+    }
+    apply4<8>(x, a);
+    if (use_b || use_c) {
+        // Every phase reads and writes only this thread's own positions of that phase's layout, and
+        // a barrier separates phases: so the previous tile's last (A) reads need no barrier before
+        // these A writes, and each phase's writes go back where it read
+#pragma unroll
+        for (int i = 0; i < 16; ++i) t[ppos(tid + 256u * i)] = x[i];
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 16; ++e) x[e] = t[ppos(u_b(tid, e))];
+        apply4<4>(x, a);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) t[ppos(u_b(tid, e))] = x[e];
+        if (use_c) {
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < 16; ++e) x[e] = t[ppos(u_c(tid, e))];
+            apply4<0>(x, a);
+#pragma unroll
+            for (int e = 0; e < 16; ++e) t[ppos(u_c(tid, e))] = x[e];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = t[ppos(tid + 256u * i)];
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        cx<R> o;
+        o.re = (R)x[i].re;
+        o.im = (R)x[i].im;
+        amps[base + u_offset<8, 12>(256u * i, a)] = o;
+    }
+}
+
+// Persistent grid: each workgroup takes a contiguous range of tiles, with the next tile's loads in
+// flight in a second register buffer while one is processed (the last prefetch re-reads an
+// in-range tile and is not used).
 template <typename R>
 __global__ __launch_bounds__(256, 2) void k_product(cx<R> *amps, ProductArgs a) {
     __shared__ cx<double> t[kPadded];
     const uint32_t tid = threadIdx.x;
-    const uint64_t off_t = u_offset<0, 8>(tid, a);       // u bits 0..7 are the thread index
-    for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
-        const uint64_t base = tile_base(tile, a) + off_t;
-        cx<R> v[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) v[i] = amps[base + u_offset<8, 12>(256u * i, a)];
-        __syncthreads();   // the previous tile has left LDS
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            cx<double> w;
-            w.re = (double)v[i].re;
-            w.im = (double)v[i].im;
-            t[ppos(tid + 256u * i)] = w;
-        }
-        __syncthreads();
-        for (int r = 0; r < a.nrounds; ++r) {
-            if (a.rcount[r] == 3) product_round<3>(t, a, r);
-            else if (a.rcount[r] == 2) product_round<2>(t, a, r);
-            else product_round<1>(t, a, r);
-            __syncthreads();
-        }
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const cx<double> w = t[ppos(tid + 256u * i)];
-            cx<R> o;
-            o.re = (R)w.re;
-            o.im = (R)w.im;
-            amps[base + u_offset<8, 12>(256u * i, a)] = o;
-        }
+    const uint64_t off_t = u_offset<0, 8>(tid, a);       // layout A: u bits 0..7 are the thread index
+    const bool use_b = (a.fmask & 0xf0u) != 0, use_c = (a.fmask & 0x0fu) != 0;
+    const uint64_t per = (a.ntiles + gridDim.x - 1) / gridDim.x;
+    uint64_t tile = (uint64_t)blockIdx.x * per;
+    const uint64_t end = min(tile + per, a.ntiles);
+    if (tile >= end) return;
+    const uint64_t last = end - 1;
+    cx<R> va[16], vb[16];
+    load_raw(va, amps, tile_base(tile, a) + off_t, a);
+    for (;;) {
+        load_raw(vb, amps, tile_base(min(tile + 1, last), a) + off_t, a);
+        product_tile(va, amps, tile_base(tile, a) + off_t, t, a, use_b, use_c, tid);
+        if (++tile >= end) break;
+        load_raw(va, amps, tile_base(min(tile + 1, last), a) + off_t, a);
+        product_tile(vb, amps, tile_base(tile, a) + off_t, t, a, use_b, use_c, tid);
+        if (++tile >= end) break;
     }
 }
 
@@ -155,11 +187,12 @@ uint32_t product_grid(uint64_t ntiles) {
         if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0) cus = p.multiProcessorCount;
     }
     (void)hipGetLastError();
-    return (uint32_t)std::min<uint64_t>(ntiles, (uint64_t)cus * kBlocksPerCU * 4);
+    return (uint32_t)std::min<uint64_t>(ntiles, (uint64_t)cus * kBlocksPerCU);
 }
 
-constexpr int kLowBits = 4;         // state bits every pass keeps in its tile
-constexpr int kHighPerPass = kTileBits - kLowBits;
+constexpr int kLowBits = kProductLowBits;           // state bits every pass keeps in its tile
+constexpr int kHighPerPass = kProductHighPerPass;   // = kTileBits - kLowBits
+static_assert(kLowBits + kHighPerPass == kTileBits, "a pass tile holds 12 state bits");
 
 // One pass: the k single-qubit gates (k <= 12, all targets of at most kHighPerPass above the low
 // bits) in one sweep.
@@ -182,22 +215,15 @@ int product_pass(qipgpu_state *s, int32_t k, const int32_t *targets, const doubl
     std::memset(&a, 0, sizeof(a));
     a.ntiles = 1ull << (n - kTileBits);
     for (int i = 0; i < kTileBits; ++i) a.pbits[i] = (uint8_t)pbits[i];
-    // rounds of up to three targets, in u-bit order
-    std::vector<std::pair<int, int>> tu;   // (u bit, gate index)
-    for (int j = 0; j < k; ++j)
-        tu.push_back({(int)(std::find(pbits.begin(), pbits.end(), tbits[j]) - pbits.begin()), j});
-    std::sort(tu.begin(), tu.end());
-    a.nrounds = (k + 2) / 3;
-    for (int r = 0; r < a.nrounds; ++r) {
-        const int lo = 3 * r, hi = std::min(k, 3 * r + 3);
-        a.rcount[r] = hi - lo;
-        for (int j = lo; j < hi; ++j) {
-            a.rbits[r][j - lo] = (uint8_t)tu[j].first;
-            const double *m = mats + 8 * tu[j].second;
-            for (int e = 0; e < 4; ++e) {
-                a.g[r][j - lo][e].re = m[2 * e];
-                a.g[r][j - lo][e].im = m[2 * e + 1];
-            }
+    // each factor sits on the u bit of its target
+    for (int j = 0; j < k; ++j) {
+        const int ub = (int)(std::find(pbits.begin(), pbits.end(), tbits[j]) - pbits.begin());
+        a.fmask |= 1u << ub;
+        const double *m = mats + 8 * j;
+        if (m[1] == 0.0 && m[3] == 0.0 && m[5] == 0.0 && m[7] == 0.0) a.rmask |= 1u << ub;
+        for (int e = 0; e < 4; ++e) {
+            a.g[ub][e].re = m[2 * e];
+            a.g[ub][e].im = m[2 * e + 1];
         }
     }
     const uint32_t grid = product_grid(a.ntiles);

@@ -80,6 +80,7 @@ struct OnlineFuser {
 struct RecBlock {
     std::vector<int32_t> k, q;     // per gate: width, and its qubits (six slots)
     std::vector<double> mats;      // gate matrices in order, interleaved complex
+    bool product = true;           // single-qubit gates only
 };
 
 }  // namespace
@@ -107,12 +108,17 @@ int validate_gate(const qipgpu_state *s, int width, int32_t g, int32_t k, const 
     return QIPGPU_OK;
 }
 
-struct Composed {
-    std::vector<int32_t> qubits;
-    std::vector<double> mat;
+// One launch: a dense block (composed matrix on its qubits) or a run of consecutive single-qubit-only
+// blocks, which is a product of one 2x2 per qubit and goes to the product sweep (FP64 arithmetic,
+// up to 4 low + 8 other factors per pass) instead of the split-BF16 MFMA kernel.
+struct Launch {
+    bool product = false;
+    std::vector<int32_t> qubits;   // dense: the block's qubits ascending; product: the factors' qubits ascending
+    std::vector<double> mat;       // dense: 2^k x 2^k; product: one 2x2 per qubit
 };
 
-int compose(const RecBlock &b, uint64_t mask, Composed &c) {
+int compose_dense(const RecBlock &b, uint64_t mask, Launch &c) {
+    c.product = false;
     c.qubits.clear();
     for (int q = 0; q < 64; ++q)
         if (mask >> q & 1) c.qubits.push_back(q);
@@ -122,19 +128,82 @@ int compose(const RecBlock &b, uint64_t mask, Composed &c) {
                           c.mat.data());
 }
 
-int launch(qipgpu_state *s, const Composed &c) {
-    const int k = (int)c.qubits.size();
-    if (s->n >= kFusedMinQubits) return qipgpu_fused_apply(s, k, c.qubits.data(), c.mat.data());
-    return qipgpu_apply_matrix(s, k, c.qubits.data(), c.mat.data(), 0, nullptr);
+// The per-qubit products of blocks [i0, i1), all single-qubit-only, each qubit's gates in program
+// order (blocks are in program order on every qubit), each product composed by qipgpu_compose.
+int compose_product(const std::deque<RecBlock> &blocks, size_t i0, size_t i1, Launch &c) {
+    c.product = true;
+    c.qubits.clear();
+    c.mat.clear();
+    std::vector<std::vector<double>> per(64);
+    for (size_t i = i0; i < i1; ++i) {
+        const RecBlock &b = blocks[i];
+        for (size_t g = 0; g < b.k.size(); ++g)
+            per[b.q[6 * g]].insert(per[b.q[6 * g]].end(), b.mats.begin() + 8 * g, b.mats.begin() + 8 * g + 8);
+    }
+    for (int q = 0; q < 64; ++q) {
+        if (per[q].empty()) continue;
+        const int32_t ng = (int32_t)(per[q].size() / 8);
+        std::vector<int32_t> gk(ng, 1), gq(6 * (size_t)ng, 0);
+        for (int32_t g = 0; g < ng; ++g) gq[6 * g] = q;
+        double out[8];
+        const int rc = qipgpu_compose(1, &q, ng, gk.data(), gq.data(), per[q].data(), out);
+        if (rc) return rc;
+        c.qubits.push_back(q);
+        c.mat.insert(c.mat.end(), out, out + 8);
+    }
+    return QIPGPU_OK;
 }
 
-// Compose the first `count` open blocks, take them out, then launch them back to back.
+int launch(qipgpu_state *s, const Launch &c) {
+    const int k = (int)c.qubits.size();
+    if (!c.product) {
+        if (s->n >= kFusedMinQubits) return qipgpu_fused_apply(s, k, c.qubits.data(), c.mat.data());
+        return qipgpu_apply_matrix(s, k, c.qubits.data(), c.mat.data(), 0, nullptr);
+    }
+    // one product call per pass: the factors on the kProductLowBits lowest state bits with up to
+    // kProductHighPerPass others, then kProductHighPerPass more per call
+    std::vector<int> low, high;
+    for (int j = 0; j < k; ++j) (s->n - 1 - c.qubits[j] < kProductLowBits ? low : high).push_back(j);
+    size_t next = 0;
+    bool first = true;
+    while (first || next < high.size()) {
+        std::vector<int> pick(first ? low : std::vector<int>());
+        for (int t = 0; t < kProductHighPerPass && next < high.size(); ++t) pick.push_back(high[next++]);
+        first = false;
+        if (pick.empty()) continue;
+        std::vector<int32_t> t;
+        std::vector<double> m;
+        for (int j : pick) {
+            t.push_back(c.qubits[j]);
+            m.insert(m.end(), c.mat.begin() + 8 * j, c.mat.begin() + 8 * j + 8);
+        }
+        const int rc = qipgpu_apply_product(s, (int32_t)t.size(), t.data(), m.data());
+        if (rc) return rc;
+    }
+    return QIPGPU_OK;
+}
+
+// Product runs exist only where the product sweep does (n >= 12).
+bool product_runs(const qipgpu_state *s) { return s->n >= kFusedMinQubits; }
+
+// Compose the first `count` open blocks, take them out, then launch them back to back. A run of
+// consecutive single-qubit-only blocks is one product launch.
 int launch_leading(qipgpu_state *s, int64_t count) {
     Recorder *r = s->recorder;
     if (count <= 0) return QIPGPU_OK;
-    std::vector<Composed> cs((size_t)count);
-    for (int64_t i = 0; i < count; ++i) {
-        const int rc = compose(r->blocks[(size_t)i], r->fuser.masks[(size_t)i], cs[(size_t)i]);
+    std::vector<Launch> ls;
+    for (size_t i = 0; i < (size_t)count;) {
+        ls.emplace_back();
+        int rc;
+        if (product_runs(s) && r->blocks[i].product) {
+            size_t j = i;
+            while (j < (size_t)count && r->blocks[j].product) ++j;
+            rc = compose_product(r->blocks, i, j, ls.back());
+            i = j;
+        } else {
+            rc = compose_dense(r->blocks[i], r->fuser.masks[i], ls.back());
+            ++i;
+        }
         if (rc) return rc;
     }
     for (int64_t i = 0; i < count; ++i) {
@@ -142,14 +211,23 @@ int launch_leading(qipgpu_state *s, int64_t count) {
         r->blocks.pop_front();
     }
     r->fuser.take(count);
+    r->launched += count;
     r->in_flush = true;
     int rc = QIPGPU_OK;
-    for (int64_t i = 0; i < count && rc == QIPGPU_OK; ++i) {
-        rc = launch(s, cs[(size_t)i]);
-        if (rc == QIPGPU_OK) ++r->launched;
-    }
+    for (size_t i = 0; i < ls.size() && rc == QIPGPU_OK; ++i) rc = launch(s, ls[i]);
     r->in_flush = false;
     return rc;
+}
+
+// Final blocks that can launch now: all of them, except a trailing run of single-qubit-only blocks,
+// which waits until the block after it is final too (or the next flush), so that a run is always
+// launched whole and the launches do not depend on when blocks became final.
+int64_t launchable(const qipgpu_state *s, int64_t final_count) {
+    const Recorder *r = s->recorder;
+    int64_t c = final_count;
+    if (!product_runs(s) || c == 0 || c == (int64_t)r->blocks.size()) return c;
+    while (c > 0 && r->blocks[(size_t)c - 1].product && r->blocks[(size_t)c].product) --c;
+    return c;
 }
 
 int ensure_recorder(qipgpu_state *s) {
@@ -222,10 +300,11 @@ int qipgpu_fused_record(qipgpu_state *s, int32_t n_gates, const int32_t *gate_k,
         b.q.insert(b.q.end(), q, q + 6);
         const size_t len = (size_t)2 << (2 * k);
         b.mats.insert(b.mats.end(), mats + moff, mats + moff + len);
+        if (k > 1) b.product = false;
         moff += len;
         ++r->pending;
         if (r->streaming) {
-            rc = launch_leading(s, r->fuser.final_count());
+            rc = launch_leading(s, launchable(s, r->fuser.final_count()));
             if (rc) return rc;
         }
     }

@@ -147,27 +147,87 @@ def test_fused_backend_layered_circuit():
 
 
 @pytest.mark.parametrize("n", [16, 10])
-def test_recorder_blocks_equal_python_plan(n):
+def _apply_python_plan(ref, n, blocks):
+    """The recorder's launches restated: each block composed by qipgpu_compose and applied (MFMA
+    kernel at n >= 12, plain dense sweep below); at n >= 12 a run of consecutive single-qubit-only
+    blocks is one product of per-qubit 2x2s (each composed by qipgpu_compose) on the product sweep."""
+    from qip_amd.fused import compose
+    i = 0
+    while i < len(blocks):
+        if n >= 12 and all(len(gq) == 1 for gq, _ in blocks[i].gates):
+            per = {}
+            while i < len(blocks) and all(len(gq) == 1 for gq, _ in blocks[i].gates):
+                for gq, G in blocks[i].gates:
+                    per.setdefault(gq[0], []).append((gq, G))
+                i += 1
+            qs = sorted(per)
+            ref.apply_product(qs, [compose([q], per[q]) for q in qs])
+            continue
+        qs, M = blocks[i].matrix()
+        if n >= 12:
+            ref.apply_fused(qs, M)
+        else:
+            ref.apply_matrix(qs, M)
+        i += 1
+
+
+@pytest.mark.parametrize("n,streaming", [(16, True), (16, False), (10, True)])
+def test_recorder_blocks_equal_python_plan(n, streaming):
     """The library's recorder launches exactly the blocks of the Python statement of the greedy
-    rule: the same blocks composed by qipgpu_compose and applied one by one (the MFMA kernel at
-    n >= 12, the plain dense sweep below) give a bitwise equal state."""
+    rule, in the same launches: a bitwise equal state, streamed or deferred."""
     from qip_amd.fused import FusedBackend, descriptor_gates, fuse
     from qip_amd.workloads import layered_random_circuit
     ops = layered_random_circuit(n, 10, seed=n)
     fz = FusedBackend.make_state(n, [], [])
+    fz.streaming = streaming
     for d in ops:
         fz.kronselect_dot(d)
     got = fz.get_state()
     blocks = fuse([g for d in ops for g in descriptor_gates(d)], 6)
     assert fz.blocks_applied == len(blocks)
     ref = GpuBackend.make_state(n, [], [], statetype=numpy.complex64)
-    for b in blocks:
-        qs, M = b.matrix()
-        if n >= 12:
-            ref.apply_fused(qs, M)
-        else:
-            ref.apply_matrix(qs, M)
+    _apply_python_plan(ref, n, blocks)
     assert numpy.array_equal(got, ref.get_state())
+
+
+def test_single_qubit_runs_take_the_product_sweep():
+    """Consecutive blocks of single-qubit gates only are one product of per-qubit 2x2s applied with
+    FP64 arithmetic: the same bits as the Python restatement, and (on random single-qubit layers)
+    much closer to the complex128 result than the same blocks on split-BF16 MFMA."""
+    from qip_amd.fused import FusedBackend, descriptor_gates, fuse
+    n = 18
+    psi = rand_state(n)
+    H = numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2)
+    ops = [{q: H for q in range(17)}, {(3, 4): rand_u(4)}, {q: rand_u(2) for q in range(0, 17, 2)},
+           {q: H for q in range(17)}]
+    fz = FusedBackend.make_state(n, [list(range(n))], [psi])
+    for d in ops:
+        fz.kronselect_dot(d)
+    got = fz.get_state()
+    blocks = fuse([g for d in ops for g in descriptor_gates(d)], 6)
+    kinds = [all(len(gq) == 1 for gq, _ in b.gates) for b in blocks]
+    assert kinds == [False, True, True]      # the 2-qubit gate lands in the first block
+    ref = GpuBackend.make_state(n, [list(range(n))], [psi], statetype=numpy.complex64)
+    _apply_python_plan(ref, n, blocks)
+    assert numpy.array_equal(got, ref.get_state())
+    exact = GpuBackend.make_state(n, [list(range(n))], [psi])
+    for d in ops:
+        exact.kronselect_dot(d)
+    assert numpy.max(numpy.abs(exact.get_state() - got)) < 2e-7
+    # random single-qubit layers only: every block a product; against the blocks on MFMA
+    ops = [{q: rand_u(2) for q in range(n)} for _ in range(3)]
+    fz.reinit([list(range(n))], [psi])
+    mf = GpuBackend.make_state(n, [list(range(n))], [psi], statetype=numpy.complex64)
+    exact.reinit([list(range(n))], [psi])
+    for d in ops:
+        fz.kronselect_dot(d)
+        exact.kronselect_dot(d)
+    for b in fuse([g for d in ops for g in descriptor_gates(d)], 6):
+        mf.apply_fused(*b.matrix())
+    e = exact.get_state()
+    err_product = numpy.max(numpy.abs(fz.get_state() - e))
+    err_mfma = numpy.max(numpy.abs(mf.get_state() - e))
+    assert err_product < err_mfma / 10, (err_product, err_mfma)
 
 
 def test_recorder_orders_wide_and_direct_calls():

@@ -47,3 +47,31 @@ def test_product_small_state_falls_back():
     ref = numpy.zeros(2 ** n, dtype=numpy.complex128)
     oracle.kronselect_dot(mats, psi, n, ref)
     assert numpy.max(numpy.abs(be.get_state() - ref)) < 1e-13
+
+
+def rand_o2():
+    """A random real orthogonal 2x2 (the product sweep's real-matrix arithmetic)."""
+    th = RNG.uniform(0, 2 * numpy.pi)
+    return numpy.array([[numpy.cos(th), -numpy.sin(th)], [numpy.sin(th), numpy.cos(th)]])
+
+
+@pytest.mark.parametrize("dtype,tol", [(numpy.complex128, 1e-13), (numpy.complex64, 1e-6)])
+@pytest.mark.parametrize("n,k", [(13, 12), (16, 9)])
+def test_product_real_and_complex_factors(n, k, dtype, tol):
+    """Real factors (H, X, rotations) take the real-matrix arithmetic, mixed with complex ones in
+    one sweep; the factors land on tile bits of every register layout."""
+    psi = RNG.standard_normal(2 ** n) + 1j * RNG.standard_normal(2 ** n)
+    psi /= numpy.linalg.norm(psi)
+    qs = [int(q) for q in RNG.permutation(n)[:k]]
+    H = numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2)
+    X = numpy.array([[0.0, 1.0], [1.0, 0.0]])
+    choice = [H, X, None, None]
+    mats = {}
+    for i, q in enumerate(qs):
+        m = choice[i % 4]
+        mats[q] = (rand_o2() if i % 4 == 2 else rand_u2()) if m is None else m
+    be = GpuBackend.make_state(n, [list(range(n))], [psi], statetype=dtype)
+    be.kronselect_dot(mats)
+    ref = numpy.zeros(2 ** n, dtype=numpy.complex128)
+    oracle.kronselect_dot(mats, psi, n, ref)
+    assert numpy.max(numpy.abs(be.get_state() - ref)) < tol * 2 ** (-n / 2) * 50
