@@ -2,17 +2,22 @@
 
 ``FusedBackend`` is a StateType (qip/backend.py:14-65) like GpuBackend, but kronselect_dot only
 records: each engine call of the plan (targets, controls, matrix) becomes a dense gate on its
-controls + targets. Recorded gates are fused greedily into blocks of at most ``fusion_width``
-qubits and the blocks are applied by the split-BF16 MFMA kernel (qipgpu_fused_apply). A block is
-launched as soon as no later gate can join it (streaming), and every remaining block when the state
+controls + targets, and the descriptor's gates go to the library in one qipgpu_fused_record call.
+The library fuses them greedily into blocks of at most ``fusion_width`` qubits and applies each
+block with the split-BF16 MFMA kernel (qipgpu_fused_apply); a run of consecutive blocks made only of
+single-qubit gates is one product of per-qubit 2x2s on the FP64 product sweep instead. A block is
+launched as soon as no later gate can join it (streaming), and everything recorded before the state
 is next read or changed any other way (func_apply, measurement, probabilities, get_state, ranges).
-Every fused block renormalises the previous block's output with its FP64 norm.
+Every MFMA block renormalises the previous block's output with its FP64 norm. Gates wider than the
+fusion width (a multi-controlled gate, say) are applied unfused by the plain path, in order.
 
 Greedy fusion. Gates are taken in program order. For gate g on qubit set S let j be the last
 block that touches S. g may join block j or any later block (those do not touch S, so g commutes
 past them) whose qubit set stays within the width; the first that fits is taken, else g opens a
 new block at the end. A block's matrix is the ordered product of its gates embedded in the
 block's qubits. On the 30-qubit depth-40 brickwork circuit this gives 128 blocks for 1780 gates.
+``OnlineFusion`` and ``fuse`` below state the rule in Python (the library's recorder is checked
+against them); ``compose`` is the library's block composition.
 
 Streaming. The rule is online: a gate's placement depends only on the blocks before it. A later gate
 scans from the last block touching one of its qubits, so once every qubit of the state has been
