@@ -179,24 +179,12 @@ double *fused_partials_of(qipgpu_state *s, int parity) {
 
 // d_fused: [0] scale and [1] norm written by k_fused_norm, then two partial arrays of `cap` doubles
 // that alternate between consecutive blocks (a block reads the other array while it writes its own).
+// A state's fused grid depends only on its size and the device, so the arrays are sized once.
 int ensure_fused_state(qipgpu_state *s, uint32_t nblocks) {
-    if (s->d_fused && s->fused_partials >= nblocks) return QIPGPU_OK;
     if (s->d_fused) {
-        QIP_HIP(hipStreamSynchronize(s->stream));
-        if (s->fused_prev_count) {   // keep the pending block's partials
-            std::vector<double> keep(s->fused_prev_count);
-            QIP_HIP(hipMemcpy(keep.data(), fused_partials_of(s, s->fused_parity ^ 1), keep.size() * sizeof(double),
-                              hipMemcpyDeviceToHost));
-            QIP_HIP(hipFree(s->d_fused));
-            s->d_fused = nullptr;
-            QIP_HIP(hipMalloc(&s->d_fused, sizeof(double) * (4 + 2 * (size_t)nblocks)));
-            s->fused_partials = nblocks;
-            QIP_HIP(hipMemcpy(fused_partials_of(s, s->fused_parity ^ 1), keep.data(), keep.size() * sizeof(double),
-                              hipMemcpyHostToDevice));
-            return QIPGPU_OK;
-        }
-        QIP_HIP(hipFree(s->d_fused));
-        s->d_fused = nullptr;
+        if (s->fused_partials >= nblocks) return QIPGPU_OK;
+        return set_error(QIPGPU_EHIP, "fused grid of %u workgroups exceeds the %zu partials sized for this state",
+                         nblocks, s->fused_partials);
     }
     QIP_HIP(hipMalloc(&s->d_fused, sizeof(double) * (4 + 2 * (size_t)nblocks)));
     s->fused_partials = nblocks;
