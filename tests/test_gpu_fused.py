@@ -375,3 +375,36 @@ def test_pattern_sweep_leaves_state_unchanged():
     with pytest.raises(ValueError):
         fz.fused_pattern_sweep([0, 0])
     fz.close()
+
+
+@pytest.mark.parametrize("width", [2, 4])
+def test_recorder_narrower_width(width):
+    """A FusedBackend with a narrower fusion width: the library fuses to that width (the blocks of the
+    Python rule at that width, bitwise), and gates wider than it go to the plain path in order."""
+    from qip_amd.fused import FusedBackend, descriptor_gates, fuse
+    from qip_amd.workloads import layered_random_circuit
+
+    class Narrow(FusedBackend):
+        fusion_width = width
+
+    n = 14
+    ops = layered_random_circuit(n, 6, seed=width)
+    fz = Narrow.make_state(n, [], [])
+    fz.streaming = False
+    for d in ops:
+        fz.kronselect_dot(d)
+    got = fz.get_state()
+    blocks = fuse([g for d in ops for g in descriptor_gates(d)], width)
+    assert fz.blocks_applied == len(blocks)
+    ref = GpuBackend.make_state(n, [], [], statetype=numpy.complex64)
+    _apply_python_plan(ref, n, blocks)
+    assert numpy.array_equal(got, ref.get_state())
+    if width == 2:                        # a 3-qubit gate is wider than the width: applied unfused
+        U = rand_u(8)
+        pl = GpuBackend.make_state(n, [], [])
+        for d in ops:
+            pl.kronselect_dot(d)
+        for be in (pl, fz):
+            be.kronselect_dot({(1, 5, 9): U})
+            be.kronselect_dot({2: numpy.array([[0, 1], [1, 0]])})
+        assert infidelity(pl, fz) < 1e-8
