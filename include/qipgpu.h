@@ -185,6 +185,12 @@ int qipgpu_fused_recorded(const qipgpu_state *s, int64_t *pending_gates, int64_t
  * a trailing run of single-qubit-only blocks). */
 int qipgpu_fuse_online(int32_t n, int32_t width, int32_t n_gates, const int32_t *gate_k, const int32_t *gate_qubits,
                        int64_t *gate_block, int64_t *final_after);
+/* Host only (no GPU call): the recorder's launches for a gate list on n qubits recorded with the given
+ * width and streaming setting, then flushed. launch_of_block[b] (room for n_gates entries) = index
+ * of the launch that applies block b; *n_blocks = number of blocks. Blocks that share a launch are a
+ * run of single-qubit-only blocks on the product sweep (n >= 12). */
+int qipgpu_fuse_launches(int32_t n, int32_t width, int32_t streaming, int32_t n_gates, const int32_t *gate_k,
+                         const int32_t *gate_qubits, int64_t *launch_of_block, int64_t *n_blocks);
 /* Launch recorded gates, then apply the pending renormalisation factor (if any) with one scaling
  * sweep. */
 int qipgpu_fused_finish(qipgpu_state *s);

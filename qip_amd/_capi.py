@@ -57,6 +57,7 @@ SIGNATURES = {
     "qipgpu_fused_flush": (ctypes.c_int, [_P]),
     "qipgpu_fused_recorded": (ctypes.c_int, [_P, _PI64, _PI64]),
     "qipgpu_fuse_online": (ctypes.c_int, [_I32, _I32, _I32, _P, _P, _P, _P]),
+    "qipgpu_fuse_launches": (ctypes.c_int, [_I32, _I32, _I32, _I32, _P, _P, _P, _PI64]),
     "qipgpu_fused_finish": (ctypes.c_int, [_P]),
     "qipgpu_fused_last_norm": (ctypes.c_int, [_P, _PD]),
     "qipgpu_overlap": (ctypes.c_int, [_P, _P, _PD, _PD, _PD, _PD]),

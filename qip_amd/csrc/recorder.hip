@@ -83,6 +83,30 @@ struct RecBlock {
     bool product = true;           // single-qubit gates only
 };
 
+// The launch decisions, shared by the recorder and its host-only restatement (qipgpu_fuse_launches);
+// `blocks[i].product` is all they read.
+//
+// End of the launch that starts at open block i of the first `count`: a run of consecutive
+// single-qubit-only blocks is one launch (where product runs apply), any other block one launch.
+template <class Blocks>
+size_t launch_end(const Blocks &blocks, size_t i, size_t count, bool runs) {
+    if (!runs || !blocks[i].product) return i + 1;
+    size_t j = i;
+    while (j < count && blocks[j].product) ++j;
+    return j;
+}
+
+// Final blocks that can launch now: all of them, except a trailing run of single-qubit-only blocks,
+// which waits until the block after it is final too (or the next flush), so that a run is always
+// launched whole and the launches do not depend on when blocks became final.
+template <class Blocks>
+int64_t launchable_count(const Blocks &blocks, int64_t final_count, bool runs) {
+    int64_t c = final_count;
+    if (!runs || c == 0 || c == (int64_t)blocks.size()) return c;
+    while (c > 0 && blocks[(size_t)c - 1].product && blocks[(size_t)c].product) --c;
+    return c;
+}
+
 }  // namespace
 
 struct Recorder {
@@ -194,17 +218,11 @@ int launch_leading(qipgpu_state *s, int64_t count) {
     std::vector<Launch> ls;
     for (size_t i = 0; i < (size_t)count;) {
         ls.emplace_back();
-        int rc;
-        if (product_runs(s) && r->blocks[i].product) {
-            size_t j = i;
-            while (j < (size_t)count && r->blocks[j].product) ++j;
-            rc = compose_product(r->blocks, i, j, ls.back());
-            i = j;
-        } else {
-            rc = compose_dense(r->blocks[i], r->fuser.masks[i], ls.back());
-            ++i;
-        }
+        const size_t j = launch_end(r->blocks, i, (size_t)count, product_runs(s));
+        const int rc = r->blocks[i].product && product_runs(s) ? compose_product(r->blocks, i, j, ls.back())
+                                                                : compose_dense(r->blocks[i], r->fuser.masks[i], ls.back());
         if (rc) return rc;
+        i = j;
     }
     for (int64_t i = 0; i < count; ++i) {
         r->pending -= (int64_t)r->blocks.front().k.size();
@@ -219,15 +237,8 @@ int launch_leading(qipgpu_state *s, int64_t count) {
     return rc;
 }
 
-// Final blocks that can launch now: all of them, except a trailing run of single-qubit-only blocks,
-// which waits until the block after it is final too (or the next flush), so that a run is always
-// launched whole and the launches do not depend on when blocks became final.
 int64_t launchable(const qipgpu_state *s, int64_t final_count) {
-    const Recorder *r = s->recorder;
-    int64_t c = final_count;
-    if (!product_runs(s) || c == 0 || c == (int64_t)r->blocks.size()) return c;
-    while (c > 0 && r->blocks[(size_t)c - 1].product && r->blocks[(size_t)c].product) --c;
-    return c;
+    return launchable_count(s->recorder->blocks, final_count, product_runs(s));
 }
 
 int ensure_recorder(qipgpu_state *s) {
@@ -343,6 +354,49 @@ int qipgpu_fuse_online(int32_t n, int32_t width, int32_t n_gates, const int32_t 
         f.take(fc);
         final_after[g] = f.base;
     }
+    return QIPGPU_OK;
+}
+
+int qipgpu_fuse_launches(int32_t n, int32_t width, int32_t streaming, int32_t n_gates, const int32_t *gate_k,
+                         const int32_t *gate_qubits, int64_t *launch_of_block, int64_t *n_blocks) {
+    if (n < 1 || n > 64) return set_error(QIPGPU_EINVAL, "%d qubits (1..64)", n);
+    if (width < 1 || width > 6) return set_error(QIPGPU_EINVAL, "fusion width %d outside 1..6", width);
+    if (n_gates < 0) return set_error(QIPGPU_EINVAL, "negative gate count");
+    struct Flag {
+        bool product = true;
+    };
+    const bool runs = n >= kFusedMinQubits;
+    OnlineFuser f;
+    f.reset(n, width);
+    std::deque<Flag> open;
+    int64_t done = 0, launches = 0;
+    auto launch_first = [&](int64_t count) {
+        for (size_t i = 0; i < (size_t)count;) {
+            const size_t j = launch_end(open, i, (size_t)count, runs);
+            for (size_t b = i; b < j; ++b) launch_of_block[done + (int64_t)b] = launches;
+            ++launches;
+            i = j;
+        }
+        for (int64_t i = 0; i < count; ++i) open.pop_front();
+        f.take(count);
+        done += count;
+    };
+    for (int32_t g = 0; g < n_gates; ++g) {
+        const int32_t k = gate_k[g];
+        const int32_t *q = gate_qubits + 6 * (size_t)g;
+        if (k < 1 || k > width) return set_error(QIPGPU_EINVAL, "gate %d on %d qubits (1..%d)", g, k, width);
+        for (int t = 0; t < k; ++t) {
+            if (q[t] < 0 || q[t] >= n) return set_error(QIPGPU_EINVAL, "gate %d: qubit %d outside 0..%d", g, q[t], n - 1);
+            for (int u = 0; u < t; ++u)
+                if (q[u] == q[t]) return set_error(QIPGPU_EINVAL, "gate %d: qubit %d repeated", g, q[t]);
+        }
+        const size_t pos = f.add(q, k);
+        if (pos == open.size()) open.emplace_back();
+        if (k > 1) open[pos].product = false;
+        if (streaming) launch_first(launchable_count(open, f.final_count(), runs));
+    }
+    launch_first((int64_t)open.size());   // the flush at the next read
+    *n_blocks = done;
     return QIPGPU_OK;
 }
 

@@ -232,3 +232,63 @@ def test_native_recorder_plan_rejects():
         _native_plan(4, 6, [((1, 1), numpy.eye(4))])    # repeated qubit
     blk, fin = _native_plan(3, 6, [])
     assert len(blk) == 0
+
+
+def _native_launches(n, width, streaming, gates):
+    from qip_amd import _capi
+    import ctypes
+    k = numpy.array([len(gq) for gq, _ in gates] or [0], dtype=numpy.int32)
+    q = numpy.zeros((max(len(gates), 1), 6), dtype=numpy.int32)
+    for i, (gq, _) in enumerate(gates):
+        q[i, :len(gq)] = gq
+    lob = numpy.full(max(len(gates), 1), -1, dtype=numpy.int64)
+    nb = ctypes.c_int64(0)
+    _capi.check(_capi.load().qipgpu_fuse_launches(n, width, int(streaming), len(gates), _capi.ptr(k), _capi.ptr(q),
+                                                  _capi.ptr(lob), ctypes.byref(nb)), "fuse_launches")
+    return lob[:nb.value]
+
+
+def _mixed_gates(n, layers, seed):
+    """Single-qubit layers on random subsets, with a few two-qubit gates between some of them."""
+    rng = numpy.random.default_rng(seed)
+    gates = []
+    for layer in range(layers):
+        for q in rng.permutation(n)[:int(rng.integers(1, n + 1))]:
+            gates.append(((int(q),), numpy.eye(2)))
+        if rng.random() < 0.5:
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            gates.append(((a, b), numpy.eye(4)))
+    return gates
+
+
+@pytest.mark.parametrize("n,seed", [(14, 1), (16, 2), (20, 3), (13, 4), (10, 5)])
+def test_native_launches_runs_and_streaming(n, seed):
+    """The recorder's launches (qipgpu_fuse_launches, the decisions qipgpu_fused_record makes): the
+    blocks of the Python rule, one launch per block except maximal runs of consecutive single-qubit-
+    only blocks (n >= 12), and the same launches streamed or deferred."""
+    from qip_amd.fused import fuse
+    gates = _mixed_gates(n, 12, seed)
+    blocks = fuse(gates, 6)
+    deferred = _native_launches(n, 6, False, gates)
+    streamed = _native_launches(n, 6, True, gates)
+    assert len(deferred) == len(blocks)
+    assert numpy.array_equal(deferred, streamed)
+    product = [all(len(gq) == 1 for gq, _ in b.gates) for b in blocks]
+    expect, cur = [], -1
+    for i, p in enumerate(product):
+        if not (n >= 12 and p and i > 0 and product[i - 1]):
+            cur += 1
+        expect.append(cur)
+    assert deferred.tolist() == expect
+
+
+def test_native_launches_known_runs():
+    """H on 17 of 18 qubits, a 2-qubit gate inside the first block, then more single-qubit layers:
+    blocks [mixed, product, product] and the two product blocks share one launch; below 12 qubits
+    every block is its own launch."""
+    H = numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2)
+    gates = [((q,), H) for q in range(17)] + [((3, 4), numpy.eye(4))] + [((q,), H) for q in range(0, 17, 2)]
+    assert _native_launches(18, 6, True, gates).tolist() == [0, 1, 1]
+    assert _native_launches(18, 6, False, gates).tolist() == [0, 1, 1]
+    small = [((q,), H) for q in range(10)] + [((3, 4), numpy.eye(4))]
+    assert _native_launches(10, 6, True, small).tolist() == [0, 1]
