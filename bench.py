@@ -277,9 +277,12 @@ def bench_grover24(args):
         err = float(numpy.max(numpy.abs(be.get_state() - expect)))
         random.seed(args.seed)
         m, p = be.measure(list(range(ns)))
+        alg, touched = grover_iteration_bytes(ns, 16 if name == "plain_c128" else 8, diff)
         res[name] = {"iterations_per_s": t_iter / dt, "s_total": dt, "diffusion": diff,
                      "max_abs_err_vs_closed_form": err, "measured": m, "measured_prob": p,
-                     "found_marked": m == x_star}
+                     "found_marked": m == x_star,
+                     "algorithmic_gbs": alg * t_iter / dt / 1e9, "hbm_frac": alg * t_iter / dt / 1e9 / HBM_PEAK_GBS,
+                     "touched_gbs": touched * t_iter / dt / 1e9}
         be.close()
     cfg = {"workload": "24-qubit iterative Grover: 23 search qubits + |-> ancilla, F(x==x*) oracle, {} iterations, "
                        "seeded measurement".format(t_iter), "n_qubits": ns + 1, "iterations": t_iter,
@@ -289,6 +292,25 @@ def bench_grover24(args):
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_grover(ns, x_star, args.cpu_seconds)
     return out
+
+
+def grover_iteration_bytes(ns: int, amp: int, diffusion: str):
+    """(algorithmic, touched) HBM bytes of one Grover iteration on ns search qubits + 1 ancilla.
+    Algorithmic, per descriptor as the reference sweeps it: read + write of every amplitude a
+    descriptor can change (2 S 2^n; the 22-controlled Z changes 2^2), plus the 4 B x 2^ns f table per
+    func_apply. Touched, what the kernels here move: each H or X layer on 23 qubits is three product
+    passes over the whole state (every pass keeps the four lowest state bits in its tile), func_apply
+    reads the table and swaps only the pairs where f(x) = 1 (one, for these oracles)."""
+    n = ns + 1
+    sweep = 2 * amp * 2 ** n
+    table = 4 * 2 ** ns
+    layers = 4 if diffusion == "mcz" else 2           # H, X, X, H  or  H, H
+    funcs = 1 if diffusion == "mcz" else 2            # the oracle (+ F(x == 0) for F diffusion)
+    mcz = 2 * amp * 2 ** (n - (ns - 1)) if diffusion == "mcz" else 0
+    algorithmic = funcs * (sweep + table) + layers * sweep + mcz
+    passes = -(-(ns - 3) // 8)                        # three low qubits ride in the first pass
+    touched = funcs * (table + 2 * 2 * amp) + layers * passes * sweep + mcz
+    return algorithmic, touched
 
 
 def cpu_baseline_grover(ns, x_star, seconds):
