@@ -191,117 +191,9 @@ int ensure_fused_state(qipgpu_state *s, uint32_t nblocks) {
     return QIPGPU_OK;
 }
 
-uint32_t fused_grid(uint64_t ntiles) {
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
-        hipDeviceProp_t p;
-        if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0) cus = p.multiProcessorCount;
-    }
-    (void)hipGetLastError();
-    return (uint32_t)std::min<uint64_t>(ntiles, (uint64_t)cus * kFusedBlocksPerCU);
-}
-
-}  // namespace
-
-namespace qipgpu {
-
-// Launch recorded gates and apply a pending renormalisation factor before the state is read or written
-// from the host. Gate,
-// diagonal, product and func_apply sweeps are linear, so they commute with the scalar and leave it
-// pending for the next fused block (which applies it as it loads) or the next read.
-int settle(qipgpu_state *s) {
-    return qipgpu_fused_finish(s);
-}
-
-}  // namespace qipgpu
-
-extern "C" {
-
-int qipgpu_fused_apply(qipgpu_state *s, int32_t k, const int32_t *targets, const double *mat) {
-    if (s->dtype != QIPGPU_C64) return set_error(QIPGPU_EINVAL, "fused blocks apply to complex64 states");
-    if (s->n < kTileBits) return set_error(QIPGPU_EUNSUP, "fused blocks need n >= %d (n=%d)", kTileBits, s->n);
-    if (k < 1 || k > 6) return set_error(QIPGPU_EINVAL, "fused block on %d qubits (1..6)", k);
-    int rc = check_qubits(s, k, targets, "fused block qubits");
-    if (rc) return rc;
-    rc = flush_recorded(s);   // recorded gates go first (a no-op when the recorder is launching)
-    if (rc) return rc;
-    const int n = s->n;
-    // tile bits: the targets padded to six, and six column bits (choose_tile_bits)
-    std::vector<int> tbits;  // state bits in the caller's matrix order (MSB first), then padding
-    uint64_t tmask = 0;
-    for (int j = 0; j < k; ++j) {
-        tbits.push_back(n - 1 - targets[j]);
-        tmask |= 1ull << (n - 1 - targets[j]);
-    }
-    int ts[6], ls[6];
-    choose_tile_bits(n, tmask, ts, ls);
-    for (int b = 0; b < 6; ++b)
-        if (!(tmask >> ts[b] & 1)) tbits.push_back(ts[b]);
-    // row index r: bit b on the b-th lowest T bit. Map r -> index of the padded caller matrix
-    // (caller bits MSB-first over tbits[0..k-1], padding bits are identity).
-    std::vector<int> tsorted(ts, ts + 6);
-    int rowbit_of_tbit[64];
-    for (int b = 0; b < 6; ++b) rowbit_of_tbit[tsorted[b]] = b;
-    auto caller_index = [&](int r, int *pad) {
-        int ci = 0, pv = 0;
-        for (int j = 0; j < k; ++j) ci = (ci << 1) | ((r >> rowbit_of_tbit[tbits[j]]) & 1);
-        for (int j = k; j < 6; ++j) pv = (pv << 1) | ((r >> rowbit_of_tbit[tbits[j]]) & 1);
-        *pad = pv;
-        return ci;
-    };
-    const int D = 1 << k;
-    std::vector<double> ur(64 * 64), ui(64 * 64);
-    for (int r = 0; r < 64; ++r) {
-        int pr;
-        const int cr = caller_index(r, &pr);
-        for (int c = 0; c < 64; ++c) {
-            int pc;
-            const int cc = caller_index(c, &pc);
-            const bool same = pr == pc;
-            ur[r * 64 + c] = same ? mat[2 * (cr * D + cc)] : 0.0;
-            ui[r * 64 + c] = same ? mat[2 * (cr * D + cc) + 1] : 0.0;
-        }
-    }
-    std::vector<uint16_t> frag(kFragBytes / 2);
-    split_fragments(ur.data(), ui.data(), frag.data());
-    void *dfrag = nullptr;
-    rc = ring_upload(s, frag.data(), kFragBytes, &dfrag);
-    if (rc) return rc;
-
-    const FusedArgs a = make_tile_args(n, ts, ls);
-
-    const uint32_t grid = fused_grid(a.ntiles);
-    rc = ensure_fused_state(s, grid);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_fused6, dim3(grid), dim3(256), 0, s->stream, (cx<float> *)s->amps, (const bf16x8 *)dfrag, a,
-                       (const double *)fused_partials_of(s, s->fused_parity ^ 1), s->fused_prev_count,
-                       fused_partials_of(s, s->fused_parity));
-    QIP_LAUNCHED();
-    s->fused_prev_count = grid;
-    s->fused_parity ^= 1;
-    return QIPGPU_OK;
-}
-
-int qipgpu_fused_pattern_sweep(qipgpu_state *s, int32_t k, const int32_t *targets) {
-    if (s->dtype != QIPGPU_C64) return set_error(QIPGPU_EINVAL, "fused blocks apply to complex64 states");
-    if (s->n < kTileBits) return set_error(QIPGPU_EUNSUP, "fused blocks need n >= %d (n=%d)", kTileBits, s->n);
-    if (k < 1 || k > 6) return set_error(QIPGPU_EINVAL, "fused block on %d qubits (1..6)", k);
-    int rc = check_qubits(s, k, targets, "fused block qubits");
-    if (rc) return rc;
-    rc = flush_recorded(s);
-    if (rc) return rc;
-    uint64_t tmask = 0;
-    for (int j = 0; j < k; ++j) tmask |= 1ull << (s->n - 1 - targets[j]);
-    int ts[6], ls[6];
-    choose_tile_bits(s->n, tmask, ts, ls);
-    const FusedArgs a = make_tile_args(s->n, ts, ls);
-    hipLaunchKernelGGL(k_tile_sweep, dim3(fused_grid(a.ntiles)), dim3(256), 0, s->stream, (cx<float> *)s->amps, a);
-    QIP_LAUNCHED();
-    return QIPGPU_OK;
-}
-
-int qipgpu_compose(int32_t k, const int32_t *qubits, int32_t n_gates, const int32_t *gate_k,
-                   const int32_t *gate_qubits, const double *mats, double *out) {
+__attribute__((always_inline)) inline int compose_impl(int32_t k, const int32_t *qubits, int32_t n_gates,
+                                                      const int32_t *gate_k, const int32_t *gate_qubits,
+                                                      const double *mats, double *out) {
     if (k < 1 || k > 6) return set_error(QIPGPU_EINVAL, "block of %d qubits (1..6)", k);
     if (n_gates < 0) return set_error(QIPGPU_EINVAL, "negative gate count");
     if (!qubits || !out || (n_gates > 0 && (!gate_k || !gate_qubits || !mats)))
@@ -421,8 +313,7 @@ int qipgpu_compose(int32_t k, const int32_t *qubits, int32_t n_gates, const int3
             for (int e = 0; e < dg; ++e) {
                 const double gr = gre[sr * dg + e], gi = gim[sr * dg + e];
                 if (gr == 0.0 && gi == 0.0) continue;
-                const double *__restrict__ are = &mre[(base | dep[e]) * D];
-                const double *__restrict__ aim = &mim[(base | dep[e]) * D];
+                const double *are = &mre[(base | dep[e]) * D], *aim = &mim[(base | dep[e]) * D];
                 for (int c = 0; c < D; ++c) {
                     ore[c] += gr * are[c] - gi * aim[c];
                     oim[c] += gr * aim[c] + gi * are[c];
@@ -476,6 +367,131 @@ int qipgpu_compose(int32_t k, const int32_t *qubits, int32_t n_gates, const int3
         out[2 * i + 1] = mim[i];
     }
     return QIPGPU_OK;
+}
+
+// The composition is host work on the launch path of every block: where the host CPU has AVX2 and FMA
+// it runs a copy built for them (the same loops, wider vectors), else the baseline build.
+__attribute__((target("avx2,fma"))) int compose_avx2(int32_t k, const int32_t *qubits, int32_t n_gates,
+                                                     const int32_t *gate_k, const int32_t *gate_qubits,
+                                                     const double *mats, double *out) {
+    return compose_impl(k, qubits, n_gates, gate_k, gate_qubits, mats, out);
+}
+
+int compose_base(int32_t k, const int32_t *qubits, int32_t n_gates, const int32_t *gate_k, const int32_t *gate_qubits,
+                 const double *mats, double *out) {
+    return compose_impl(k, qubits, n_gates, gate_k, gate_qubits, mats, out);
+}
+
+uint32_t fused_grid(uint64_t ntiles) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0) cus = p.multiProcessorCount;
+    }
+    (void)hipGetLastError();
+    return (uint32_t)std::min<uint64_t>(ntiles, (uint64_t)cus * kFusedBlocksPerCU);
+}
+
+}  // namespace
+
+namespace qipgpu {
+
+// Launch recorded gates and apply a pending renormalisation factor before the state is read or written
+// from the host. Gate,
+// diagonal, product and func_apply sweeps are linear, so they commute with the scalar and leave it
+// pending for the next fused block (which applies it as it loads) or the next read.
+int settle(qipgpu_state *s) {
+    return qipgpu_fused_finish(s);
+}
+
+}  // namespace qipgpu
+
+extern "C" {
+
+int qipgpu_fused_apply(qipgpu_state *s, int32_t k, const int32_t *targets, const double *mat) {
+    if (s->dtype != QIPGPU_C64) return set_error(QIPGPU_EINVAL, "fused blocks apply to complex64 states");
+    if (s->n < kTileBits) return set_error(QIPGPU_EUNSUP, "fused blocks need n >= %d (n=%d)", kTileBits, s->n);
+    if (k < 1 || k > 6) return set_error(QIPGPU_EINVAL, "fused block on %d qubits (1..6)", k);
+    int rc = check_qubits(s, k, targets, "fused block qubits");
+    if (rc) return rc;
+    rc = flush_recorded(s);   // recorded gates go first (a no-op when the recorder is launching)
+    if (rc) return rc;
+    const int n = s->n;
+    // tile bits: the targets padded to six, and six column bits (choose_tile_bits)
+    std::vector<int> tbits;  // state bits in the caller's matrix order (MSB first), then padding
+    uint64_t tmask = 0;
+    for (int j = 0; j < k; ++j) {
+        tbits.push_back(n - 1 - targets[j]);
+        tmask |= 1ull << (n - 1 - targets[j]);
+    }
+    int ts[6], ls[6];
+    choose_tile_bits(n, tmask, ts, ls);
+    for (int b = 0; b < 6; ++b)
+        if (!(tmask >> ts[b] & 1)) tbits.push_back(ts[b]);
+    // row index r: bit b on the b-th lowest T bit. Map r -> index of the padded caller matrix
+    // (caller bits MSB-first over tbits[0..k-1], padding bits are identity).
+    std::vector<int> tsorted(ts, ts + 6);
+    int rowbit_of_tbit[64];
+    for (int b = 0; b < 6; ++b) rowbit_of_tbit[tsorted[b]] = b;
+    int cidx[64], pad[64];
+    for (int r = 0; r < 64; ++r) {
+        int ci = 0, pv = 0;
+        for (int j = 0; j < k; ++j) ci = (ci << 1) | ((r >> rowbit_of_tbit[tbits[j]]) & 1);
+        for (int j = k; j < 6; ++j) pv = (pv << 1) | ((r >> rowbit_of_tbit[tbits[j]]) & 1);
+        cidx[r] = ci;
+        pad[r] = pv;
+    }
+    const int D = 1 << k;
+    std::vector<double> ur(64 * 64), ui(64 * 64);
+    for (int r = 0; r < 64; ++r)
+        for (int c = 0; c < 64; ++c) {
+            const bool same = pad[r] == pad[c];
+            ur[r * 64 + c] = same ? mat[2 * (cidx[r] * D + cidx[c])] : 0.0;
+            ui[r * 64 + c] = same ? mat[2 * (cidx[r] * D + cidx[c]) + 1] : 0.0;
+        }
+    std::vector<uint16_t> frag(kFragBytes / 2);
+    split_fragments(ur.data(), ui.data(), frag.data());
+    void *dfrag = nullptr;
+    rc = ring_upload(s, frag.data(), kFragBytes, &dfrag);
+    if (rc) return rc;
+
+    const FusedArgs a = make_tile_args(n, ts, ls);
+
+    const uint32_t grid = fused_grid(a.ntiles);
+    rc = ensure_fused_state(s, grid);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fused6, dim3(grid), dim3(256), 0, s->stream, (cx<float> *)s->amps, (const bf16x8 *)dfrag, a,
+                       (const double *)fused_partials_of(s, s->fused_parity ^ 1), s->fused_prev_count,
+                       fused_partials_of(s, s->fused_parity));
+    QIP_LAUNCHED();
+    s->fused_prev_count = grid;
+    s->fused_parity ^= 1;
+    return QIPGPU_OK;
+}
+
+int qipgpu_fused_pattern_sweep(qipgpu_state *s, int32_t k, const int32_t *targets) {
+    if (s->dtype != QIPGPU_C64) return set_error(QIPGPU_EINVAL, "fused blocks apply to complex64 states");
+    if (s->n < kTileBits) return set_error(QIPGPU_EUNSUP, "fused blocks need n >= %d (n=%d)", kTileBits, s->n);
+    if (k < 1 || k > 6) return set_error(QIPGPU_EINVAL, "fused block on %d qubits (1..6)", k);
+    int rc = check_qubits(s, k, targets, "fused block qubits");
+    if (rc) return rc;
+    rc = flush_recorded(s);
+    if (rc) return rc;
+    uint64_t tmask = 0;
+    for (int j = 0; j < k; ++j) tmask |= 1ull << (s->n - 1 - targets[j]);
+    int ts[6], ls[6];
+    choose_tile_bits(s->n, tmask, ts, ls);
+    const FusedArgs a = make_tile_args(s->n, ts, ls);
+    hipLaunchKernelGGL(k_tile_sweep, dim3(fused_grid(a.ntiles)), dim3(256), 0, s->stream, (cx<float> *)s->amps, a);
+    QIP_LAUNCHED();
+    return QIPGPU_OK;
+}
+
+int qipgpu_compose(int32_t k, const int32_t *qubits, int32_t n_gates, const int32_t *gate_k,
+                   const int32_t *gate_qubits, const double *mats, double *out) {
+    static const bool avx2 = __builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma");
+    return avx2 ? compose_avx2(k, qubits, n_gates, gate_k, gate_qubits, mats, out)
+                : compose_base(k, qubits, n_gates, gate_k, gate_qubits, mats, out);
 }
 
 int qipgpu_fused_finish(qipgpu_state *s) {

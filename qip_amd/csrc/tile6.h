@@ -304,21 +304,39 @@ __host__ __device__ inline FragEntry frag_entry(int w, int ks, int lane, int j) 
 }
 
 // Split the 64x64 operator (row-bit order, Ur/Ui row-major) into hi/lo BF16 A fragments ordered
-// [wave][ks][hi/lo][lane][j] (kFragBytes in all).
+// [wave][ks][hi/lo][lane][j] (kFragBytes in all). Each of the 8192 distinct entries is split once;
+// the negated copies of Ui flip the sign bit of both halves (round-to-nearest-even is symmetric),
+// except a zero low half, which stays +0 as x - x does.
 inline void split_fragments(const double *ur, const double *ui, uint16_t *frag) {
+    std::vector<uint16_t> hl(4 * 4096);   // [Ur hi | Ur lo | Ui hi | Ui lo]
+    for (int p = 0; p < 2; ++p) {
+        const double *src = p ? ui : ur;
+        uint16_t *hi = hl.data() + p * 8192, *lo = hi + 4096;
+        for (int i = 0; i < 4096; ++i) {
+            hi[i] = bf16_bits_rne((float)src[i]);
+            lo[i] = bf16_bits_rne((float)(src[i] - (double)bf16_value(hi[i])));
+        }
+    }
+    // frag_entry for the 8 entries j of one (wave, k-step, lane): one row, 8 consecutive columns
     for (int w = 0; w < 4; ++w)
         for (int ks = 0; ks < 8; ++ks)
-            for (int lane = 0; lane < 64; ++lane)
-                for (int j = 0; j < 8; ++j) {
-                    const FragEntry e = frag_entry(w, ks, lane, j);
-                    double val = e.imag ? ui[e.rr * 64 + e.cc] : ur[e.rr * 64 + e.cc];
-                    if (e.negate) val = -val;
-                    const uint16_t hi = bf16_bits_rne((float)val);
-                    const uint16_t lo = bf16_bits_rne((float)(val - (double)bf16_value(hi)));
-                    const size_t base = ((size_t)((w * 8 + ks) * 2) * 64 + lane) * 8 + j;
-                    frag[base] = hi;
-                    frag[base + 64 * 8] = lo;
+            for (int lane = 0; lane < 64; ++lane) {
+                const FragEntry e = frag_entry(w, ks, lane, 0);
+                const uint16_t *h = hl.data() + (e.imag ? 8192 : 0) + e.rr * 64 + e.cc;
+                const uint16_t *l = h + 4096;
+                uint16_t *fh = frag + ((size_t)((w * 8 + ks) * 2) * 64 + lane) * 8, *fl = fh + 64 * 8;
+                if (e.negate) {
+                    for (int j = 0; j < 8; ++j) {
+                        fh[j] = (uint16_t)(h[j] ^ 0x8000u);
+                        fl[j] = (l[j] & 0x7fffu) ? (uint16_t)(l[j] ^ 0x8000u) : l[j];
+                    }
+                } else {
+                    for (int j = 0; j < 8; ++j) {
+                        fh[j] = h[j];
+                        fl[j] = l[j];
+                    }
                 }
+            }
 }
 
 }  // namespace tile

@@ -203,6 +203,11 @@ class FusedBackend(GpuBackend):
         super().__init__(n, handle, dtype)
         self._streaming = True
         _capi.check(self._lib.qipgpu_fused_set_recording(self._h, self.fusion_width, 1), "set_recording")
+        # staging for the one-gate descriptors most circuits are made of (kept for the state's life)
+        self._one_k = numpy.zeros(1, dtype=numpy.int32)
+        self._one_q = numpy.zeros(6, dtype=numpy.int32)
+        self._one_m = numpy.zeros(1 << (2 * self.fusion_width), dtype=numpy.complex128)
+        self._one_ptrs = (self._one_k.ctypes.data, self._one_q.ctypes.data, self._one_m.ctypes.data)
 
     @classmethod
     def make_state(cls, n, index_groups, feed_list, inputstartindex=None, inputendindex=None,
@@ -225,6 +230,15 @@ class FusedBackend(GpuBackend):
     # -- recording ----------------------------------------------------------------------------
     def kronselect_dot(self, mats, input_offset: int = 0, output_offset: int = 0) -> None:
         check_windows(self.n, 2 ** self.n, input_offset, output_offset)
+        if len(mats) == 1:
+            # one plain matrix on int qubits: what the planner would record, without planning
+            ((key, m),) = mats.items()
+            gq = (key,) if type(key) is int else key
+            if (type(m) is numpy.ndarray and type(gq) is tuple and 0 < len(gq) <= self.fusion_width
+                    and m.ndim == 2 and m.shape[0] == m.shape[1] == 1 << len(gq) and m.dtype.kind in "fc"
+                    and all(type(q) is int for q in gq)):
+                self._record_one(gq, m)
+                return
         rec = _GateBuffer()
         for call in planner.plan(mats):
             if isinstance(call, planner.ProductCall):
@@ -236,6 +250,14 @@ class FusedBackend(GpuBackend):
             else:
                 rec.add(*controlled_dense(call))
         rec.record(self)
+
+    def _record_one(self, gq, G) -> None:
+        L = len(gq)
+        self._one_k[0] = L
+        self._one_q[:L] = gq
+        self._one_m[:1 << (2 * L)] = G.ravel()
+        k, q, m = self._one_ptrs
+        _capi.check(self._lib.qipgpu_fused_record(self._h, 1, k, q, m), "record")
 
     def apply_matrix(self, targets, mat, controls=()) -> None:
         if len(targets) + len(controls) > self.fusion_width:

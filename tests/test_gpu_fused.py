@@ -408,3 +408,23 @@ def test_recorder_narrower_width(width):
             be.kronselect_dot({(1, 5, 9): U})
             be.kronselect_dot({2: numpy.array([[0, 1], [1, 0]])})
         assert infidelity(pl, fz) < 1e-8
+
+
+def test_one_gate_fast_path_equals_planner():
+    """A one-gate descriptor with a numpy matrix skips the planner; the same circuit written with list
+    matrices (which go through the planner) records the same gates: bitwise equal states. A bad
+    descriptor still raises as the planner does."""
+    from qip_amd.fused import FusedBackend
+    from qip_amd.workloads import layered_random_circuit
+    n = 13
+    ops = layered_random_circuit(n, 6, seed=21)
+    a = FusedBackend.make_state(n, [], [])
+    b = FusedBackend.make_state(n, [], [])
+    for d in ops:
+        a.kronselect_dot(d)
+        b.kronselect_dot({k: numpy.asarray(v).tolist() for k, v in d.items()})
+    assert numpy.array_equal(a.get_state(), b.get_state())
+    with pytest.raises(Exception):
+        a.kronselect_dot({(0, 1): numpy.eye(2)})          # shape does not match the qubit count
+    with pytest.raises(ValueError):
+        a.kronselect_dot({n: numpy.eye(2)})               # qubit out of range
