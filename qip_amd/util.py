@@ -20,7 +20,12 @@ def flatten(lst: Iterable[Any]) -> List[Any]:
     Python >= 3.10."""
     out = []
     for item in lst:
-        if isinstance(item, collections.abc.Iterable):
+        t = type(item)
+        if t is list or t is tuple or t is range:          # the common cases, without the ABC check
+            out.extend(item)
+        elif t is int:
+            out.append(item)
+        elif isinstance(item, collections.abc.Iterable):
             out.extend(item)
         else:
             out.append(item)
