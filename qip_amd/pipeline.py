@@ -6,6 +6,8 @@ of qubit indices in order of its `qid`, builds the state through the backend con
 feeds every node to the state in dependency order, lowest qid first. The default backend is the
 MI355X ``GpuBackend``; any StateType constructor can be passed as ``backend_constructor``.
 """
+import heapq
+import itertools
 from typing import AbstractSet, Any, Callable, Mapping, Sequence, Tuple, Type, Union
 
 import numpy
@@ -148,17 +150,24 @@ def run_graph(frontier: Sequence[PipelineObject], graphnodes: AbstractSet[Pipeli
     for qbit in sorted(frontier, key=lambda q: q.qid):
         qbitindex[qbit] = list(range(n, n + qbit.n))
         n += qbit.n
-    frontier = list(frontier)
-    while frontier:
-        frontier.sort(key=lambda q: q.qid)
-        node = frontier.pop(0)
+    # The reference re-sorts its frontier list by qid (a stable sort) and pops the head: lowest qid
+    # first, equal qids in the order they joined the frontier. A heap keyed (qid, arrival) takes the
+    # same order without re-sorting.
+    arrival = itertools.count()
+    heap = [(node.qid, next(arrival), node) for node in frontier]
+    heapq.heapify(heap)
+    queued = set(frontier)
+    while heap:
+        node = heapq.heappop(heap)[2]
+        queued.discard(node)
         graphacc.feed(qbitindex, node)
         qbitindex = node.remap_index(qbitindex, n)
         seen.add(node)
         for nxt in node.sink:
-            if nxt in graphnodes and nxt not in seen and nxt not in frontier:
+            if nxt in graphnodes and nxt not in seen and nxt not in queued:
                 if all(prev in seen for prev in nxt.inputs):
-                    frontier.append(nxt)
+                    heapq.heappush(heap, (nxt.qid, next(arrival), nxt))
+                    queued.add(nxt)
                     qbitindex[nxt] = nxt.select_index(flatten(qbitindex[j] for j in nxt.inputs))
 
 
