@@ -260,7 +260,6 @@ def test_recorder_edges():
     """Recorded gates and host calls: reinit drops the record; range writes, measurement and a
     change of streaming launch it first; closing with gates recorded is fine; a bad gate is
     rejected with nothing recorded; a complex128 state cannot record."""
-    import ctypes
     import random
     from qip_amd import _capi
     from qip_amd.fused import FusedBackend

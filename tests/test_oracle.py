@@ -20,7 +20,7 @@ import pytest
 
 from oracle import oracle
 from oracle.oracle_backend import OracleBackend
-from qip_amd import C, CMat, F, H, Not, QFFT, Qubit, StochasticMeasure, SwapMat, run
+from qip_amd import CMat, F, H, QFFT, Qubit, StochasticMeasure, SwapMat, run
 
 RNG = numpy.random.default_rng(20240611)
 
