@@ -50,7 +50,6 @@ __device__ __forceinline__ double partials_sum(const double (&pv)[8], const doub
 __global__ __launch_bounds__(256, 2) void k_fused6(cx<float> *amps, const bf16x8 *__restrict__ wfrag, FusedArgs a,
                                                    const double *__restrict__ prev_partial, uint32_t prev_count,
                                                    double *partial) {
-    typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f4;
     __shared__ TileLds lds;
     __shared__ double red[4];
     const int tid = threadIdx.x;
