@@ -96,7 +96,7 @@ def run_fused_step(be, ops):
 
 
 def fused_kernel_ms(be, ops, ev):
-    """Average time per block pass of the fused kernel: the circuit's blocks (the recorder's greedy
+    """Average time per block pass of the fused kernel: the circuit's blocks (the recorder's light-cone
     rule restated by qip_amd.fused.fuse, composed up front) launched back to back between HIP events
     on the state's stream; the final scaling sweep counts as one more pass of the same bytes.
     Returns (ms per pass, blocks)."""

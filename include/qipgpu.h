@@ -160,14 +160,16 @@ int qipgpu_compose(int32_t k, const int32_t *qubits, int32_t n_gates, const int3
 /* Record gates on a complex64 state for online fusion: the precision form's replacement of the
  * gate-by-gate kronselect_dot -> cdot_loop calls (kronprod.pyx:300-345, 41-200). Gate g has gate_k[g]
  * (1..width) distinct qubits gate_qubits[6 g ..] (first = MSB of its matrix) and the next
- * 4^gate_k[g] interleaved complex entries of `mats`. Gates are fused greedily, in program order, into
- * blocks of at most `width` qubits: a gate joins the first block, from the last one touching its
- * qubits on, that stays within the width, else opens a block. A block that no later gate can join is
- * composed (qipgpu_compose) and launched at once when streaming (qipgpu_fused_apply for n >= 12, the
- * plain dense sweep below). For n >= 12 a run of consecutive blocks holding single-qubit gates only
- * is one product of per-qubit 2x2s on the product sweep (qipgpu_apply_product, FP64 arithmetic); a
- * trailing run waits until the block after it is final, so runs launch whole. Every other entry
- * point that touches the state, and qipgpu_synchronize,
+ * 4^gate_k[g] interleaved complex entries of `mats`. Gates are planned into blocks of at most
+ * `width` qubits by the light-cone rule (a block grows from the earliest gate whose predecessors are
+ * all placed, absorbing every such gate within its qubits and taking the qubits of the earliest one
+ * that shares a qubit with it while the width allows). A block that no later gate can change is
+ * composed (qipgpu_compose) and launched when streaming (qipgpu_fused_apply for n >= 12, the plain
+ * dense sweep below); the record is planned again every max(16, pending / 8) gates. For n >= 12 a run
+ * of consecutive blocks holding single-qubit gates only is one product of per-qubit 2x2s on the
+ * product sweep (qipgpu_apply_product, FP64 arithmetic); a trailing run waits until a final block
+ * that is not single-qubit-only follows it, so runs launch whole. Streaming and deferred recording
+ * launch the same blocks. Every other entry point that touches the state, and qipgpu_synchronize,
  * launches what is recorded first; qipgpu_init_product drops it. All gates are checked before any is
  * recorded. */
 int qipgpu_fused_record(qipgpu_state *s, int32_t n_gates, const int32_t *gate_k, const int32_t *gate_qubits,
@@ -181,8 +183,8 @@ int qipgpu_fused_flush(qipgpu_state *s);
 int qipgpu_fused_recorded(const qipgpu_state *s, int64_t *pending_gates, int64_t *blocks_launched);
 /* Host only (no GPU call): the recorder's streaming fusion plan for a gate list on n qubits (<= 64).
  * gate_block[g] = global number of gate g's block (blocks numbered from 0 in launch order);
- * final_after[g] = number of blocks final once gate g is recorded (launched when streaming, but for
- * a trailing run of single-qubit-only blocks). */
+ * final_after[g] = number of blocks found final, by the plans the streaming recorder makes, once
+ * gate g is recorded (launched then, but for a trailing run of single-qubit-only blocks). */
 int qipgpu_fuse_online(int32_t n, int32_t width, int32_t n_gates, const int32_t *gate_k, const int32_t *gate_qubits,
                        int64_t *gate_block, int64_t *final_after);
 /* Host only (no GPU call): the recorder's launches for a gate list on n qubits recorded with the given

@@ -7,7 +7,7 @@ device arena and a run applies every state's gate list at once (include/qipgpu.h
   * states of n <= lds_max_qubits run whole inside one workgroup, the state in LDS, FP64 gate
     arithmetic;
   * larger states have their gates fused into <= fusion_width-qubit blocks (the precision form's
-    greedy rule), applied wave by wave with the split-BF16 MFMA tile kernel, each block
+    light-cone rule), applied wave by wave with the split-BF16 MFMA tile kernel, each block
     renormalised by the FP64 norm of the previous one;
   * the run is one captured hipGraph, replayed.
 An autotuner times candidate tunings (tiles per workgroup, LDS threshold, fusion width) on the

@@ -5,8 +5,8 @@
 //   * "LDS states" (n <= lds_max_qubits): one workgroup per state keeps the whole state in LDS
 //     and applies its gate list in order, FP64 arithmetic per gate, then renormalises in FP64 and
 //     writes the state back (k_batch_small), on a stream of its own beside the waves.
-//   * "Tiled states": gates fused greedily into <= fusion_width-qubit blocks (the precision
-//     form's rule). Block j of every tiled state belongs to wave j. Per wave:
+//   * "Tiled states": gates fused into <= fusion_width-qubit blocks by the precision form's
+//     light-cone rule (plan.h). Block j of every tiled state belongs to wave j. Per wave:
 //       k_batch_build -- one workgroup per block: the 64x64 operator (block qubits padded to six
 //         with free state bits, tile6.h) composed from its gates in FP64 in LDS, split into BF16
 //         hi/lo MFMA fragments. Builds run on a second stream, one wave ahead, into two
@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "plan.h"
 #include "tile6.h"
 
 using qipgpu::cx;
@@ -496,38 +497,19 @@ int validate_gates(int32_t count, const std::vector<int32_t> &ns, int64_t n_gate
     return QIPGPU_OK;
 }
 
-// Greedy fusion of one state's gates (program order): a gate joins the first block, from the
-// last block that touches its qubits onwards, whose qubit set stays within `width`; else it opens
-// a new block. Returns the block qubit masks and each gate's block.
+// Fusion of one state's gates (program order) by the light-cone rule of plan.h: the block qubit masks
+// in launch order and each gate's block.
 void fuse_state(const std::vector<int64_t> &gids, const GateList &gl, int width, std::vector<uint64_t> &block_mask,
                 std::vector<int32_t> &gate_block) {
     block_mask.clear();
-    int32_t last[64];
-    for (int q = 0; q < 64; ++q) last[q] = -1;
-    for (int64_t g : gids) {
-        uint64_t S = 0;
-        int32_t j = -1;
-        for (int t = 0; t < gl.k[g]; ++t) {
-            const int q = gl.qubits[6 * g + t];
-            S |= 1ull << q;
-            j = std::max(j, last[q]);
-        }
-        int32_t target = -1;
-        for (size_t i = (size_t)std::max(j, 0); i < block_mask.size(); ++i)
-            if (__builtin_popcountll(block_mask[i] | S) <= width) {
-                target = (int32_t)i;
-                break;
-            }
-        if (target < 0) {
-            target = (int32_t)block_mask.size();
-            block_mask.push_back(0);
-        }
-        block_mask[target] |= S;
-        for (int t = 0; t < gl.k[g]; ++t) {
-            const int q = gl.qubits[6 * g + t];
-            last[q] = std::max(last[q], target);
-        }
-        gate_block[g] = target;
+    std::vector<uint64_t> masks(gids.size(), 0);
+    for (size_t i = 0; i < gids.size(); ++i)
+        for (int t = 0; t < gl.k[gids[i]]; ++t) masks[i] |= 1ull << gl.qubits[6 * gids[i] + t];
+    std::vector<PlannedBlock> blocks;
+    plan_blocks(masks.data(), (int64_t)masks.size(), width, 64, false, blocks);
+    for (size_t b = 0; b < blocks.size(); ++b) {
+        block_mask.push_back(blocks[b].mask);
+        for (int64_t i : blocks[b].gates) gate_block[gids[i]] = (int32_t)b;
     }
 }
 

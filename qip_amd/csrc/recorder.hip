@@ -1,22 +1,21 @@
-// recorder.hip -- precision form, host side: gates recorded on a complex64 state are fused online
-// into blocks of at most `width` qubits, and each block is composed (qipgpu_compose) and launched
-// (qipgpu_fused_apply, or the plain dense sweep below 12 qubits) as soon as no later gate can join it.
+// recorder.hip -- precision form, host side: gates recorded on a complex64 state are planned into
+// blocks of at most `width` qubits by the light-cone rule (plan.h), and each block is composed
+// (qipgpu_compose) and launched (qipgpu_fused_apply, or the plain dense sweep below 12 qubits) as soon
+// as it is final, i.e. no gate recorded later can change it.
 //
-// Greedy rule, one gate at a time in program order: for a gate on qubit set S let j be the last block
-// that touches S. The gate joins the first block from j on (those after j do not touch S, so the gate
-// commutes past them) whose qubit set stays within the width, else it opens a block at the end.
-// Since a later gate scans from the last block touching one of its qubits, once every qubit of the
-// state has been touched by a block after block i, no later gate can reach block i: it is final.
-// Launching final blocks while the caller records the rest gives the same blocks in the same order,
-// and the same results bit for bit, as fusing the whole record at the next read.
+// Streaming. The recorded gates not launched yet are planned again every max(16, pending / 8)
+// recorded gates (so planning costs a bounded number of steps per gate), and the final leading
+// blocks are launched. Final blocks are exactly the blocks of planning the whole record at once, so
+// streaming and deferred recording launch the same blocks in the same order, with the same results
+// bit for bit, whenever the plans happen.
 //
 // Every other entry point that touches the state first launches what is recorded (flush_recorded),
 // so recorded gates always act in program order with direct calls.
 #include <algorithm>
-#include <deque>
 #include <vector>
 
 #include "engine.h"
+#include "plan.h"
 
 namespace qipgpu {
 
@@ -24,98 +23,68 @@ namespace {
 
 constexpr int kFusedMinQubits = 12;   // the MFMA tile spans 12 state bits
 
-// The greedy rule over qubit masks (n <= 64), blocks numbered globally from 0; `base` is the global
-// number of the oldest open block.
-struct OnlineFuser {
-    int n = 0, width = 6;
-    int64_t base = 0;
-    std::deque<uint64_t> masks;      // open blocks
-    std::vector<int64_t> last;       // per qubit: global number of the last block touching it, or -1
-    int touched = 0;                 // qubits with last >= 0
-
-    void reset(int n_, int width_) {
-        n = n_;
-        width = width_;
-        base = 0;
-        masks.clear();
-        last.assign(n, -1);
-        touched = 0;
-    }
-
-    // Place a gate on qubits q[0..k-1]; returns the open-block position (0 = oldest) it joined.
-    size_t add(const int32_t *q, int k) {
-        uint64_t S = 0;
-        int64_t j = -1;
-        for (int t = 0; t < k; ++t) {
-            S |= 1ull << q[t];
-            j = std::max(j, last[q[t]]);
-        }
-        size_t pos = j > base ? (size_t)(j - base) : 0;
-        for (; pos < masks.size(); ++pos)
-            if (__builtin_popcountll(masks[pos] | S) <= width) break;
-        if (pos == masks.size()) masks.push_back(0);
-        masks[pos] |= S;
-        const int64_t g = base + (int64_t)pos;
-        for (int t = 0; t < k; ++t) {
-            if (last[q[t]] < 0) ++touched;
-            last[q[t]] = std::max(last[q[t]], g);
-        }
-        return pos;
-    }
-
-    // Leading open blocks that no later gate can join.
-    int64_t final_count() const {
-        if (touched < n) return 0;
-        int64_t m = last[0];
-        for (int q = 1; q < n; ++q) m = std::min(m, last[q]);
-        return std::max<int64_t>(0, m - base);
-    }
-
-    void take(int64_t count) {
-        for (int64_t i = 0; i < count; ++i) masks.pop_front();
-        base += count;
-    }
-};
-
-struct RecBlock {
-    std::vector<int32_t> k, q;     // per gate: width, and its qubits (six slots)
-    std::vector<double> mats;      // gate matrices in order, interleaved complex
-    bool product = true;           // single-qubit gates only
-};
+// After each recorded gate (streaming): plan again once this many gates have been recorded since the
+// last plan.
+bool plan_due(int64_t since_plan, int64_t pending) {
+    return since_plan >= std::max<int64_t>(16, pending / 8);
+}
 
 // The launch decisions, shared by the recorder and its host-only restatement (qipgpu_fuse_launches);
-// `blocks[i].product` is all they read.
+// product[b] says block b holds single-qubit gates only.
 //
-// End of the launch that starts at open block i of the first `count`: a run of consecutive
-// single-qubit-only blocks is one launch (where product runs apply), any other block one launch.
-template <class Blocks>
-size_t launch_end(const Blocks &blocks, size_t i, size_t count, bool runs) {
-    if (!runs || !blocks[i].product) return i + 1;
+// End of the launch that starts at block i of the first `count`: a run of consecutive single-qubit-
+// only blocks is one launch (where product runs apply), any other block one launch.
+size_t launch_end(const std::vector<char> &product, size_t i, size_t count, bool runs) {
+    if (!runs || !product[i]) return i + 1;
     size_t j = i;
-    while (j < count && blocks[j].product) ++j;
+    while (j < count && product[j]) ++j;
     return j;
 }
 
 // Final blocks that can launch now: all of them, except a trailing run of single-qubit-only blocks,
-// which waits until the block after it is final too (or the next flush), so that a run is always
-// launched whole and the launches do not depend on when blocks became final.
-template <class Blocks>
-int64_t launchable_count(const Blocks &blocks, int64_t final_count, bool runs) {
+// which waits until a final block that is not single-qubit-only follows it (or the next flush), so
+// that a run is always launched whole and the launches do not depend on when the plans happen.
+int64_t launchable_count(const std::vector<char> &product, int64_t final_count, bool runs) {
     int64_t c = final_count;
-    if (!runs || c == 0 || c == (int64_t)blocks.size()) return c;
-    while (c > 0 && blocks[(size_t)c - 1].product && blocks[(size_t)c].product) --c;
+    if (runs)
+        while (c > 0 && product[(size_t)c - 1]) --c;
     return c;
+}
+
+std::vector<char> product_flags(const std::vector<PlannedBlock> &blocks, const int32_t *gate_k) {
+    std::vector<char> p(blocks.size(), 1);
+    for (size_t b = 0; b < blocks.size(); ++b)
+        for (int64_t g : blocks[b].gates)
+            if (gate_k[g] > 1) p[b] = 0;
+    return p;
 }
 
 }  // namespace
 
 struct Recorder {
-    OnlineFuser fuser;
-    std::deque<RecBlock> blocks;   // parallel to fuser.masks
+    int n = 0, width = 6;
+    // recorded gates not launched yet, in program order
+    std::vector<int32_t> k, q;        // width and qubits (six slots) per gate
+    std::vector<uint64_t> masks;      // qubit mask per gate
+    std::vector<size_t> moff;         // offset of each gate's matrix in mats
+    std::vector<double> mats;         // interleaved complex
     bool streaming = true;
-    bool in_flush = false;         // launching: the entry points called from here must not flush again
-    int64_t pending = 0;           // recorded gates not launched
-    int64_t launched = 0;          // blocks launched since the recorder was made
+    bool in_flush = false;            // launching: the entry points called from here must not flush again
+    int64_t since_plan = 0;           // gates recorded since the last plan
+    int64_t launched = 0;             // blocks launched since the recorder was made
+
+    int64_t pending() const { return (int64_t)k.size(); }
+
+    void reset(int n_, int width_) {
+        n = n_;
+        width = width_;
+        k.clear();
+        q.clear();
+        masks.clear();
+        moff.clear();
+        mats.clear();
+        since_plan = 0;
+    }
 };
 
 namespace {
@@ -141,29 +110,38 @@ struct Launch {
     std::vector<double> mat;       // dense: 2^k x 2^k; product: one 2x2 per qubit
 };
 
-int compose_dense(const RecBlock &b, uint64_t mask, Launch &c) {
+size_t mat_len(int32_t k) { return (size_t)2 << (2 * k); }
+
+int compose_dense(const Recorder &r, const PlannedBlock &b, Launch &c) {
     c.product = false;
     c.qubits.clear();
     for (int q = 0; q < 64; ++q)
-        if (mask >> q & 1) c.qubits.push_back(q);
+        if (b.mask >> q & 1) c.qubits.push_back(q);
     const int k = (int)c.qubits.size();
-    c.mat.resize((size_t)2 << (2 * k));
-    return qipgpu_compose(k, c.qubits.data(), (int32_t)b.k.size(), b.k.data(), b.q.data(), b.mats.data(),
-                          c.mat.data());
+    std::vector<int32_t> gk, gq;
+    std::vector<double> gm;
+    for (int64_t g : b.gates) {
+        gk.push_back(r.k[g]);
+        gq.insert(gq.end(), r.q.begin() + 6 * g, r.q.begin() + 6 * g + 6);
+        gm.insert(gm.end(), r.mats.begin() + r.moff[g], r.mats.begin() + r.moff[g] + mat_len(r.k[g]));
+    }
+    c.mat.resize(mat_len(k));
+    return qipgpu_compose(k, c.qubits.data(), (int32_t)gk.size(), gk.data(), gq.data(), gm.data(), c.mat.data());
 }
 
-// The per-qubit products of blocks [i0, i1), all single-qubit-only, each qubit's gates in program
-// order (blocks are in program order on every qubit), each product composed by qipgpu_compose.
-int compose_product(const std::deque<RecBlock> &blocks, size_t i0, size_t i1, Launch &c) {
+// The per-qubit products of blocks [i0, i1), all single-qubit-only. Dependencies keep each qubit's
+// gates in block order, and a block's gates are in program order, so each qubit's list is in program
+// order; each product is composed by qipgpu_compose.
+int compose_product(const Recorder &r, const std::vector<PlannedBlock> &blocks, size_t i0, size_t i1, Launch &c) {
     c.product = true;
     c.qubits.clear();
     c.mat.clear();
     std::vector<std::vector<double>> per(64);
-    for (size_t i = i0; i < i1; ++i) {
-        const RecBlock &b = blocks[i];
-        for (size_t g = 0; g < b.k.size(); ++g)
-            per[b.q[6 * g]].insert(per[b.q[6 * g]].end(), b.mats.begin() + 8 * g, b.mats.begin() + 8 * g + 8);
-    }
+    for (size_t i = i0; i < i1; ++i)
+        for (int64_t g : blocks[i].gates) {
+            const int32_t q = r.q[6 * g];
+            per[q].insert(per[q].end(), r.mats.begin() + r.moff[g], r.mats.begin() + r.moff[g] + 8);
+        }
     for (int q = 0; q < 64; ++q) {
         if (per[q].empty()) continue;
         const int32_t ng = (int32_t)(per[q].size() / 8);
@@ -210,25 +188,46 @@ int launch(qipgpu_state *s, const Launch &c) {
 // Product runs exist only where the product sweep does (n >= 12).
 bool product_runs(const qipgpu_state *s) { return s->n >= kFusedMinQubits; }
 
-// Compose the first `count` open blocks, take them out, then launch them back to back. A run of
-// consecutive single-qubit-only blocks is one product launch.
-int launch_leading(qipgpu_state *s, int64_t count) {
+// Drop the gates of blocks [0, count) from the pending record (the rest keep program order).
+void remove_planned(Recorder &r, const std::vector<PlannedBlock> &blocks, size_t count) {
+    std::vector<char> used(r.k.size(), 0);
+    for (size_t b = 0; b < count; ++b)
+        for (int64_t g : blocks[b].gates) used[g] = 1;
+    size_t w = 0;
+    std::vector<double> mats;
+    std::vector<size_t> moff;
+    for (size_t g = 0; g < r.k.size(); ++g) {
+        if (used[g]) continue;
+        moff.push_back(mats.size());
+        mats.insert(mats.end(), r.mats.begin() + r.moff[g], r.mats.begin() + r.moff[g] + mat_len(r.k[g]));
+        r.k[w] = r.k[g];
+        std::copy(r.q.begin() + 6 * g, r.q.begin() + 6 * g + 6, r.q.begin() + 6 * w);
+        r.masks[w] = r.masks[g];
+        ++w;
+    }
+    r.k.resize(w);
+    r.q.resize(6 * w);
+    r.masks.resize(w);
+    r.moff.swap(moff);
+    r.mats.swap(mats);
+}
+
+// Compose blocks [0, count) of the plan, take their gates out of the record, then launch them back
+// to back. A run of consecutive single-qubit-only blocks is one product launch.
+int launch_planned(qipgpu_state *s, const std::vector<PlannedBlock> &blocks, int64_t count) {
     Recorder *r = s->recorder;
     if (count <= 0) return QIPGPU_OK;
+    const std::vector<char> product = product_flags(blocks, r->k.data());
     std::vector<Launch> ls;
     for (size_t i = 0; i < (size_t)count;) {
         ls.emplace_back();
-        const size_t j = launch_end(r->blocks, i, (size_t)count, product_runs(s));
-        const int rc = r->blocks[i].product && product_runs(s) ? compose_product(r->blocks, i, j, ls.back())
-                                                                : compose_dense(r->blocks[i], r->fuser.masks[i], ls.back());
+        const size_t j = launch_end(product, i, (size_t)count, product_runs(s));
+        const int rc = product[i] && product_runs(s) ? compose_product(*r, blocks, i, j, ls.back())
+                                                     : compose_dense(*r, blocks[i], ls.back());
         if (rc) return rc;
         i = j;
     }
-    for (int64_t i = 0; i < count; ++i) {
-        r->pending -= (int64_t)r->blocks.front().k.size();
-        r->blocks.pop_front();
-    }
-    r->fuser.take(count);
+    remove_planned(*r, blocks, (size_t)count);
     r->launched += count;
     r->in_flush = true;
     int rc = QIPGPU_OK;
@@ -237,33 +236,122 @@ int launch_leading(qipgpu_state *s, int64_t count) {
     return rc;
 }
 
-int64_t launchable(const qipgpu_state *s, int64_t final_count) {
-    return launchable_count(s->recorder->blocks, final_count, product_runs(s));
+// Streaming: plan what is recorded and launch the final leading blocks that can launch now.
+int launch_final(qipgpu_state *s) {
+    Recorder *r = s->recorder;
+    r->since_plan = 0;
+    std::vector<PlannedBlock> blocks;
+    const int64_t fc = plan_blocks(r->masks.data(), r->pending(), r->width, r->n, true, blocks);
+    const std::vector<char> product = product_flags(blocks, r->k.data());
+    return launch_planned(s, blocks, launchable_count(product, fc, product_runs(s)));
 }
 
 int ensure_recorder(qipgpu_state *s) {
     if (s->dtype != QIPGPU_C64) return set_error(QIPGPU_EINVAL, "gates are recorded on complex64 states");
     if (!s->recorder) {
         s->recorder = new Recorder();
-        s->recorder->fuser.reset(s->n, 6);
+        s->recorder->reset(s->n, 6);
     }
     return QIPGPU_OK;
+}
+
+int check_plan_args(int32_t n, int32_t width, int32_t n_gates) {
+    if (n < 1 || n > 64) return set_error(QIPGPU_EINVAL, "%d qubits (1..64)", n);
+    if (width < 1 || width > 6) return set_error(QIPGPU_EINVAL, "fusion width %d outside 1..6", width);
+    if (n_gates < 0) return set_error(QIPGPU_EINVAL, "negative gate count");
+    return QIPGPU_OK;
+}
+
+// Qubit masks of a host-only gate list, checked as the recorder checks recorded gates.
+int gate_masks(int32_t n, int32_t width, int32_t n_gates, const int32_t *gate_k, const int32_t *gate_qubits,
+               std::vector<uint64_t> &masks) {
+    masks.assign((size_t)n_gates, 0);
+    for (int32_t g = 0; g < n_gates; ++g) {
+        const int32_t k = gate_k[g];
+        const int32_t *q = gate_qubits + 6 * (size_t)g;
+        if (k < 1 || k > width) return set_error(QIPGPU_EINVAL, "gate %d on %d qubits (1..%d)", g, k, width);
+        for (int t = 0; t < k; ++t) {
+            if (q[t] < 0 || q[t] >= n) return set_error(QIPGPU_EINVAL, "gate %d: qubit %d outside 0..%d", g, q[t], n - 1);
+            if (masks[g] >> q[t] & 1) return set_error(QIPGPU_EINVAL, "gate %d: qubit %d repeated", g, q[t]);
+            masks[g] |= 1ull << q[t];
+        }
+    }
+    return QIPGPU_OK;
+}
+
+// The recorder's schedule over a host-only gate list: gates recorded one at a time, planned when
+// plan_due says so (when streaming), flushed at the end. For every taken block, `take` receives the
+// global ids of its gates and whether it starts a new launch.
+template <class Take>
+void simulate_recorder(int32_t n, int32_t width, bool streaming, bool use_launchable, int32_t n_gates,
+                       const int32_t *gate_k, const std::vector<uint64_t> &all_masks, Take take,
+                       int64_t *final_after) {
+    const bool runs = n >= kFusedMinQubits;
+    std::vector<uint64_t> masks;
+    std::vector<int32_t> ks;
+    std::vector<int64_t> ids;
+    std::vector<PlannedBlock> blocks;
+    int64_t since = 0, taken = 0;
+    auto take_first = [&](int64_t count) {
+        if (count <= 0) return;
+        const std::vector<char> product = product_flags(blocks, ks.data());
+        std::vector<char> used(ids.size(), 0);
+        for (size_t i = 0; i < (size_t)count;) {
+            const size_t j = launch_end(product, i, (size_t)count, runs);
+            for (size_t b = i; b < j; ++b) {
+                std::vector<int64_t> gids;
+                for (int64_t g : blocks[b].gates) {
+                    gids.push_back(ids[g]);
+                    used[g] = 1;
+                }
+                take(gids, b == i);
+            }
+            i = j;
+        }
+        size_t w = 0;
+        for (size_t g = 0; g < ids.size(); ++g) {
+            if (used[g]) continue;
+            masks[w] = masks[g];
+            ks[w] = ks[g];
+            ids[w] = ids[g];
+            ++w;
+        }
+        masks.resize(w);
+        ks.resize(w);
+        ids.resize(w);
+        taken += count;
+    };
+    for (int32_t g = 0; g < n_gates; ++g) {
+        masks.push_back(all_masks[g]);
+        ks.push_back(gate_k[g]);
+        ids.push_back(g);
+        ++since;
+        if (streaming && plan_due(since, (int64_t)masks.size())) {
+            since = 0;
+            const int64_t fc = plan_blocks(masks.data(), (int64_t)masks.size(), width, n, true, blocks);
+            take_first(use_launchable ? launchable_count(product_flags(blocks, ks.data()), fc, runs) : fc);
+        }
+        if (final_after) final_after[g] = taken;
+    }
+    plan_blocks(masks.data(), (int64_t)masks.size(), width, n, false, blocks);
+    take_first((int64_t)blocks.size());
 }
 
 }  // namespace
 
 int flush_recorded(qipgpu_state *s) {
     Recorder *r = s->recorder;
-    if (!r || r->in_flush || r->blocks.empty()) return QIPGPU_OK;
-    return launch_leading(s, (int64_t)r->blocks.size());
+    if (!r || r->in_flush || r->k.empty()) return QIPGPU_OK;
+    std::vector<PlannedBlock> blocks;
+    plan_blocks(r->masks.data(), r->pending(), r->width, r->n, false, blocks);
+    r->since_plan = 0;
+    return launch_planned(s, blocks, (int64_t)blocks.size());
 }
 
 void recorder_discard(qipgpu_state *s) {
     Recorder *r = s->recorder;
     if (!r || r->in_flush) return;
-    r->blocks.clear();
-    r->fuser.reset(s->n, r->fuser.width);
-    r->pending = 0;
+    r->reset(s->n, r->width);
 }
 
 void recorder_free(qipgpu_state *s) {
@@ -283,7 +371,7 @@ int qipgpu_fused_set_recording(qipgpu_state *s, int32_t width, int32_t streaming
     if (rc) return rc;
     rc = flush_recorded(s);
     if (rc) return rc;
-    s->recorder->fuser.reset(s->n, width);
+    s->recorder->reset(s->n, width);
     s->recorder->streaming = streaming != 0;
     return QIPGPU_OK;
 }
@@ -297,25 +385,24 @@ int qipgpu_fused_record(qipgpu_state *s, int32_t n_gates, const int32_t *gate_k,
     Recorder *r = s->recorder;
     if (r->in_flush) return set_error(QIPGPU_EINVAL, "recording while recorded blocks launch");
     for (int32_t g = 0; g < n_gates; ++g) {   // all or nothing: check every gate first
-        rc = validate_gate(s, r->fuser.width, g, gate_k[g], gate_qubits + 6 * (size_t)g);
+        rc = validate_gate(s, r->width, g, gate_k[g], gate_qubits + 6 * (size_t)g);
         if (rc) return rc;
     }
     size_t moff = 0;
     for (int32_t g = 0; g < n_gates; ++g) {
         const int k = gate_k[g];
         const int32_t *q = gate_qubits + 6 * (size_t)g;
-        const size_t pos = r->fuser.add(q, k);
-        if (pos == r->blocks.size()) r->blocks.emplace_back();
-        RecBlock &b = r->blocks[pos];
-        b.k.push_back(k);
-        b.q.insert(b.q.end(), q, q + 6);
-        const size_t len = (size_t)2 << (2 * k);
-        b.mats.insert(b.mats.end(), mats + moff, mats + moff + len);
-        if (k > 1) b.product = false;
-        moff += len;
-        ++r->pending;
-        if (r->streaming) {
-            rc = launch_leading(s, launchable(s, r->fuser.final_count()));
+        uint64_t mask = 0;
+        for (int t = 0; t < k; ++t) mask |= 1ull << q[t];
+        r->k.push_back(k);
+        r->q.insert(r->q.end(), q, q + 6);
+        r->masks.push_back(mask);
+        r->moff.push_back(r->mats.size());
+        r->mats.insert(r->mats.end(), mats + moff, mats + moff + mat_len(k));
+        moff += mat_len(k);
+        ++r->since_plan;
+        if (r->streaming && plan_due(r->since_plan, r->pending())) {
+            rc = launch_final(s);
             if (rc) return rc;
         }
     }
@@ -327,76 +414,43 @@ int qipgpu_fused_flush(qipgpu_state *s) {
 }
 
 int qipgpu_fused_recorded(const qipgpu_state *s, int64_t *pending_gates, int64_t *blocks_launched) {
-    *pending_gates = s->recorder ? s->recorder->pending : 0;
+    *pending_gates = s->recorder ? s->recorder->pending() : 0;
     *blocks_launched = s->recorder ? s->recorder->launched : 0;
     return QIPGPU_OK;
 }
 
 int qipgpu_fuse_online(int32_t n, int32_t width, int32_t n_gates, const int32_t *gate_k, const int32_t *gate_qubits,
                        int64_t *gate_block, int64_t *final_after) {
-    if (n < 1 || n > 64) return set_error(QIPGPU_EINVAL, "%d qubits (1..64)", n);
-    if (width < 1 || width > 6) return set_error(QIPGPU_EINVAL, "fusion width %d outside 1..6", width);
-    if (n_gates < 0) return set_error(QIPGPU_EINVAL, "negative gate count");
-    OnlineFuser f;
-    f.reset(n, width);
-    for (int32_t g = 0; g < n_gates; ++g) {
-        const int32_t k = gate_k[g];
-        const int32_t *q = gate_qubits + 6 * (size_t)g;
-        if (k < 1 || k > width) return set_error(QIPGPU_EINVAL, "gate %d on %d qubits (1..%d)", g, k, width);
-        for (int t = 0; t < k; ++t) {
-            if (q[t] < 0 || q[t] >= n) return set_error(QIPGPU_EINVAL, "gate %d: qubit %d outside 0..%d", g, q[t], n - 1);
-            for (int u = 0; u < t; ++u)
-                if (q[u] == q[t]) return set_error(QIPGPU_EINVAL, "gate %d: qubit %d repeated", g, q[t]);
-        }
-        gate_block[g] = f.base + (int64_t)f.add(q, k);
-        // as the streaming recorder does: take out every block that is final after this gate
-        const int64_t fc = f.final_count();
-        f.take(fc);
-        final_after[g] = f.base;
-    }
+    int rc = check_plan_args(n, width, n_gates);
+    if (rc) return rc;
+    std::vector<uint64_t> masks;
+    rc = gate_masks(n, width, n_gates, gate_k, gate_qubits, masks);
+    if (rc) return rc;
+    int64_t block = 0;
+    simulate_recorder(n, width, true, false, n_gates, gate_k, masks,
+                      [&](const std::vector<int64_t> &gids, bool) {
+                          for (int64_t g : gids) gate_block[g] = block;
+                          ++block;
+                      },
+                      final_after);
     return QIPGPU_OK;
 }
 
 int qipgpu_fuse_launches(int32_t n, int32_t width, int32_t streaming, int32_t n_gates, const int32_t *gate_k,
                          const int32_t *gate_qubits, int64_t *launch_of_block, int64_t *n_blocks) {
-    if (n < 1 || n > 64) return set_error(QIPGPU_EINVAL, "%d qubits (1..64)", n);
-    if (width < 1 || width > 6) return set_error(QIPGPU_EINVAL, "fusion width %d outside 1..6", width);
-    if (n_gates < 0) return set_error(QIPGPU_EINVAL, "negative gate count");
-    struct Flag {
-        bool product = true;
-    };
-    const bool runs = n >= kFusedMinQubits;
-    OnlineFuser f;
-    f.reset(n, width);
-    std::deque<Flag> open;
-    int64_t done = 0, launches = 0;
-    auto launch_first = [&](int64_t count) {
-        for (size_t i = 0; i < (size_t)count;) {
-            const size_t j = launch_end(open, i, (size_t)count, runs);
-            for (size_t b = i; b < j; ++b) launch_of_block[done + (int64_t)b] = launches;
-            ++launches;
-            i = j;
-        }
-        for (int64_t i = 0; i < count; ++i) open.pop_front();
-        f.take(count);
-        done += count;
-    };
-    for (int32_t g = 0; g < n_gates; ++g) {
-        const int32_t k = gate_k[g];
-        const int32_t *q = gate_qubits + 6 * (size_t)g;
-        if (k < 1 || k > width) return set_error(QIPGPU_EINVAL, "gate %d on %d qubits (1..%d)", g, k, width);
-        for (int t = 0; t < k; ++t) {
-            if (q[t] < 0 || q[t] >= n) return set_error(QIPGPU_EINVAL, "gate %d: qubit %d outside 0..%d", g, q[t], n - 1);
-            for (int u = 0; u < t; ++u)
-                if (q[u] == q[t]) return set_error(QIPGPU_EINVAL, "gate %d: qubit %d repeated", g, q[t]);
-        }
-        const size_t pos = f.add(q, k);
-        if (pos == open.size()) open.emplace_back();
-        if (k > 1) open[pos].product = false;
-        if (streaming) launch_first(launchable_count(open, f.final_count(), runs));
-    }
-    launch_first((int64_t)open.size());   // the flush at the next read
-    *n_blocks = done;
+    int rc = check_plan_args(n, width, n_gates);
+    if (rc) return rc;
+    std::vector<uint64_t> masks;
+    rc = gate_masks(n, width, n_gates, gate_k, gate_qubits, masks);
+    if (rc) return rc;
+    int64_t block = 0, launches = -1;
+    simulate_recorder(n, width, streaming != 0, true, n_gates, gate_k, masks,
+                      [&](const std::vector<int64_t> &, bool starts) {
+                          if (starts) ++launches;
+                          launch_of_block[block++] = launches;
+                      },
+                      nullptr);
+    *n_blocks = block;
     return QIPGPU_OK;
 }
 

@@ -99,73 +99,108 @@ def descriptor_gates(mats) -> List[Tuple[Tuple[int, ...], numpy.ndarray]]:
     return out
 
 
-class OnlineFusion:
-    """The greedy rule applied one gate at a time on an n-qubit state, with the leading blocks that
-    no later gate can join handed out as they become final."""
+def plan_blocks(qsets: Sequence[Sequence[int]], width: int, n: int = 64,
+                stop_at_open: bool = False) -> Tuple[List[Tuple[frozenset, List[int]]], int]:
+    """The light-cone rule over gates given by their qubit sets, in program order. Returns the
+    blocks in launch order, each (qubit set, indices of its gates ascending), and how many leading
+    blocks are final: no gate recorded later can change them. With ``stop_at_open`` the plan stops
+    after the first block that is not final."""
+    count = len(qsets)
+    sets = [frozenset(q) for q in qsets]
+    full = min(width, n)
+    last, npred = {}, [0] * count
+    succ = [[] for _ in range(count)]
+    open_on = {}
+    for g, S in enumerate(sets):
+        preds = {last[q] for q in S if q in last}
+        npred[g] = len(preds)
+        for p in preds:
+            succ[p].append(g)
+        for q in S:
+            last[q] = g
+            open_on[q] = open_on.get(q, 0) + 1
+    ready = {g for g in range(count) if npred[g] == 0}
+    blocks, absorbed, final, prefix_final = [], 0, 0, True
+    while absorbed < count:
+        S = set(sets[min(ready)])
+        taken = []
+        while True:
+            work = [g for g in ready if sets[g] <= S]
+            while work:
+                g = work.pop()
+                ready.discard(g)
+                taken.append(g)
+                absorbed += 1
+                for q in sets[g]:
+                    open_on[q] -= 1
+                for t in succ[g]:
+                    npred[t] -= 1
+                    if npred[t] == 0:
+                        ready.add(t)
+                        if sets[t] <= S:
+                            work.append(t)
+            order = sorted(ready)
+            pick = next((g for g in order if sets[g] & S and len(S | sets[g]) <= width), None)
+            if pick is None:
+                pick = next((g for g in order if len(S | sets[g]) <= width), None)
+            if pick is None:
+                break
+            S |= sets[pick]
+        blocks.append((frozenset(S), sorted(taken)))
+        # A later gate changes nothing up to here unless it becomes ready during the plan, which
+        # needs every qubit of it without an open recorded gate: so the block's own qubits must
+        # all have one, and, unless the block is full, so must every qubit of the state.
+        is_final = (all(open_on[q] > 0 for q in S)
+                    and (len(S) == full or all(open_on.get(q, 0) > 0 for q in range(n))))
+        prefix_final = prefix_final and is_final
+        final += prefix_final
+        if stop_at_open and not prefix_final:
+            break
+    return blocks, final
+
+
+def fuse(gates: Sequence[Tuple[Tuple[int, ...], numpy.ndarray]], width: int) -> List[Block]:
+    """The blocks of the whole record under the light-cone rule, in launch order."""
+    out = []
+    for S, taken in plan_blocks([gq for gq, _ in gates], width)[0]:
+        b = Block()
+        b.qubits = set(S)
+        b.gates = [gates[i] for i in taken]
+        out.append(b)
+    return out
+
+
+class StreamingFusion:
+    """The rule applied while gates arrive: ``take_final`` removes and returns the leading blocks
+    that are final so far; whenever it is called, the blocks taken plus those of the rest at the
+    end are the blocks of ``fuse`` on the whole record."""
 
     def __init__(self, n: int, width: int):
         self.n = n
         self.width = width
-        self.blocks: List[Block] = []   # open blocks; blocks[i] has global index base + i
-        self.base = 0
-        self.last = {}                  # qubit -> global index of the last block touching it
-        self.pending = 0                # gates in open blocks
+        self.gates = []          # recorded, not taken
 
     def add(self, gq: Tuple[int, ...], G: numpy.ndarray) -> None:
-        S = set(gq)
-        j = max(self.last.get(q, -1) for q in S)
-        start = max(max(j, 0) - self.base, 0)
-        target = None
-        for i in range(start, len(self.blocks)):
-            if len(self.blocks[i].qubits | S) <= self.width:
-                target = i
-                break
-        if target is None:
-            self.blocks.append(Block())
-            target = len(self.blocks) - 1
-        blk = self.blocks[target]
-        blk.qubits |= S
-        blk.gates.append((gq, G))
-        g = self.base + target
-        for q in S:
-            self.last[q] = max(self.last.get(q, -1), g)
-        self.pending += 1
+        self.gates.append((gq, G))
 
-    def final_count(self) -> int:
-        """Leading open blocks that no later gate can join."""
-        if len(self.last) < self.n:
-            return 0
-        return max(0, min(self.last.values()) - self.base)
-
-    def take(self, count: int = None) -> List[Block]:
-        """Remove and return the first `count` open blocks (all when None)."""
-        count = len(self.blocks) if count is None else count
-        out, self.blocks = self.blocks[:count], self.blocks[count:]
-        self.base += len(out)
-        self.pending -= sum(len(b.gates) for b in out)
+    def _take(self, plan) -> List[Block]:
+        out, used = [], set()
+        for S, taken in plan:
+            b = Block()
+            b.qubits = set(S)
+            b.gates = [self.gates[i] for i in taken]
+            used.update(taken)
+            out.append(b)
+        self.gates = [g for i, g in enumerate(self.gates) if i not in used]
         return out
 
+    def take_final(self) -> List[Block]:
+        blocks, final = plan_blocks([gq for gq, _ in self.gates], self.width, self.n, stop_at_open=True)
+        return self._take(blocks[:final])
 
-def fuse(gates: Sequence[Tuple[Tuple[int, ...], numpy.ndarray]], width: int) -> List[Block]:
-    blocks: List[Block] = []
-    for gq, G in gates:
-        S = set(gq)
-        j = -1
-        for i in range(len(blocks) - 1, -1, -1):
-            if blocks[i].qubits & S:
-                j = i
-                break
-        target = None
-        for i in range(max(j, 0), len(blocks)):
-            if len(blocks[i].qubits | S) <= width:
-                target = blocks[i]
-                break
-        if target is None:
-            target = Block()
-            blocks.append(target)
-        target.qubits |= S
-        target.gates.append((gq, G))
-    return blocks
+    def take(self) -> List[Block]:
+        """Every recorded gate, planned as the rest of the record."""
+        return self._take(plan_blocks([gq for gq, _ in self.gates], self.width, self.n)[0])
 
 
 class _GateBuffer:

@@ -1,6 +1,7 @@
-"""CPU tests of the precision form's host logic: greedy fusion keeps program semantics (the
+"""CPU tests of the precision form's host logic: the light-cone fusion keeps program semantics (the
 ordered product of block matrices equals the ordered product of the gates, both expanded to the
-full 2^n operator independently), respects the width, and gives the expected block counts."""
+full 2^n operator independently), respects the width and every dependency, takes out only final
+blocks while streaming, and gives the expected block counts."""
 import numpy
 import pytest
 
@@ -29,10 +30,26 @@ def gates_of(ops):
     return gates
 
 
-@pytest.mark.parametrize("width", [2, 3, 4, 6])
-def test_fusion_preserves_operator(width):
+def _random_gates(n, count, seed, kmax=3):
+    """Gates of 1..kmax qubits on random qubit sets, random unitaries."""
+    rng = numpy.random.default_rng(seed)
+    gates = []
+    for _ in range(count):
+        L = int(rng.integers(1, kmax + 1))
+        gq = tuple(int(q) for q in rng.choice(n, size=L, replace=False))
+        Q, _ = numpy.linalg.qr(rng.standard_normal((2 ** L, 2 ** L)) + 1j * rng.standard_normal((2 ** L, 2 ** L)))
+        gates.append((gq, Q))
+    return gates
+
+
+@pytest.mark.parametrize("width,shape", [(2, "layered"), (3, "layered"), (4, "layered"), (6, "layered"),
+                                         (3, "random"), (4, "random"), (6, "random")])
+def test_fusion_preserves_operator(width, shape):
     n, depth = 7, 4
-    gates = gates_of(layered_random_circuit(n, depth, seed=5))
+    if shape == "layered":
+        gates = gates_of(layered_random_circuit(n, depth, seed=5))
+    else:
+        gates = _random_gates(n, 40, seed=width, kmax=min(width, 3))
     want = numpy.eye(2 ** n, dtype=complex)
     for gq, G in gates:
         want = full_operator(gq, G, n) @ want
@@ -47,9 +64,60 @@ def test_fusion_preserves_operator(width):
 
 
 def test_block_counts_brickwork():
+    """On the brickwork circuits the light-cone rule covers diamonds of the circuit: 77 six-qubit
+    blocks for the 1780 gates at 30 qubits (128 when each gate joins the first block, in program
+    order, that it fits), 42 for the 830 gates at 28 qubits depth 20."""
     gates = gates_of(layered_random_circuit(30, 40, seed=1))
     assert len(gates) == 1780
-    assert len(fuse(gates, 6)) == 128
+    assert len(fuse(gates, 6)) == 77
+    assert len(fuse(gates_of(layered_random_circuit(28, 20, seed=2024)), 6)) == 42
+
+
+def _check_plan(gates, blocks, width):
+    """Every gate in exactly one block, gates in program order inside a block, blocks within the
+    width, and every dependency (two gates sharing a qubit) kept in order across blocks."""
+    where = {}
+    for b, blk in enumerate(blocks):
+        assert len(blk.qubits) <= width
+        prev = -1
+        for gq, G in blk.gates:
+            i = next(j for j, (q2, G2) in enumerate(gates) if G2 is G)
+            assert i > prev
+            prev = i
+            assert set(gq) <= blk.qubits
+            assert i not in where
+            where[i] = b
+    assert sorted(where) == list(range(len(gates)))
+    last = {}
+    for i, (gq, _) in enumerate(gates):
+        for q in gq:
+            if q in last:
+                assert where[last[q]] <= where[i]
+            last[q] = i
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_fusion_plan_is_valid(seed):
+    n = 12 + seed
+    gates = _random_gates(n, 150, seed, kmax=4)
+    for width in (4, 6):
+        _check_plan(gates, fuse(gates, width), width)
+    gates = gates_of(layered_random_circuit(n, 10, seed=seed))
+    _check_plan(gates, fuse(gates, 6), 6)
+
+
+def test_fusion_tiny_cases():
+    """Hand-checked plans: a chain of CNOTs along a line at width 3 grows along the chain; gates on
+    disjoint qubits share a block up to the width."""
+    from qip_amd.fused import plan_blocks
+    blocks, _ = plan_blocks([(0, 1), (1, 2), (2, 3), (3, 4)], 3)
+    assert [(sorted(S), t) for S, t in blocks] == [([0, 1, 2], [0, 1]), ([2, 3, 4], [2, 3])]
+    blocks, _ = plan_blocks([(0,), (1,), (2,), (3,), (0,)], 2)
+    assert [(sorted(S), t) for S, t in blocks] == [([0, 1], [0, 1, 4]), ([2, 3], [2, 3])]
+    # a gate waits for its predecessors: (0, 1) is not ready before (1, 2) is placed, and (1, 2)
+    # does not fit beside qubit 0 at width 2, so the first block keeps qubit 0 alone
+    blocks, _ = plan_blocks([(0,), (1, 2), (0, 1)], 2)
+    assert [(sorted(S), t) for S, t in blocks] == [([0], [0]), ([1, 2], [1]), ([0, 1], [2])]
 
 
 def test_circuit_shape():
@@ -66,40 +134,49 @@ def test_circuit_shape():
 
 
 @pytest.mark.parametrize("n,depth,seed", [(30, 40, 2024), (16, 12, 3), (13, 5, 8)])
-def test_online_fusion_equals_batch_fusion(n, depth, seed):
-    """Gate-by-gate fusion with final blocks taken out as soon as they are final gives the blocks of
-    fuse() on the whole record, in the same order; taken blocks really are final (no later gate
-    would have joined them)."""
-    from qip_amd.fused import OnlineFusion
+def test_streaming_fusion_equals_batch_fusion(n, depth, seed):
+    """Final blocks taken out while the gates arrive, at any moments, plus the plan of the rest at the
+    end, are the blocks of fuse() on the whole record, in the same order."""
+    import random
+    from qip_amd.fused import StreamingFusion
     gates = gates_of(layered_random_circuit(n, depth, seed=seed))
     ref = fuse(gates, 6)
-    of = OnlineFusion(n, 6)
-    got = []
-    for gq, G in gates:
-        of.add(gq, G)
-        k = of.final_count()
-        if k:
-            got.extend(of.take(k))
-    assert of.pending == sum(len(b.gates) for b in of.blocks)
-    got.extend(of.take())
-    assert of.pending == 0
-    assert len(got) == len(ref)
-    for a, b in zip(got, ref):
-        assert a.qubits == b.qubits
-        assert [g[0] for g in a.gates] == [g[0] for g in b.gates]
-    if n == 30:
-        assert len(got) == 128
+    for cadence in (0.05, 0.3):
+        rnd = random.Random(seed)
+        sf = StreamingFusion(n, 6)
+        got = []
+        for gq, G in gates:
+            sf.add(gq, G)
+            if rnd.random() < cadence:
+                got.extend(sf.take_final())
+        early = len(got)
+        got.extend(sf.take())
+        assert not sf.gates
+        assert len(got) == len(ref)
+        for a, b in zip(got, ref):
+            assert a.qubits == b.qubits
+            assert [id(g[1]) for g in a.gates] == [id(g[1]) for g in b.gates]
+        if n == 30:
+            assert len(got) == 77 and early >= 50
 
 
-def test_online_fusion_waits_for_untouched_qubits():
-    """While some qubit has not been touched, a later gate on it could join the first block, so
-    nothing is final."""
-    from qip_amd.fused import OnlineFusion
-    of = OnlineFusion(14, 6)
-    for q in range(13):
-        of.add((q,), numpy.eye(2))
-        of.add((q, q + 1) if q < 12 else (q,), numpy.eye(4) if q < 12 else numpy.eye(2))
-    assert of.final_count() == 0
+def test_streaming_fusion_waits_for_untouched_qubits():
+    """(0, 1), (1, 2, 3), (0, 2) at width 3: the first block is {0, 1} (the three-qubit gate does not
+    fit beside it) and both of its qubits keep a recorded gate outside it. On 4 qubits it is final;
+    on 5 it is not, since qubit 4 has no recorded gate and a later gate there could join it."""
+    from qip_amd.fused import plan_blocks
+    qs = [(0, 1), (1, 2, 3), (0, 2)]
+    blocks, final = plan_blocks(qs, 3, 4)
+    assert [(sorted(S), t) for S, t in blocks][0] == ([0, 1], [0])
+    assert final >= 1
+    assert plan_blocks(qs, 3, 5)[1] == 0
+    # a full-width block is final once each of its qubits keeps a recorded gate, untouched qubits or not
+    blocks, final = plan_blocks([(0, 1, 2), (2, 3), (0, 1)], 3, 8)
+    assert [(sorted(S), t) for S, t in blocks][0] == ([0, 1, 2], [0, 2])
+    assert final == 0                       # qubits 0 and 1 have nothing left outside the block...
+    blocks, final = plan_blocks([(0, 1, 2), (2, 3), (0, 3), (1, 3)], 3, 8)
+    assert [(sorted(S), t) for S, t in blocks][0] == ([0, 1, 2], [0])
+    assert final == 1                       # ...here each of 0, 1, 2 has one
 
 
 def _compose_tensordot(qs, gates):
@@ -197,28 +274,39 @@ def _native_plan(n, width, gates):
     return blk[:len(gates)], fin[:len(gates)]
 
 
+def _python_gate_blocks(gates, width):
+    where = {}
+    for b, blk in enumerate(fuse(gates, width)):
+        for _, G in blk.gates:
+            where[id(G)] = b
+    return [where[id(G)] for _, G in gates]
+
+
 @pytest.mark.parametrize("n,depth,seed,width", [(30, 40, 2024, 6), (16, 12, 3, 6), (13, 5, 8, 4), (9, 6, 1, 2)])
-def test_native_recorder_plan_equals_online_fusion(n, depth, seed, width):
-    """The library's recorder (qipgpu_fuse_online, the routine qipgpu_fused_record runs) places every
-    gate in the block the Python statement of the rule does, and has the same blocks final after
-    every gate."""
-    from qip_amd.fused import OnlineFusion
-    gates = gates_of(layered_random_circuit(n, depth, seed=seed))
+def test_native_recorder_plan_equals_python_rule(n, depth, seed, width):
+    """The library's recorder (qipgpu_fuse_online, the schedule qipgpu_fused_record runs) places every
+    gate in the block the Python statement of the rule does; the blocks it has taken out after each
+    gate are final for the gates recorded so far (the Python rule finds at least as many)."""
+    from qip_amd.fused import plan_blocks
+    gates = [(gq, G.copy()) for gq, G in gates_of(layered_random_circuit(n, depth, seed=seed))]
     blk, fin = _native_plan(n, width, gates)
-    of = OnlineFusion(n, width)
-    for i, (gq, G) in enumerate(gates):
-        G = G.view()                      # a fresh object, so the gate is found by identity
-        of.add(gq, G)
-        # the block of gate i: the open block whose last gate is this one
-        for bi, b in enumerate(of.blocks):
-            if b.gates and b.gates[-1][1] is G:
-                assert blk[i] == of.base + bi, (i, blk[i], of.base + bi)
-                break
-        else:
-            raise AssertionError("gate {} not found".format(i))
-        of.take(of.final_count())
-        assert fin[i] == of.base, (i, fin[i], of.base)
-    assert int(fin[-1]) + len(of.blocks) == int(blk.max()) + 1
+    assert list(blk) == _python_gate_blocks(gates, width)
+    assert all(a <= b for a, b in zip(fin[:-1], fin[1:]))
+    qs = [gq for gq, _ in gates]
+    for g in range(0, len(gates), max(1, len(gates) // 60)):
+        assert plan_blocks(qs[:g + 1], width, n)[1] >= fin[g]
+        assert all(blk[i] >= fin[g] for i in range(g + 1, len(gates)))   # taken blocks hold no later gate
+    if n == 30:
+        assert fin[-1] >= 50     # most blocks launch while the circuit is still being recorded
+
+
+def test_native_plan_random_gates():
+    """The same on gates of 1..4 qubits on random qubit sets, for widths 4 and 6."""
+    for seed in range(3):
+        gates = _random_gates(14 + seed, 300, seed, kmax=4)
+        for width in (4, 6):
+            blk, _ = _native_plan(14 + seed, width, gates)
+            assert list(blk) == _python_gate_blocks(gates, width)
 
 
 def test_native_recorder_plan_rejects():

@@ -173,7 +173,7 @@ def _apply_python_plan(ref, n, blocks):
 
 @pytest.mark.parametrize("n,streaming", [(16, True), (16, False), (10, True)])
 def test_recorder_blocks_equal_python_plan(n, streaming):
-    """The library's recorder launches exactly the blocks of the Python statement of the greedy
+    """The library's recorder launches exactly the blocks of the Python statement of the light-cone
     rule, in the same launches: a bitwise equal state, streamed or deferred."""
     from qip_amd.fused import FusedBackend, descriptor_gates, fuse
     from qip_amd.workloads import layered_random_circuit
