@@ -3,27 +3,33 @@
 ``FusedBackend`` is a StateType (qip/backend.py:14-65) like GpuBackend, but kronselect_dot only
 records: each engine call of the plan (targets, controls, matrix) becomes a dense gate on its
 controls + targets, and the descriptor's gates go to the library in one qipgpu_fused_record call.
-The library fuses them greedily into blocks of at most ``fusion_width`` qubits and applies each
-block with the split-BF16 MFMA kernel (qipgpu_fused_apply); a run of consecutive blocks made only of
+The library plans them into blocks of at most ``fusion_width`` qubits and applies each block with
+the split-BF16 MFMA kernel (qipgpu_fused_apply); a run of consecutive blocks made only of
 single-qubit gates is one product of per-qubit 2x2s on the FP64 product sweep instead. A block is
-launched as soon as no later gate can join it (streaming), and everything recorded before the state
-is next read or changed any other way (func_apply, measurement, probabilities, get_state, ranges).
-Every MFMA block renormalises the previous block's output with its FP64 norm. Gates wider than the
-fusion width (a multi-controlled gate, say) are applied unfused by the plain path, in order.
+launched as soon as no later gate can change it (streaming), and everything recorded before the
+state is next read or changed any other way (func_apply, measurement, probabilities, get_state,
+ranges). Every MFMA block renormalises the previous block's output with its FP64 norm. Gates wider
+than the fusion width (a multi-controlled gate, say) are applied unfused by the plain path, in order.
 
-Greedy fusion. Gates are taken in program order. For gate g on qubit set S let j be the last
-block that touches S. g may join block j or any later block (those do not touch S, so g commutes
-past them) whose qubit set stays within the width; the first that fits is taken, else g opens a
-new block at the end. A block's matrix is the ordered product of its gates embedded in the
-block's qubits. On the 30-qubit depth-40 brickwork circuit this gives 128 blocks for 1780 gates.
-``OnlineFusion`` and ``fuse`` below state the rule in Python (the library's recorder is checked
-against them); ``compose`` is the library's block composition.
+Light-cone rule. A gate waits for the previous gate on each of its qubits and is ready once those
+are placed. A block starts from the earliest ready gate with its qubits as the block's set S, then
+alternates: every ready gate within S joins; S takes the qubits of the earliest ready gate that
+shares a qubit with S and keeps S within the width (else of the earliest ready gate that keeps it
+within the width), until neither step changes anything. The next block starts from what is left. A
+block's matrix is the ordered product of its gates (program order) embedded in the block's qubits.
+Growing along shared qubits follows the gates' light cone: on the 30-qubit depth-40 brickwork
+circuit this gives 77 blocks for 1780 gates, where adding each gate to the first block it fits in
+program order gives 128. ``plan_blocks``, ``fuse`` and ``StreamingFusion`` below state the rule in
+Python (the library's recorder is checked against them); ``compose`` is the library's block
+composition.
 
-Streaming. The rule is online: a gate's placement depends only on the blocks before it. A later gate
-scans from the last block touching one of its qubits, so once every qubit of the state has been
-touched by a block after block i, no later gate can reach block i: it is final and can run while the
-host records the rest. Launching final blocks early gives the same blocks, in the same order, with
-the same results, as fusing the whole record at the end.
+Streaming. A gate recorded later can change the plan only if it becomes ready while a block is being
+planned, which needs all its qubits to be without a recorded gate still outside the blocks. So a
+block whose qubits all keep such a gate, and which is full or leaves no qubit of the state without
+one, is final: it and the blocks before it are the same whatever comes later. The library plans
+the record again every max(16, pending / 8) gates and launches the final leading blocks while the
+host records the rest; the blocks, their order and the results are those of planning the whole
+record at the end.
 """
 import ctypes
 from typing import List, Sequence, Tuple
