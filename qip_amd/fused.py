@@ -113,10 +113,9 @@ def plan_blocks(qsets: Sequence[Sequence[int]], width: int, n: int = 64,
     after the first block that is not final."""
     count = len(qsets)
     sets = [frozenset(q) for q in qsets]
-    full = min(width, n)
     last, npred = {}, [0] * count
     succ = [[] for _ in range(count)]
-    open_on = {}
+    open_on = [0] * max(n, 1 + max((q for S in sets for q in S), default=0))
     for g, S in enumerate(sets):
         preds = {last[q] for q in S if q in last}
         npred[g] = len(preds)
@@ -124,40 +123,73 @@ def plan_blocks(qsets: Sequence[Sequence[int]], width: int, n: int = 64,
             succ[p].append(g)
         for q in S:
             last[q] = g
-            open_on[q] = open_on.get(q, 0) + 1
-    ready = {g for g in range(count) if npred[g] == 0}
+            open_on[q] += 1
+    ready = set(g for g in range(count) if npred[g] == 0)
+
+    def absorb(S, commit, taken=None):
+        """Absorb every ready gate within S, transitively; with commit=False only count them.
+        Returns (count, True if every qubit of S keeps a recorded gate outside)."""
+        pend = {}
+        used = {}
+        work = [g for g in ready if sets[g] <= S]
+        seen = set(work)
+        c = 0
+        while work:
+            g = work.pop()
+            c += 1
+            if commit:
+                ready.discard(g)
+                taken.append(g)
+            for q in sets[g]:
+                used[q] = used.get(q, 0) + 1
+            for t in succ[g]:
+                v = pend.get(t, npred[t]) - 1
+                pend[t] = v
+                if v == 0:
+                    if commit:
+                        ready.add(t)
+                    if sets[t] <= S and t not in seen:
+                        seen.add(t)
+                        work.append(t)
+        if commit:
+            for t, v in pend.items():
+                npred[t] = v
+            for q, u in used.items():
+                open_on[q] -= u
+            return c, True
+        return c, all(open_on[q] - used.get(q, 0) > 0 for q in S)
+
     blocks, absorbed, final, prefix_final = [], 0, 0, True
     while absorbed < count:
         S = set(sets[min(ready)])
         taken = []
+        exhausted = False        # a look-ahead used up some qubit's recorded gates
         while True:
-            work = [g for g in ready if sets[g] <= S]
-            while work:
-                g = work.pop()
-                ready.discard(g)
-                taken.append(g)
-                absorbed += 1
-                for q in sets[g]:
-                    open_on[q] -= 1
-                for t in succ[g]:
-                    npred[t] -= 1
-                    if npred[t] == 0:
-                        ready.add(t)
-                        if sets[t] <= S:
-                            work.append(t)
-            order = sorted(ready)
-            pick = next((g for g in order if sets[g] & S and len(S | sets[g]) <= width), None)
-            if pick is None:
-                pick = next((g for g in order if len(S | sets[g]) <= width), None)
-            if pick is None:
+            absorbed += absorb(S, True, taken)[0]
+            cands = [g for g in sorted(ready) if len(S | sets[g]) <= width]
+            if not cands:
                 break
-            S |= sets[pick]
+            best_all = best_shared = None       # (absorbed, qubits added, gate)
+            for g in cands:
+                add = len(sets[g] - S)
+                c, kept = absorb(S | sets[g], False)
+                exhausted = exhausted or not kept
+                if best_all is None or c * best_all[1] > best_all[0] * add:
+                    best_all = (c, add, g)
+                if sets[g] & S and (best_shared is None or c * best_shared[1] > best_shared[0] * add):
+                    best_shared = (c, add, g)
+            # a candidate that shares a qubit with the block, unless another absorbs more than
+            # 1.5 times as many gates per added qubit
+            if best_shared is None or 2 * best_all[0] * best_shared[1] > 3 * best_shared[0] * best_all[1]:
+                S |= sets[best_all[2]]
+            else:
+                S |= sets[best_shared[2]]
         blocks.append((frozenset(S), sorted(taken)))
-        # A later gate changes nothing up to here unless it becomes ready during the plan, which
-        # needs every qubit of it without an open recorded gate: so the block's own qubits must
-        # all have one, and, unless the block is full, so must every qubit of the state.
-        is_final = (all(open_on[q] > 0 for q in S)
-                    and (len(S) == full or all(open_on.get(q, 0) > 0 for q in range(n))))
+        # A later gate changes nothing up to here unless it enters the plan: as a ready gate, which
+        # needs all its qubits without an open recorded gate, or through a look-ahead that used up
+        # some qubit's recorded gates. So every qubit of the state must keep one, and no look-ahead
+        # may have used one up.
+        is_final = not exhausted and all(open_on[q] > 0 for q in range(n))
         prefix_final = prefix_final and is_final
         final += prefix_final
         if stop_at_open and not prefix_final:

@@ -64,13 +64,23 @@ def test_fusion_preserves_operator(width, shape):
 
 
 def test_block_counts_brickwork():
-    """On the brickwork circuits the light-cone rule covers diamonds of the circuit: 77 six-qubit
+    """On the brickwork circuits the light-cone rule covers diamonds of the circuit: 71 six-qubit
     blocks for the 1780 gates at 30 qubits (128 when each gate joins the first block, in program
-    order, that it fits), 42 for the 830 gates at 28 qubits depth 20."""
+    order, that it fits), 37 for the 830 gates at 28 qubits depth 20."""
     gates = gates_of(layered_random_circuit(30, 40, seed=1))
     assert len(gates) == 1780
-    assert len(fuse(gates, 6)) == 77
-    assert len(fuse(gates_of(layered_random_circuit(28, 20, seed=2024)), 6)) == 42
+    assert len(fuse(gates, 6)) == 71
+    assert len(fuse(gates_of(layered_random_circuit(28, 20, seed=2024)), 6)) == 37
+
+
+def test_block_counts_qft():
+    """QFT circuits (H, then controlled phases from every later qubit): the look-ahead lets a block take
+    the qubits its next gates need, 28 blocks for 24 qubits (48 without it)."""
+    gates = []
+    for i in range(24):
+        gates.append(((i,), numpy.eye(2)))
+        gates.extend(((j, i), numpy.eye(4)) for j in range(i + 1, 24))
+    assert len(fuse(gates, 6)) == 28
 
 
 def _check_plan(gates, blocks, width):
@@ -157,26 +167,64 @@ def test_streaming_fusion_equals_batch_fusion(n, depth, seed):
             assert a.qubits == b.qubits
             assert [id(g[1]) for g in a.gates] == [id(g[1]) for g in b.gates]
         if n == 30:
-            assert len(got) == 77 and early >= 50
+            assert len(got) == 71 and early >= 50
 
 
 def test_streaming_fusion_waits_for_untouched_qubits():
     """(0, 1), (1, 2, 3), (0, 2) at width 3: the first block is {0, 1} (the three-qubit gate does not
-    fit beside it) and both of its qubits keep a recorded gate outside it. On 4 qubits it is final;
-    on 5 it is not, since qubit 4 has no recorded gate and a later gate there could join it."""
+    fit beside it) and every qubit keeps a recorded gate outside it. On 4 qubits it is final; on 5 it
+    is not, since qubit 4 has no recorded gate and a later gate there could join it."""
     from qip_amd.fused import plan_blocks
     qs = [(0, 1), (1, 2, 3), (0, 2)]
     blocks, final = plan_blocks(qs, 3, 4)
     assert [(sorted(S), t) for S, t in blocks][0] == ([0, 1], [0])
-    assert final >= 1
+    assert final == 1
     assert plan_blocks(qs, 3, 5)[1] == 0
-    # a full-width block is final once each of its qubits keeps a recorded gate, untouched qubits or not
-    blocks, final = plan_blocks([(0, 1, 2), (2, 3), (0, 1)], 3, 8)
+    # qubits 0 and 1 have nothing left outside the first block: not final
+    blocks, final = plan_blocks([(0, 1, 2), (2, 3), (0, 1)], 3, 4)
     assert [(sorted(S), t) for S, t in blocks][0] == ([0, 1, 2], [0, 2])
-    assert final == 0                       # qubits 0 and 1 have nothing left outside the block...
-    blocks, final = plan_blocks([(0, 1, 2), (2, 3), (0, 3), (1, 3)], 3, 8)
+    assert final == 0
+    # each of them keeps one here
+    blocks, final = plan_blocks([(0, 1, 2), (2, 3), (0, 3), (1, 3)], 3, 4)
     assert [(sorted(S), t) for S, t in blocks][0] == ([0, 1, 2], [0])
-    assert final == 1                       # ...here each of 0, 1, 2 has one
+    assert final == 1
+
+
+def test_streaming_fusion_waits_after_exhausting_look_ahead():
+    """(1, 2), (0, 1), (2, 3), (1, 3), (0, 1) at width 3: the first block is {0, 1, 2} and every qubit
+    keeps a recorded gate outside it, but the look-ahead of (2, 3) absorbed qubit 2's last recorded
+    gate. A later gate on qubit 2 changes that look-ahead, and the block: it is not final."""
+    from qip_amd.fused import plan_blocks
+    qs = [(1, 2), (0, 1), (2, 3), (1, 3), (0, 1)]
+    blocks, final = plan_blocks(qs, 3, 4)
+    assert [(sorted(S), t) for S, t in blocks][0] == ([0, 1, 2], [0, 1])
+    assert final == 0
+    assert sorted(plan_blocks(qs + [(2,)], 3, 4)[0][0][0]) == [1, 2, 3]
+
+
+def test_streaming_fusion_random_records():
+    """Random gate lists of 1..4 qubits, random widths, final blocks taken at random moments: always
+    the blocks of fuse() on the whole record."""
+    import random
+    from qip_amd.fused import StreamingFusion
+    for trial in range(60):
+        rng = numpy.random.default_rng(trial)
+        n, width = int(rng.integers(3, 14)), int(rng.integers(2, 7))
+        gates = [(tuple(int(q) for q in rng.choice(n, int(rng.integers(1, min(width, 4, n) + 1)), replace=False)),
+                  numpy.eye(2)) for _ in range(int(rng.integers(1, 120)))]
+        gates = [(gq, numpy.eye(2 ** len(gq))) for gq, _ in gates]
+        ref = fuse(gates, width)
+        rnd = random.Random(trial)
+        sf = StreamingFusion(n, width)
+        got = []
+        cadence = rnd.choice([0.05, 0.3, 1.0])
+        for gq, G in gates:
+            sf.add(gq, G)
+            if rnd.random() < cadence:
+                got.extend(sf.take_final())
+        got.extend(sf.take())
+        assert [(b.qubits, [id(g[1]) for g in b.gates]) for b in got] == \
+               [(b.qubits, [id(g[1]) for g in b.gates]) for b in ref], trial
 
 
 def _compose_tensordot(qs, gates):
@@ -371,12 +419,14 @@ def test_native_launches_runs_and_streaming(n, seed):
 
 
 def test_native_launches_known_runs():
-    """H on 17 of 18 qubits, a 2-qubit gate inside the first block, then more single-qubit layers:
-    blocks [mixed, product, product] and the two product blocks share one launch; below 12 qubits
-    every block is its own launch."""
+    """A 2-qubit gate, H on 17 of 18 qubits, then H on the even ones: the plan is [mixed, product,
+    product] and the two product blocks share one launch; below 12 qubits every block is its own
+    launch."""
     H = numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2)
-    gates = [((q,), H) for q in range(17)] + [((3, 4), numpy.eye(4))] + [((q,), H) for q in range(0, 17, 2)]
+    gates = [((3, 4), numpy.eye(4))] + [((q,), H) for q in range(17)] + [((q,), H) for q in range(0, 17, 2)]
+    kinds = [all(len(gq) == 1 for gq, _ in b.gates) for b in fuse(gates, 6)]
+    assert kinds == [False, True, True]
     assert _native_launches(18, 6, True, gates).tolist() == [0, 1, 1]
     assert _native_launches(18, 6, False, gates).tolist() == [0, 1, 1]
-    small = [((q,), H) for q in range(10)] + [((3, 4), numpy.eye(4))]
+    small = [((3, 4), numpy.eye(4))] + [((q,), H) for q in range(10)]
     assert _native_launches(10, 6, True, small).tolist() == [0, 1]

@@ -198,7 +198,7 @@ def test_single_qubit_runs_take_the_product_sweep():
     n = 18
     psi = rand_state(n)
     H = numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2)
-    ops = [{q: H for q in range(17)}, {(3, 4): rand_u(4)}, {q: rand_u(2) for q in range(0, 17, 2)},
+    ops = [{(3, 4): rand_u(4)}, {q: H for q in range(17)}, {q: rand_u(2) for q in range(0, 17, 2)},
            {q: H for q in range(17)}]
     fz = FusedBackend.make_state(n, [list(range(n))], [psi])
     for d in ops:
