@@ -5,14 +5,15 @@
 // all of those are in blocks. A block starts from the earliest ready gate, with qubit set S = its
 // qubits, and alternates two steps until no ready gate fits beside S:
 //   absorb: every ready gate whose qubits lie in S joins the block (which can make later gates ready);
-//   grow:   each ready gate g that keeps S within the width is scored by a look-ahead, the number of
-//           gates the block would absorb with S u qubits(g) per qubit added. S takes the qubits of the
-//           best-scoring gate that shares a qubit with S, unless the best of all scores more than 1.5
-//           times as high (or none shares one); ties go to the earliest gate.
+//   grow:   each ready gate g that keeps S within the width is scored by a two-step look-ahead: the
+//           number of gates the block would absorb with S u qubits(g) per qubit added, plus half of
+//           the same for the best further gate after that. S takes the qubits of the best-scoring
+//           gate that shares a qubit with S, unless the best of all scores more than 1.5 times as
+//           high (or none shares one); ties go to the earliest gate. Scores are exact fractions.
 // The next block starts from what is left. Each block's gates, in program order, compose to its
 // operator; the order of the blocks respects every dependency. Growing along shared qubits follows
 // the light cone of the gates already in the block, so on brickwork circuits a block covers a diamond
-// of the circuit instead of a few layers of one window (on the 30-qubit depth-40 circuit: 71 blocks of
+// of the circuit instead of a few layers of one window (on the 30-qubit depth-40 circuit: 66 blocks of
 // six qubits, where adding each gate to the first compatible block in program order gives 128); the
 // look-ahead lets blocks of QFT-like circuits take the qubits their next gates need (QFT on 24 qubits:
 // 28 blocks instead of 48).
@@ -107,14 +108,18 @@ inline int64_t plan_blocks(const uint64_t *masks, int64_t count, int width, int 
         }
         return c;
     };
-    // how many gates absorb(S) would take; `kept` = every qubit of S keeps a recorded gate outside
-    auto look_ahead = [&](uint64_t S, bool &kept) {
+    // how many gates absorb(S) would take; `kept` = every qubit of S keeps a recorded gate outside;
+    // `left` (if given) receives the gates that would be ready outside S afterwards
+    auto look_ahead = [&](uint64_t S, bool &kept, std::vector<int64_t> *left) {
         ++cur;
         int64_t used[64] = {0};
         int64_t c = 0;
         work.clear();
-        for (int64_t g : ready)
+        if (left) left->clear();
+        for (int64_t g : ready) {
             if ((masks[g] & ~S) == 0) work.push_back(g);
+            else if (left) left->push_back(g);
+        }
         while (!work.empty()) {
             const int64_t g = work.back();
             work.pop_back();
@@ -126,7 +131,10 @@ inline int64_t plan_blocks(const uint64_t *masks, int64_t count, int width, int 
                     stamp[t] = cur;
                     sim_pred[t] = npred[t];
                 }
-                if (--sim_pred[t] == 0 && (masks[t] & ~S) == 0) work.push_back(t);
+                if (--sim_pred[t] == 0) {
+                    if ((masks[t] & ~S) == 0) work.push_back(t);
+                    else if (left) left->push_back(t);
+                }
             }
         }
         kept = true;
@@ -135,6 +143,30 @@ inline int64_t plan_blocks(const uint64_t *masks, int64_t count, int width, int 
             kept = kept && open_on[q] - used[q] > 0;
         }
         return c;
+    };
+    // Look-ahead score of growing S by gate g as the fraction num / den: the gates absorbed per qubit
+    // added, plus half of the same for the best further step; `kept` as in look_ahead, over every
+    // simulation made.
+    std::vector<int64_t> left;
+    auto score = [&](uint64_t S, int64_t g, int64_t &num, int64_t &den, bool &kept) {
+        const uint64_t S2 = S | masks[g];
+        const int64_t a = __builtin_popcountll(S2) - __builtin_popcountll(S);
+        const int64_t c = look_ahead(S2, kept, &left);
+        int64_t d = 0, a2 = 1;
+        for (int64_t h : left) {
+            const uint64_t S3 = S2 | masks[h];
+            if (__builtin_popcountll(S3) > width) continue;
+            bool kept3;
+            const int64_t c3 = look_ahead(S3, kept3, nullptr);
+            kept = kept && kept3;
+            const int64_t a3 = __builtin_popcountll(S3) - __builtin_popcountll(S2);
+            if ((c3 - c) * a2 > d * a3) {
+                d = c3 - c;
+                a2 = a3;
+            }
+        }
+        num = 2 * c * a2 + a * d;
+        den = 2 * a * a2;
     };
 
     int64_t absorbed = 0, final_prefix = 0;
@@ -145,27 +177,27 @@ inline int64_t plan_blocks(const uint64_t *masks, int64_t count, int width, int 
         bool exhausted = false;   // a look-ahead used up some qubit's recorded gates
         for (;;) {
             absorbed += absorb(S, b);
-            int64_t ga = -1, ca = 0, aa = 1;   // best of all: gate, gates absorbed, qubits added
-            int64_t gs = -1, cs = 0, as = 1;   // best sharing a qubit with S
+            int64_t ga = -1, na = 0, da = 1;   // best of all: gate, score numerator, denominator
+            int64_t gs = -1, ns = 0, ds = 1;   // best sharing a qubit with S
             for (int64_t g : ready) {
                 if (__builtin_popcountll(S | masks[g]) > width) continue;
-                const int64_t add = __builtin_popcountll(masks[g] & ~S);
+                int64_t num, den;
                 bool kept;
-                const int64_t c = look_ahead(S | masks[g], kept);
+                score(S, g, num, den, kept);
                 exhausted = exhausted || !kept;
-                if (ga < 0 || c * aa > ca * add) {
+                if (ga < 0 || num * da > na * den) {
                     ga = g;
-                    ca = c;
-                    aa = add;
+                    na = num;
+                    da = den;
                 }
-                if ((masks[g] & S) && (gs < 0 || c * as > cs * add)) {
+                if ((masks[g] & S) && (gs < 0 || num * ds > ns * den)) {
                     gs = g;
-                    cs = c;
-                    as = add;
+                    ns = num;
+                    ds = den;
                 }
             }
             if (ga < 0) break;
-            S |= (gs < 0 || 2 * ca * as > 3 * cs * aa) ? masks[ga] : masks[gs];
+            S |= (gs < 0 || 2 * na * ds > 3 * ns * da) ? masks[ga] : masks[gs];
         }
         std::sort(b.gates.begin(), b.gates.end());
         b.mask = S;

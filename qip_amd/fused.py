@@ -127,12 +127,13 @@ def plan_blocks(qsets: Sequence[Sequence[int]], width: int, n: int = 64,
     ready = set(g for g in range(count) if npred[g] == 0)
 
     def absorb(S, commit, taken=None):
-        """Absorb every ready gate within S, transitively; with commit=False only count them.
-        Returns (count, True if every qubit of S keeps a recorded gate outside)."""
+        """Absorb every ready gate within S, transitively; with commit=False only simulate it.
+        Returns (count, True if every qubit of S keeps a recorded gate outside, the ready gates
+        left outside S)."""
         pend = {}
         used = {}
         work = [g for g in ready if sets[g] <= S]
-        seen = set(work)
+        left = [g for g in ready if not sets[g] <= S]
         c = 0
         while work:
             g = work.pop()
@@ -148,16 +149,35 @@ def plan_blocks(qsets: Sequence[Sequence[int]], width: int, n: int = 64,
                 if v == 0:
                     if commit:
                         ready.add(t)
-                    if sets[t] <= S and t not in seen:
-                        seen.add(t)
+                    if sets[t] <= S:
                         work.append(t)
+                    else:
+                        left.append(t)
         if commit:
             for t, v in pend.items():
                 npred[t] = v
             for q, u in used.items():
                 open_on[q] -= u
-            return c, True
-        return c, all(open_on[q] - used.get(q, 0) > 0 for q in S)
+            return c, True, left
+        return c, all(open_on[q] - used.get(q, 0) > 0 for q in S), left
+
+    def score(S, g):
+        """Look-ahead score of growing S by gate g, as a fraction (num, den): the gates absorbed per
+        qubit added, plus half of the same for the best further step. Also whether every simulation
+        kept a recorded gate on each of its qubits."""
+        S2 = S | sets[g]
+        a = len(S2) - len(S)
+        c, kept, left = absorb(S2, False)
+        d, a2 = 0, 1
+        for h in left:
+            S3 = S2 | sets[h]
+            if len(S3) > width:
+                continue
+            c3, kept3, _ = absorb(S3, False)
+            kept = kept and kept3
+            if (c3 - c) * a2 > d * (len(S3) - len(S2)):
+                d, a2 = c3 - c, len(S3) - len(S2)
+        return (2 * c * a2 + a * d, 2 * a * a2), kept
 
     blocks, absorbed, final, prefix_final = [], 0, 0, True
     while absorbed < count:
@@ -166,20 +186,20 @@ def plan_blocks(qsets: Sequence[Sequence[int]], width: int, n: int = 64,
         exhausted = False        # a look-ahead used up some qubit's recorded gates
         while True:
             absorbed += absorb(S, True, taken)[0]
-            cands = [g for g in sorted(ready) if len(S | sets[g]) <= width]
-            if not cands:
-                break
-            best_all = best_shared = None       # (absorbed, qubits added, gate)
-            for g in cands:
-                add = len(sets[g] - S)
-                c, kept = absorb(S | sets[g], False)
+            best_all = best_shared = None       # (score numerator, denominator, gate)
+            for g in sorted(ready):
+                if len(S | sets[g]) > width:
+                    continue
+                (num, den), kept = score(S, g)
                 exhausted = exhausted or not kept
-                if best_all is None or c * best_all[1] > best_all[0] * add:
-                    best_all = (c, add, g)
-                if sets[g] & S and (best_shared is None or c * best_shared[1] > best_shared[0] * add):
-                    best_shared = (c, add, g)
-            # a candidate that shares a qubit with the block, unless another absorbs more than
-            # 1.5 times as many gates per added qubit
+                if best_all is None or num * best_all[1] > best_all[0] * den:
+                    best_all = (num, den, g)
+                if sets[g] & S and (best_shared is None or num * best_shared[1] > best_shared[0] * den):
+                    best_shared = (num, den, g)
+            if best_all is None:
+                break
+            # a candidate that shares a qubit with the block, unless another scores more than 1.5
+            # times as high
             if best_shared is None or 2 * best_all[0] * best_shared[1] > 3 * best_shared[0] * best_all[1]:
                 S |= sets[best_all[2]]
             else:

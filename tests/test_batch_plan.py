@@ -114,11 +114,11 @@ def test_host_plan_equals_python_fusion(width):
 
 
 def test_host_plan_workload_blocks():
-    """30-qubit depth-40 circuit: the C++ plan gives the 71 blocks the single-state path uses."""
+    """30-qubit depth-40 circuit: the C++ plan gives the 66 blocks the single-state path uses."""
     gates = workloads.layered_circuit_gates(30, 40, 2024)
     packed = batch.PackedCircuits.from_gates([30], [gates])
     _, state_blocks = batch.plan_fusion(packed)
-    assert state_blocks[0] == 71
+    assert state_blocks[0] == 66
 
 
 @pytest.mark.parametrize("bad", [
