@@ -37,12 +37,12 @@ HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, published
 
 # HBM traffic of one k_fused6 launch on a 30-qubit complex64 state (8 GiB), measured by two
 # rocprofv3 --pmc runs of `bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-plain
-# --copy-calibration-gib 1` (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch, mean over the 213
+# --copy-calibration-gib 1` (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch, mean over the 198
 # block launches of the run). Each counter is scaled by what it reads for the 1 GiB device-to-device
-# copy in the same run: FETCH_SIZE reads 524315.0 KiB for 1048576 KiB moved (x2.0), WRITE_SIZE
+# copy in the same run: FETCH_SIZE reads 524314.1 KiB for 1048576 KiB moved (x2.0), WRITE_SIZE
 # 1048576 KiB (x1.0). Commands and per-kernel figures: profiles/k_fused6_circuit30_depth40.txt.
-FUSED6_PMC_30Q = {"fetch_kib": 4195898.7, "fetch_scale": 1048576.0 / 524315.0,
-                  "write_kib": 8388630.1, "write_scale": 1.0}
+FUSED6_PMC_30Q = {"fetch_kib": 4195892.4, "fetch_scale": 1048576.0 / 524314.125,
+                  "write_kib": 8388629.0, "write_scale": 1.0}
 
 
 def fused6_traffic_bytes(n: int):

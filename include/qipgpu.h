@@ -162,10 +162,11 @@ int qipgpu_compose(int32_t k, const int32_t *qubits, int32_t n_gates, const int3
  * (1..width) distinct qubits gate_qubits[6 g ..] (first = MSB of its matrix) and the next
  * 4^gate_k[g] interleaved complex entries of `mats`. Gates are planned into blocks of at most
  * `width` qubits by the light-cone rule (a block grows from the earliest gate whose predecessors are
- * all placed, absorbing every such gate within its qubits and taking the qubits of the earliest one
- * that shares a qubit with it while the width allows). A block that no later gate can change is
- * composed (qipgpu_compose) and launched when streaming (qipgpu_fused_apply for n >= 12, the plain
- * dense sweep below); the record is planned again every max(16, pending / 8) gates. For n >= 12 a run
+ * all placed, absorbing every such gate within its qubits and taking the qubits of the one, preferably
+ * sharing a qubit with it, that a two-step look-ahead scores best, while the width allows). A block
+ * that no later gate can change is composed (qipgpu_compose) and launched when streaming
+ * (qipgpu_fused_apply for n >= 12, the plain dense sweep below); the record is planned again every
+ * max(16, pending / 8) gates. For n >= 12 a run
  * of consecutive blocks holding single-qubit gates only is one product of per-qubit 2x2s on the
  * product sweep (qipgpu_apply_product, FP64 arithmetic); a trailing run waits until a final block
  * that is not single-qubit-only follows it, so runs launch whole. Streaming and deferred recording

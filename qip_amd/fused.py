@@ -13,23 +13,24 @@ than the fusion width (a multi-controlled gate, say) are applied unfused by the 
 
 Light-cone rule. A gate waits for the previous gate on each of its qubits and is ready once those
 are placed. A block starts from the earliest ready gate with its qubits as the block's set S, then
-alternates: every ready gate within S joins; S takes the qubits of the earliest ready gate that
-shares a qubit with S and keeps S within the width (else of the earliest ready gate that keeps it
-within the width), until neither step changes anything. The next block starts from what is left. A
-block's matrix is the ordered product of its gates (program order) embedded in the block's qubits.
-Growing along shared qubits follows the gates' light cone: on the 30-qubit depth-40 brickwork
-circuit this gives 77 blocks for 1780 gates, where adding each gate to the first block it fits in
-program order gives 128. ``plan_blocks``, ``fuse`` and ``StreamingFusion`` below state the rule in
-Python (the library's recorder is checked against them); ``compose`` is the library's block
-composition.
+alternates until no ready gate fits beside S: every ready gate within S joins; then S grows by the
+qubits of one ready gate that keeps it within the width, chosen by a two-step look-ahead (gates the
+block would absorb per qubit added, plus half of the same for the best further gate), preferring a
+gate that shares a qubit with S unless another scores more than 1.5 times as high. The next block
+starts from what is left. A block's matrix is the ordered product of its gates (program order)
+embedded in the block's qubits. Growing along shared qubits follows the gates' light cone: on the
+30-qubit depth-40 brickwork circuit this gives 66 blocks for 1780 gates, where adding each gate to
+the first block it fits in program order gives 128. ``plan_blocks``, ``fuse`` and
+``StreamingFusion`` below state the rule in Python (the library's recorder is checked against them);
+``compose`` is the library's block composition.
 
-Streaming. A gate recorded later can change the plan only if it becomes ready while a block is being
-planned, which needs all its qubits to be without a recorded gate still outside the blocks. So a
-block whose qubits all keep such a gate, and which is full or leaves no qubit of the state without
-one, is final: it and the blocks before it are the same whatever comes later. The library plans
-the record again every max(16, pending / 8) gates and launches the final leading blocks while the
-host records the rest; the blocks, their order and the results are those of planning the whole
-record at the end.
+Streaming. A gate recorded later can change the plan only if it enters it: as a ready gate, which
+needs all its qubits to be without a recorded gate still outside the blocks, or inside a look-ahead
+that used up some qubit's recorded gates. So a block is final (it and the blocks before it are the
+same whatever comes later) when every qubit of the state keeps such a gate and none of the block's
+look-aheads used one up. The library plans the record again every max(16, pending / 8) gates and
+launches the final leading blocks while the host records the rest; the blocks, their order and the
+results are those of planning the whole record at the end.
 """
 import ctypes
 from typing import List, Sequence, Tuple
