@@ -146,7 +146,6 @@ def test_fused_backend_layered_circuit():
     assert abs(fz.total_prob() - 1.0) < 1e-6
 
 
-@pytest.mark.parametrize("n", [16, 10])
 def _apply_python_plan(ref, n, blocks):
     """The recorder's launches restated: each block composed by qipgpu_compose and applied (MFMA
     kernel at n >= 12, plain dense sweep below); at n >= 12 a run of consecutive single-qubit-only
@@ -188,6 +187,38 @@ def test_recorder_blocks_equal_python_plan(n, streaming):
     ref = GpuBackend.make_state(n, [], [], statetype=numpy.complex64)
     _apply_python_plan(ref, n, blocks)
     assert numpy.array_equal(got, ref.get_state())
+
+
+@pytest.mark.parametrize("n,seed", [(14, 1), (17, 2)])
+def test_recorder_random_gates(n, seed):
+    """Gates of 1..4 qubits on random qubit sets (the look-ahead of the plan at work): streamed and
+    deferred recording give the same bits, so do the Python plan's blocks applied one by one, and
+    the result stays within 1e-9 infidelity per block of complex128."""
+    from qip_amd.fused import FusedBackend, fuse
+    rng = numpy.random.default_rng(seed)
+    gates = []
+    for _ in range(160):
+        L = int(rng.integers(1, 5))
+        gates.append((tuple(int(q) for q in rng.choice(n, L, replace=False)), rand_u(2 ** L)))
+    psi = rand_state(n)
+    outs = []
+    for streaming in (True, False):
+        fz = FusedBackend.make_state(n, [list(range(n))], [psi])
+        fz.streaming = streaming
+        for gq, G in gates:
+            fz.kronselect_dot({gq: G})
+        outs.append(fz.get_state())
+    assert numpy.array_equal(outs[0], outs[1])
+    blocks = fuse(gates, 6)
+    assert fz.blocks_applied == len(blocks)
+    ref = GpuBackend.make_state(n, [list(range(n))], [psi], statetype=numpy.complex64)
+    _apply_python_plan(ref, n, blocks)
+    assert numpy.array_equal(outs[0], ref.get_state())
+    exact = GpuBackend.make_state(n, [list(range(n))], [psi])
+    for gq, G in gates:
+        exact.kronselect_dot({gq: G})
+    assert infidelity(exact, fz) < 1e-9 * len(blocks)
+    assert numpy.max(numpy.abs(exact.get_state() - outs[0])) < 1e-5
 
 
 def test_single_qubit_runs_take_the_product_sweep():
