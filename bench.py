@@ -21,6 +21,11 @@ Beside it, in the same run:
     the reference's -O3 -ffast-math -fopenmp flags, all visible cores) on a time-bounded sample of
     the circuit's own gates at full size, extrapolated to the circuit's gate mix.
 
+Other configurations, one JSON line each (--workload): circuit28 (28-qubit depth-20 circuit, plain
+complex128 vs unfused complex64), grover24 (24-qubit Grover search in both forms), qft16 (16-qubit
+QFT graph), batch512 (512 uneven circuits in the batching form), measure28 (marginal probabilities,
+norm and measurement on a 28-qubit complex128 state).
+
 --no-cpu-baseline leaves out the CPU leg (for profiler runs); it does not change the timed work.
 """
 import argparse
@@ -43,6 +48,14 @@ HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, published
 # 1048576 KiB (x1.0). Commands and per-kernel figures: profiles/k_fused6_circuit30_depth40.txt.
 FUSED6_PMC_30Q = {"fetch_kib": 4195892.4, "fetch_scale": 1048576.0 / 524314.125,
                   "write_kib": 8388629.0, "write_scale": 1.0}
+
+
+# HBM traffic of one k_marg_pass launch on a 28-qubit complex128 state (4 GiB), measured the same way
+# by two rocprofv3 --pmc runs of `bench.py --workload measure28 --no-cpu-baseline
+# --copy-calibration-gib 1` (mean over the 42 launches of the run; FETCH_SIZE reads 524313.75 KiB for
+# the 1 GiB copy, WRITE_SIZE 1048576 KiB). Commands and figures: profiles/k_marg_pass_measure28.txt.
+MARG_PMC_28Q = {"fetch_kib": 2097182.76, "fetch_scale": 1048576.0 / 524313.75,
+                "write_kib": 3026.64, "write_scale": 1.0}
 
 
 def fused6_traffic_bytes(n: int):
@@ -381,6 +394,83 @@ def bench_qft16(args):
     return out
 
 
+def bench_measure28(args):
+    """Measurement reductions on a 28-qubit complex128 state (4 GiB; H then one Haar unitary on every
+    qubit): marginal probabilities over 8 high, 8 low and 8 scattered qubits, 12 low, one high and one
+    low qubit (measure_probabilities, one pass over the state each); prob_magnitude (total_prob); and
+    measure on the 8 scattered qubits (norm, marginals, collapse). Algorithmic bytes per call as
+    defined for these functions: S 2^n + 8 2^k for the probabilities, S 2^n for the norm, 4 S 2^n for
+    measure."""
+    import random
+    import numpy
+    from qip_amd import GpuBackend
+    from qip_amd.workloads import random_product_unitaries
+    n, amp = 28, 16
+    H = numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2)
+    be = GpuBackend.make_state(n, [], [])
+    be.kronselect_dot({q: H for q in range(n)})
+    for d in random_product_unitaries(n, seed=args.seed):
+        be.kronselect_dot(d)
+    be.synchronize()
+    ev = HipEvents()
+    state_bytes = amp * 2 ** n
+    reps = max(3, args.steps)
+
+    def timed(call):
+        call()
+        be.synchronize()
+        t0, t1 = ev.create(), ev.create()
+        w = time.perf_counter()
+        ev.record(t0, be.stream)
+        for _ in range(reps):
+            call()
+        ev.record(t1, be.stream)
+        be.synchronize()
+        return ev.elapsed_ms(t0, t1) / reps, (time.perf_counter() - w) * 1e3 / reps
+
+    cases = {"high8": list(range(8)), "low8": list(range(n - 8, n)), "scattered8": [0, 5, 9, 13, 17, 21, 25, 27],
+             "low12": list(range(n - 12, n)), "high1": [0], "low1": [n - 1]}
+    probs = {}
+    for name, idx in cases.items():
+        ms, wall = timed(lambda idx=idx: be.measure_probabilities(idx))
+        b = state_bytes + 8 * 2 ** len(idx)
+        probs[name] = {"qubits": idx, "ms": ms, "wall_ms": wall, "algorithmic_gbs": b / ms / 1e6,
+                       "hbm_frac": b / ms / 1e6 / HBM_PEAK_GBS}
+    ms, wall = timed(lambda: be.total_prob())
+    norm = {"ms": ms, "wall_ms": wall, "algorithmic_gbs": state_bytes / ms / 1e6,
+            "hbm_frac": state_bytes / ms / 1e6 / HBM_PEAK_GBS}
+    random.seed(args.seed)
+    ms, wall = timed(lambda: be.measure(cases["scattered8"]))
+    meas = {"qubits": cases["scattered8"], "ms": ms, "wall_ms": wall, "algorithmic_gbs": 4 * state_bytes / ms / 1e6,
+            "hbm_frac": 4 * state_bytes / ms / 1e6 / HBM_PEAK_GBS}
+    eight = [probs[c] for c in ("high8", "low8", "scattered8")]
+    value = sum(c["algorithmic_gbs"] for c in eight) / 3
+    avg_ms = sum(c["ms"] for c in eight) / 3
+    cfg = {"workload": "28-qubit complex128 state (H, then a Haar unitary on every qubit): measure_probabilities "
+                       "over 8 high / 8 low / 8 scattered qubits (value: their mean), 12 low, 1 high, 1 low; "
+                       "total_prob; measure on 8 scattered qubits", "n_qubits": n, "seed": args.seed}
+    out = _line("28-qubit c128 marginal probabilities over 8 qubits: algorithmic GB/s", value, "GB/s", reps, 1,
+                avg_ms, "f64", cfg, probabilities=probs, total_prob=norm, measure=meas,
+                roofline={"bound": "hbm", "kernel": "k_marg_pass (one pass over the state, FP64 sums)",
+                          "achieved": value, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": value / HBM_PEAK_GBS,
+                          "traffic": int((MARG_PMC_28Q["fetch_kib"] * MARG_PMC_28Q["fetch_scale"] +
+                                          MARG_PMC_28Q["write_kib"] * MARG_PMC_28Q["write_scale"]) * 1024),
+                          "bytes_per_launch": state_bytes + 8 * 2 ** 8, "avg_launch_ms": avg_ms,
+                          "note": "time per call: k_marg_pass, the fold of its parts and the copy of the 2^k "
+                                  "results to the host (the kernel alone is in profiles/k_marg_pass_measure28.txt)"})
+    if not args.no_cpu_baseline:
+        from oracle import oracle
+        v = be.get_state()
+        t = time.perf_counter()
+        oracle.measure_probabilities(cases["scattered8"], n, v)
+        cdt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": (state_bytes + 8 * 2 ** 8) / cdt / 1e9, "unit": "GB/s", "cores": 1,
+                               "kind": "port", "sample": "one full-size call of the oracle's measure_probabilities "
+                                                         "(kronprod.pyx:238-261 restated, serial) on the scattered "
+                                                         "8 qubits, {:.1f} s".format(cdt)}
+    return out
+
+
 def bench_batch512(args):
     """512 independent layered random circuits, n ~ U{10..22}, depth ~ U{8..32} (seeded), complex64:
     the batching form (one captured graph per run, autotuned) against the plain complex64 path with
@@ -517,7 +607,7 @@ def main():
                     help="after the benchmark, one device-to-device copy of this many GiB (a kernel of known "
                          "traffic for calibrating hardware byte counters)")
     ap.add_argument("--workload", default="circuit30",
-                    choices=["circuit30", "circuit28", "grover24", "qft16", "batch512"],
+                    choices=["circuit30", "circuit28", "grover24", "qft16", "batch512", "measure28"],
                     help="circuit30 (default, the headline) or one of the other configurations, each printed as "
                          "its own JSON line")
     args = ap.parse_args()
@@ -527,8 +617,10 @@ def main():
         sys.exit(2)
     if args.workload != "circuit30":
         result = {"circuit28": bench_circuit28, "grover24": bench_grover24, "qft16": bench_qft16,
-                  "batch512": bench_batch512}[args.workload](args)
+                  "batch512": bench_batch512, "measure28": bench_measure28}[args.workload](args)
         print(json.dumps(result), flush=True)
+        if args.copy_calibration_gib > 0:
+            copy_calibration(HipEvents(), int(args.copy_calibration_gib * 2 ** 30))
         return
 
     import numpy

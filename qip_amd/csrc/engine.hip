@@ -1068,9 +1068,174 @@ __global__ __launch_bounds__(256) void k_marg_final(const double *__restrict__ p
     out[t] = acc;
 }
 
+// Marginals of every outcome in one pass over the state (n >= 12). The state is read in chunks of
+// 4096 amplitudes, the state bits 0..11; the other bits are the k_hi measured bits above 11 and the
+// free (unmeasured) bits above 11. Block b takes outcome bits m_hi = b / nparts and the part
+// b % nparts of the free-bit combinations, sums |amp|^2 per chunk position over its chunks (thread t
+// keeps positions t + 256 j in registers, so every load is contiguous across the block), then folds
+// the 4096 position sums into the 2^k_lo outcomes of the measured bits among 0..11, in a fixed order.
+// k_marg_pass_final adds the parts of each outcome in order and writes it at its output index.
+constexpr int kMargChunkBits = 12;
+
+struct MargPassArgs {
+    uint64_t per;                // free-bit combinations per block
+    uint64_t free_mask;          // unmeasured state bits above 11
+    uint64_t m_total;            // 2^k outcomes
+    uint32_t nparts;
+    int32_t k, klo, khi, nfree_lo;
+    uint8_t hi_meas[QIPGPU_MAX_QUBITS];   // measured state bits above 11, ascending (m_hi bit i)
+    uint8_t lo_meas[kMargChunkBits];      // measured state bits among 0..11, ascending (m_lo bit i)
+    uint8_t lo_free[kMargChunkBits];      // unmeasured state bits among 0..11, ascending
+    uint8_t opos[QIPGPU_MAX_QUBITS];      // state bit of output bit j
+};
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_marg_pass(const cx<R> *__restrict__ amps, MargPassArgs a, double *partial) {
+    __shared__ double sums[1 << kMargChunkBits];
+    __shared__ double red[256];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t m_hi = blockIdx.x / a.nparts, part = blockIdx.x % a.nparts;
+    const uint64_t hi_base = scatter_bits(m_hi, a.hi_meas, a.khi);
+    // free bits of the first combination of this part; the next combination in ascending order is
+    // (x - free_mask) & free_mask
+    uint64_t x = 0;
+    {
+        const uint64_t v0 = part * a.per;
+        uint64_t m = a.free_mask;
+        for (int b = 0; m; ++b, m &= m - 1)
+            if (v0 >> b & 1) x |= m & (~m + 1);
+    }
+    double acc[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+    for (uint64_t v = 0; v < a.per; ++v) {
+        const cx<R> *p = amps + (hi_base | x) + tid;
+        cx<R> y[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) y[j] = p[256 * j];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const double re = y[j].re, im = y[j].im;
+            acc[j] = fma(re, re, acc[j]);
+            acc[j] = fma(im, im, acc[j]);
+        }
+        x = (x - a.free_mask) & a.free_mask;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) sums[tid + 256 * j] = acc[j];
+    __syncthreads();
+    // outcome m_lo sums the positions with its measured bits, the free ones ascending; with fewer
+    // than 256 outcomes, g = 256 / 2^klo threads share one (free combinations u = g', g' + g, ...)
+    const uint32_t nlo = 1u << a.klo;
+    const uint32_t g = nlo >= 256 ? 1u : 256u / nlo;
+    for (uint32_t idx = tid; idx < nlo * g; idx += 256) {
+        const uint32_t mlo = idx / g, gi = idx % g;
+        const uint32_t lb = (uint32_t)scatter_bits(mlo, a.lo_meas, a.klo);
+        double s = 0.0;
+        for (uint32_t u = gi; u < (uint32_t)a.nfree_lo; u += g) s += sums[lb | (uint32_t)scatter_bits(u, a.lo_free, kMargChunkBits - a.klo)];
+        if (g == 1) partial[(uint64_t)blockIdx.x * nlo + mlo] = s;
+        else red[idx] = s;
+    }
+    if (g > 1) {
+        __syncthreads();
+        if (tid < nlo) {
+            double s = 0.0;
+            for (uint32_t gi = 0; gi < g; ++gi) s += red[tid * g + gi];
+            partial[(uint64_t)blockIdx.x * nlo + tid] = s;
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t marg_out_index(uint64_t m_hi, uint64_t mlo, const MargPassArgs &a) {
+    const uint64_t sb = scatter_bits(m_hi, a.hi_meas, a.khi) | scatter_bits(mlo, a.lo_meas, a.klo);
+    uint64_t oi = 0;
+    for (int j = 0; j < a.k; ++j) oi |= ((sb >> a.opos[j]) & 1ull) << j;
+    return oi;
+}
+
+// one thread per outcome, the parts in order (few parts)
+__global__ __launch_bounds__(256) void k_marg_pass_final(const double *__restrict__ partial, MargPassArgs a, double *out) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.m_total) return;
+    const uint64_t m_hi = t >> a.klo, mlo = t & ((1ull << a.klo) - 1ull);
+    double s = 0.0;
+    for (uint32_t p = 0; p < a.nparts; ++p) s += partial[((m_hi * a.nparts + p) << a.klo) + mlo];
+    out[marg_out_index(m_hi, mlo, a)] = s;
+}
+
+// one wavefront per outcome (many parts): lane l adds parts l, l + 64, ... in order, then a butterfly
+__global__ __launch_bounds__(64) void k_marg_pass_final_wave(const double *__restrict__ partial, MargPassArgs a,
+                                                             double *out) {
+    const uint64_t t = blockIdx.x;
+    const uint64_t m_hi = t >> a.klo, mlo = t & ((1ull << a.klo) - 1ull);
+    double s = 0.0;
+    for (uint32_t p = threadIdx.x; p < a.nparts; p += 64) s += partial[((m_hi * a.nparts + p) << a.klo) + mlo];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (threadIdx.x == 0) out[marg_out_index(m_hi, mlo, a)] = s;
+}
+
 namespace {
 
 enum class Order { LsbFirst, Entwine };
+
+// Every outcome's marginal in one pass over the state (k_marg_pass), into host `out` (2^k values in
+// the output order given by opos).
+static int marginals_pass(qipgpu_state *s, const std::vector<int> &mpos, const uint8_t *opos, double *out) {
+    const int n = s->n, k = (int)mpos.size();
+    MargPassArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.k = k;
+    a.m_total = 1ull << k;
+    std::vector<int> sorted(mpos);
+    std::sort(sorted.begin(), sorted.end());
+    uint64_t meas_mask = 0;
+    for (int p : sorted) {
+        meas_mask |= 1ull << p;
+        if (p < kMargChunkBits) a.lo_meas[a.klo++] = (uint8_t)p;
+        else a.hi_meas[a.khi++] = (uint8_t)p;
+    }
+    for (int b = 0; b < kMargChunkBits; ++b)
+        if (!(meas_mask >> b & 1)) a.lo_free[a.nfree_lo++] = (uint8_t)b;
+    a.nfree_lo = 1 << a.nfree_lo;   // combinations of the free low bits
+    for (int b = kMargChunkBits; b < n; ++b)
+        if (!(meas_mask >> b & 1)) a.free_mask |= 1ull << b;
+    for (int j = 0; j < k; ++j) a.opos[j] = opos[j];
+    // about eight blocks per CU, each summing a power-of-two share of the free combinations. The
+    // partials (2^k per part, read back strided) stay within 8 MiB unless that leaves fewer than two
+    // blocks per CU, and within 256 MiB.
+    const int nfree_hi = n - kMargChunkBits - a.khi;
+    uint64_t nparts = 1;
+    auto blocks_of = [&](uint64_t parts) { return (1ull << a.khi) * parts; };
+    while (nparts < (1ull << nfree_hi) && blocks_of(nparts) < 2048 &&
+           (sizeof(double) * a.m_total * nparts * 2 <= (8ull << 20) || blocks_of(nparts) < 512) &&
+           sizeof(double) * a.m_total * nparts * 2 <= (256ull << 20))
+        nparts *= 2;
+    a.nparts = (uint32_t)nparts;
+    a.per = (1ull << nfree_hi) / nparts;
+    const uint64_t nblk = (1ull << a.khi) * nparts;
+    if (nblk > 0x7fffffffull) return set_error(QIPGPU_EUNSUP, "marginal grid too large");
+    const size_t out_bytes = sizeof(double) * a.m_total;
+    const size_t part_off = ((out_bytes + 255) / 256) * 256;
+    int rc = ensure_scratch(s, part_off + sizeof(double) * a.m_total * nparts);
+    if (rc) return rc;
+    double *d_out = static_cast<double *>(s->scratch);
+    double *d_part = reinterpret_cast<double *>(static_cast<char *>(s->scratch) + part_off);
+    if (s->dtype == QIPGPU_C128)
+        hipLaunchKernelGGL(k_marg_pass<double>, dim3((uint32_t)nblk), dim3(256), 0, s->stream, (const cx<double> *)s->amps,
+                           a, d_part);
+    else
+        hipLaunchKernelGGL(k_marg_pass<float>, dim3((uint32_t)nblk), dim3(256), 0, s->stream, (const cx<float> *)s->amps,
+                           a, d_part);
+    QIP_LAUNCHED();
+    if (nparts >= 64)   // then 2^k_hi <= 32 and m_total <= 2^17 wavefronts
+        hipLaunchKernelGGL(k_marg_pass_final_wave, dim3((uint32_t)a.m_total), dim3(64), 0, s->stream, d_part, a, d_out);
+    else
+        hipLaunchKernelGGL(k_marg_pass_final, dim3(blocks_for(a.m_total, 256)), dim3(256), 0, s->stream, d_part, a, d_out);
+    QIP_LAUNCHED();
+    QIP_HIP(hipMemcpyAsync(out, d_out, out_bytes, hipMemcpyDeviceToHost, s->stream));
+    QIP_HIP(hipStreamSynchronize(s->stream));
+    return QIPGPU_OK;
+}
 
 // Compute marginals over outcomes [m_lo, m_lo + m_count) into host `out`.
 static int marginals_host(qipgpu_state *s, int32_t k, const int32_t *indices, Order order, uint64_t m_lo,
@@ -1098,6 +1263,9 @@ static int marginals_host(qipgpu_state *s, int32_t k, const int32_t *indices, Or
         for (int j = 0; j < k; ++j) a.opos[j] = (uint8_t)sorted[j];
     }
     fill_poslist(a.mpl, mpos);
+    // every outcome of up to 2^26: one pass over the state (its partials stay within 1 GiB)
+    if (n >= kMargChunkBits && k >= 1 && k <= 26 && m_lo == 0 && m_count == (1ull << k))
+        return marginals_pass(s, mpos, a.opos, out);
     const bool serial = a.terms <= 64;
     a.chunk = 256 * 64;
     a.nchunks = serial ? 1 : (a.terms + a.chunk - 1) / a.chunk;

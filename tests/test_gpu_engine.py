@@ -185,6 +185,36 @@ def test_probabilities_and_marginals(trial):
     assert ms == rms and numpy.allclose(ps, rps, rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("dtype", [numpy.complex128, numpy.complex64])
+@pytest.mark.parametrize("n,idx", [
+    (12, [11, 10, 9]),                            # the lowest state bits
+    (13, [0, 1]),                                 # the highest
+    (18, [17, 3, 9, 0, 12]),                      # both sides of the 4096-amplitude chunk
+    (18, list(range(18))),                        # every qubit: 2^18 outcomes
+    (20, [19]), (20, [0]),
+    (20, [5, 19, 2, 14, 8, 11, 17, 0, 13, 6]),
+])
+def test_marginals_one_pass(n, idx, dtype):
+    """All-outcome marginals (one pass over the state from 12 qubits on) against the oracle in the
+    LSB-first order, and against the definition of the measurement order (output bit j = the j-th
+    lowest measured state bit) by bincount; the same bits on a second call."""
+    psi = rand_state(n)
+    be = gpu_from(psi, n, dtype)
+    v = be.get_state().astype(numpy.complex128)
+    got = be.measure_probabilities(idx)
+    assert numpy.max(numpy.abs(got - oracle.measure_probabilities(idx, n, v))) < 1e-13
+    pos = sorted(n - 1 - q for q in idx)
+    i = numpy.arange(2 ** n, dtype=numpy.int64)
+    m = numpy.zeros(2 ** n, dtype=numpy.int64)
+    for j, b in enumerate(pos):
+        m |= ((i >> b) & 1) << j
+    want = numpy.bincount(m, weights=numpy.abs(v) ** 2, minlength=2 ** len(idx))
+    marg = be.marginals(idx)
+    assert numpy.max(numpy.abs(marg - want)) < 1e-13
+    assert numpy.array_equal(marg, be.marginals(idx))
+    assert numpy.array_equal(got, be.measure_probabilities(idx))
+
+
 @pytest.mark.parametrize("trial", range(8))
 def test_measure_bit_exact_outcomes(trial):
     n = int(RNG.integers(3, 15))
