@@ -975,7 +975,20 @@ __global__ __launch_bounds__(256) void k_norm_partial(const cx<R> *__restrict__ 
     const uint64_t per = (len + gridDim.x - 1) / gridDim.x;
     const uint64_t lo = (uint64_t)blockIdx.x * per, hi = min(len, lo + per);
     double acc = 0.0;
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += 256) {
+    uint64_t i = lo + threadIdx.x;
+    // sixteen independent loads per step keep enough bytes in flight to fill HBM
+    for (; i + 256 * 15 < hi; i += 256 * 16) {
+        cx<R> y[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) y[j] = amps[i + 256 * j];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const double re = y[j].re, im = y[j].im;
+            acc = fma(re, re, acc);
+            acc = fma(im, im, acc);
+        }
+    }
+    for (; i < hi; i += 256) {
         const cx<R> v = amps[i];
         const double re = v.re, im = v.im;
         acc = fma(re, re, acc);
