@@ -29,7 +29,8 @@
 
 #include <algorithm>
 #include <cstdint>
-#include <set>
+#include <unordered_map>
+#include <utility>
 #include <vector>
 
 namespace qipgpu {
@@ -75,10 +76,12 @@ inline int64_t plan_blocks(const uint64_t *masks, int64_t count, int width, int 
         for (int64_t g = 0; g < count; ++g)
             for (int j = 0; j < npred[g]; ++j) succ[fill[pred[6 * g + j]]++] = g;
     }
-    // ready gates have pairwise disjoint qubits, so there are at most n of them
-    std::set<int64_t> ready;
+    // ready gates have pairwise disjoint qubits, so there are at most n of them: a sorted vector
+    std::vector<int64_t> ready;
     for (int64_t g = 0; g < count; ++g)
-        if (npred[g] == 0) ready.insert(g);
+        if (npred[g] == 0) ready.push_back(g);
+    auto ready_insert = [&](int64_t g) { ready.insert(std::lower_bound(ready.begin(), ready.end(), g), g); };
+    auto ready_erase = [&](int64_t g) { ready.erase(std::lower_bound(ready.begin(), ready.end(), g)); };
     std::vector<int64_t> work;
     // look-ahead scratch: a gate's pending predecessor count is sim_pred[g] when stamp[g] == cur
     std::vector<int32_t> sim_pred((size_t)count);
@@ -94,14 +97,14 @@ inline int64_t plan_blocks(const uint64_t *masks, int64_t count, int width, int 
         while (!work.empty()) {
             const int64_t g = work.back();
             work.pop_back();
-            ready.erase(g);
+            ready_erase(g);
             b.gates.push_back(g);
             ++c;
             for (uint64_t m = masks[g]; m; m &= m - 1) --open_on[__builtin_ctzll(m)];
             for (int64_t i = sstart[g]; i < sstart[g + 1]; ++i) {
                 const int64_t t = succ[i];
                 if (--npred[t] == 0) {
-                    ready.insert(t);
+                    ready_insert(t);
                     if ((masks[t] & ~S) == 0) work.push_back(t);
                 }
             }
@@ -112,7 +115,8 @@ inline int64_t plan_blocks(const uint64_t *masks, int64_t count, int width, int 
     // `left` (if given) receives the gates that would be ready outside S afterwards
     auto look_ahead = [&](uint64_t S, bool &kept, std::vector<int64_t> *left) {
         ++cur;
-        int64_t used[64] = {0};
+        int64_t used[64];
+        uint64_t touched = 0;     // qubits whose used[] count is live
         int64_t c = 0;
         work.clear();
         if (left) left->clear();
@@ -124,7 +128,14 @@ inline int64_t plan_blocks(const uint64_t *masks, int64_t count, int width, int 
             const int64_t g = work.back();
             work.pop_back();
             ++c;
-            for (uint64_t m = masks[g]; m; m &= m - 1) ++used[__builtin_ctzll(m)];
+            for (uint64_t m = masks[g]; m; m &= m - 1) {
+                const int q = __builtin_ctzll(m);
+                if (!(touched >> q & 1)) {
+                    touched |= 1ull << q;
+                    used[q] = 0;
+                }
+                ++used[q];
+            }
             for (int64_t i = sstart[g]; i < sstart[g + 1]; ++i) {
                 const int64_t t = succ[i];
                 if (stamp[t] != cur) {
@@ -140,8 +151,21 @@ inline int64_t plan_blocks(const uint64_t *masks, int64_t count, int width, int 
         kept = true;
         for (uint64_t m = S; m; m &= m - 1) {
             const int q = __builtin_ctzll(m);
-            kept = kept && open_on[q] - used[q] > 0;
+            kept = kept && open_on[q] - (touched >> q & 1 ? used[q] : 0) > 0;
         }
+        return c;
+    };
+    // look_ahead without `left`, memoised by S while the placed gates do not change (the second steps
+    // of different candidates often reach the same set)
+    std::unordered_map<uint64_t, std::pair<int64_t, bool>> memo;
+    auto look_ahead_memo = [&](uint64_t S, bool &kept) {
+        const auto it = memo.find(S);
+        if (it != memo.end()) {
+            kept = it->second.second;
+            return it->second.first;
+        }
+        const int64_t c = look_ahead(S, kept, nullptr);
+        memo.emplace(S, std::make_pair(c, kept));
         return c;
     };
     // Look-ahead score of growing S by gate g as the fraction num / den: the gates absorbed per qubit
@@ -152,12 +176,13 @@ inline int64_t plan_blocks(const uint64_t *masks, int64_t count, int width, int 
         const uint64_t S2 = S | masks[g];
         const int64_t a = __builtin_popcountll(S2) - __builtin_popcountll(S);
         const int64_t c = look_ahead(S2, kept, &left);
+        memo.emplace(S2, std::make_pair(c, kept));
         int64_t d = 0, a2 = 1;
         for (int64_t h : left) {
             const uint64_t S3 = S2 | masks[h];
             if (__builtin_popcountll(S3) > width) continue;
             bool kept3;
-            const int64_t c3 = look_ahead(S3, kept3, nullptr);
+            const int64_t c3 = look_ahead_memo(S3, kept3);
             kept = kept && kept3;
             const int64_t a3 = __builtin_popcountll(S3) - __builtin_popcountll(S2);
             if ((c3 - c) * a2 > d * a3) {
@@ -173,10 +198,11 @@ inline int64_t plan_blocks(const uint64_t *masks, int64_t count, int width, int 
     bool prefix_final = true;
     while (absorbed < count) {
         PlannedBlock b;
-        uint64_t S = masks[*ready.begin()];
+        uint64_t S = masks[ready.front()];
         bool exhausted = false;   // a look-ahead used up some qubit's recorded gates
         for (;;) {
             absorbed += absorb(S, b);
+            memo.clear();
             int64_t ga = -1, na = 0, da = 1;   // best of all: gate, score numerator, denominator
             int64_t gs = -1, ns = 0, ds = 1;   // best sharing a qubit with S
             for (int64_t g : ready) {

@@ -12,6 +12,7 @@
 // Every other entry point that touches the state first launches what is recorded (flush_recorded),
 // so recorded gates always act in program order with direct calls.
 #include <algorithm>
+#include <deque>
 #include <vector>
 
 #include "engine.h"
@@ -61,8 +62,22 @@ std::vector<char> product_flags(const std::vector<PlannedBlock> &blocks, const i
 
 }  // namespace
 
+// A plan depends only on the gates' qubit masks, the width, n and whether it stops at the first open
+// block, so programs that record the same pattern again (the layers of an iterative algorithm) reuse
+// it instead of planning again.
+struct PlanMemo {
+    std::vector<uint64_t> masks;
+    int width = 0, n = 0;
+    bool stop_at_open = false;
+    std::vector<PlannedBlock> blocks;
+    int64_t final_count = 0;
+};
+constexpr size_t kPlanMemos = 4;
+constexpr size_t kPlanMemoMaxGates = 4096;
+
 struct Recorder {
     int n = 0, width = 6;
+    std::deque<PlanMemo> memos;       // most recent first
     // recorded gates not launched yet, in program order
     std::vector<int32_t> k, q;        // width and qubits (six slots) per gate
     std::vector<uint64_t> masks;      // qubit mask per gate
@@ -88,6 +103,32 @@ struct Recorder {
 };
 
 namespace {
+
+// plan_blocks over the recorded gates, through the recorder's memo of recent plans
+int64_t plan_recorded(Recorder &r, bool stop_at_open, std::vector<PlannedBlock> &blocks) {
+    for (size_t i = 0; i < r.memos.size(); ++i) {
+        const PlanMemo &m = r.memos[i];
+        if (m.width == r.width && m.n == r.n && m.stop_at_open == stop_at_open && m.masks == r.masks) {
+            blocks = m.blocks;
+            const int64_t fc = m.final_count;
+            if (i) std::rotate(r.memos.begin(), r.memos.begin() + (long)i, r.memos.begin() + (long)i + 1);
+            return fc;
+        }
+    }
+    const int64_t fc = plan_blocks(r.masks.data(), r.pending(), r.width, r.n, stop_at_open, blocks);
+    if (r.masks.size() <= kPlanMemoMaxGates) {
+        if (r.memos.size() == kPlanMemos) r.memos.pop_back();
+        PlanMemo m;
+        m.masks = r.masks;
+        m.width = r.width;
+        m.n = r.n;
+        m.stop_at_open = stop_at_open;
+        m.blocks = blocks;
+        m.final_count = fc;
+        r.memos.push_front(std::move(m));
+    }
+    return fc;
+}
 
 int validate_gate(const qipgpu_state *s, int width, int32_t g, int32_t k, const int32_t *q) {
     if (k < 1 || k > width)
@@ -241,7 +282,7 @@ int launch_final(qipgpu_state *s) {
     Recorder *r = s->recorder;
     r->since_plan = 0;
     std::vector<PlannedBlock> blocks;
-    const int64_t fc = plan_blocks(r->masks.data(), r->pending(), r->width, r->n, true, blocks);
+    const int64_t fc = plan_recorded(*r, true, blocks);
     const std::vector<char> product = product_flags(blocks, r->k.data());
     return launch_planned(s, blocks, launchable_count(product, fc, product_runs(s)));
 }
@@ -343,7 +384,7 @@ int flush_recorded(qipgpu_state *s) {
     Recorder *r = s->recorder;
     if (!r || r->in_flush || r->k.empty()) return QIPGPU_OK;
     std::vector<PlannedBlock> blocks;
-    plan_blocks(r->masks.data(), r->pending(), r->width, r->n, false, blocks);
+    plan_recorded(*r, false, blocks);
     r->since_plan = 0;
     return launch_planned(s, blocks, (int64_t)blocks.size());
 }

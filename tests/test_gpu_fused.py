@@ -221,6 +221,25 @@ def test_recorder_random_gates(n, seed):
     assert numpy.max(numpy.abs(exact.get_state() - outs[0])) < 1e-5
 
 
+def test_recorder_repeated_pattern():
+    """The same gate pattern recorded again (new matrices, a flush after each repetition, as in an
+    iterative algorithm) is planned from the recorder's memo of recent plans: bitwise the blocks of
+    the Python plan of each repetition."""
+    from qip_amd.fused import FusedBackend, descriptor_gates, fuse
+    from qip_amd.workloads import layered_random_circuit
+    n = 15
+    psi = rand_state(n)
+    fz = FusedBackend.make_state(n, [list(range(n))], [psi])
+    ref = GpuBackend.make_state(n, [list(range(n))], [psi], statetype=numpy.complex64)
+    for rep in range(3):
+        ops = layered_random_circuit(n, 6, seed=40 + rep)      # same pattern, new unitaries
+        for d in ops:
+            fz.kronselect_dot(d)
+        fz.flush()
+        _apply_python_plan(ref, n, fuse([g for d in ops for g in descriptor_gates(d)], 6))
+    assert numpy.array_equal(fz.get_state(), ref.get_state())
+
+
 def test_single_qubit_runs_take_the_product_sweep():
     """Consecutive blocks of single-qubit gates only are one product of per-qubit 2x2s applied with
     FP64 arithmetic: the same bits as the Python restatement, and (on random single-qubit layers)
