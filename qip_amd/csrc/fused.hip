@@ -97,10 +97,14 @@ __global__ __launch_bounds__(256, 2) void k_fused6(cx<float> *amps, const bf16x8
     if (tile < end) {
         const uint64_t last = end - 1;
         for (;;) {
+            __builtin_amdgcn_s_setprio(2);
             load_tile(vb, amps, tile_base(min(tile + 1, last), a) + off_t, a);
+            __builtin_amdgcn_s_setprio(0);
             process_tile(va, tile_base(tile, a) + off_t, amps, lds, ah, al, ctx, a, nacc);
             if (++tile >= end) break;
+            __builtin_amdgcn_s_setprio(2);
             load_tile(va, amps, tile_base(min(tile + 1, last), a) + off_t, a);
+            __builtin_amdgcn_s_setprio(0);
             process_tile(vb, tile_base(tile, a) + off_t, amps, lds, ah, al, ctx, a, nacc);
             if (++tile >= end) break;
         }
