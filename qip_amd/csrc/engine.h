@@ -105,6 +105,8 @@ int set_error(int code, const char *fmt, ...);
 
 // Make sure s->scratch holds at least `bytes`.
 int ensure_scratch(qipgpu_state *s, size_t bytes);
+// Compute units of the current device, queried once per device (persistent grids are sized by it).
+int device_cus();
 // Validate a list of qubit indices against n (range, distinctness); returns QIPGPU_OK or EINVAL.
 int check_qubits(const qipgpu_state *s, int32_t k, const int32_t *q, const char *what);
 // Reserve a staging ring slot, copy `bytes` from `src` to the device, return device pointer.

@@ -7,6 +7,7 @@
 // state bits are free. Reductions accumulate in FP64 and combine partials in a fixed order, so
 // results do not depend on scheduling.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -30,6 +31,22 @@ int set_error(int code, const char *fmt, ...) {
     va_end(ap);
     g_last_error = buf;
     return code;
+}
+
+int device_cus() {
+    static std::atomic<int> cached[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        return 256;
+    }
+    int cus = cached[dev].load(std::memory_order_relaxed);
+    if (cus > 0) return cus;
+    hipDeviceProp_t p;
+    cus = (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0) ? p.multiProcessorCount : 256;
+    (void)hipGetLastError();
+    cached[dev].store(cus, std::memory_order_relaxed);
+    return cus;
 }
 
 int ensure_scratch(qipgpu_state *s, size_t bytes) {
@@ -832,7 +849,18 @@ int qipgpu_apply_diagonal(qipgpu_state *s, int32_t k, const int32_t *targets, co
 struct qipgpu_table {
     int32_t n_reg1;
     int32_t *d;
+    uint64_t nonzero;   // entries f(x) != 0
 };
+
+namespace {
+
+uint64_t count_nonzero(const int32_t *table, uint64_t count) {
+    uint64_t nz = 0;
+    for (uint64_t i = 0; i < count; ++i) nz += table[i] != 0;
+    return nz;
+}
+
+}  // namespace
 
 // For x with y = f(x) & (2^n2 - 1) != 0 the map q -> q xor y pairs the reg2 values; each thread
 // owns one pair (q with bit hb(y) clear, q xor y) for one rest value r and swaps the two
@@ -866,10 +894,108 @@ __global__ __launch_bounds__(256) void k_func_apply(cx<R> *amps, const int32_t *
     }
 }
 
+// The same permutation tile by tile (n >= 12, reg2 of at most 8 qubits, reg1 of at most 31). A tile is the 4096
+// amplitudes spanned by the reg2 state bits (its rows, q) and the lowest other state bits (its
+// columns); each column has one x, so its rows move by q -> q xor y(x). The block reads the y of
+// every column; a tile where all of them are 0 is left untouched, else it is read in address order
+// into LDS and written back in address order with out[q, c] = in[q xor y(c), c]. Every amplitude of a
+// moved tile is read and written once, in runs of at least 2^(lowest non-reg2 bits) amplitudes.
+constexpr int kFuncTileBits = 12;
+
+struct FuncTileArgs {
+    uint64_t ntiles;
+    uint32_t ymask;
+    int32_t n1, n2, ncol_bits;             // ncol_bits = 12 - n2
+    uint8_t pbits[kFuncTileBits];          // u bit i <-> state bit pbits[i], ascending
+    uint8_t ru[8];                         // u bit of reg2 value bit b
+    uint8_t cu[kFuncTileBits];             // u bit of column bit b, ascending
+    uint8_t xpos[QIPGPU_MAX_QUBITS];       // reg1 value bit b -> state bit
+};
+
+__device__ __forceinline__ uint64_t func_tile_base(uint64_t t, const FuncTileArgs &a) {
+#pragma unroll
+    for (int i = 0; i < kFuncTileBits; ++i) {
+        const uint32_t p = a.pbits[i];
+        const uint64_t lo = t & ((1ull << p) - 1ull);
+        t = ((t ^ lo) << 1) | lo;
+    }
+    return t;
+}
+
+template <int LO, int HI>
+__device__ __forceinline__ uint64_t func_u_offset(uint32_t u, const FuncTileArgs &a) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int i = LO; i < HI; ++i) o |= (uint64_t)((u >> i) & 1u) << a.pbits[i];
+    return o;
+}
+
+// column index of tile position u (its bits at the column positions, in order)
+__device__ __forceinline__ uint32_t func_col(uint32_t u, const FuncTileArgs &a) {
+    uint32_t c = 0;
+    for (int b = 0; b < a.ncol_bits; ++b) c |= ((u >> a.cu[b]) & 1u) << b;
+    return c;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_func_tile(cx<R> *amps, const int32_t *__restrict__ table, FuncTileArgs a) {
+    __shared__ cx<R> t[1 << kFuncTileBits];
+    __shared__ uint32_t xcol[1 << (kFuncTileBits - 1)];   // the x bits of each column (at most 2048)
+    __shared__ uint8_t ys[1 << (kFuncTileBits - 1)];      // y of each column of the tile (< 2^8)
+    const uint32_t tid = threadIdx.x;
+    const uint32_t ncol = 1u << a.ncol_bits;
+    // x = (x bits of the tile) | (x bits of the column): the column part once per kernel
+    for (uint32_t c = tid; c < ncol; c += 256) {
+        uint32_t u = 0;
+        for (int b = 0; b < a.ncol_bits; ++b) u |= ((c >> b) & 1u) << a.cu[b];
+        const uint64_t st = func_u_offset<0, kFuncTileBits>(u, a);
+        uint32_t x = 0;
+        for (int b = 0; b < a.n1; ++b) x |= (uint32_t)((st >> a.xpos[b]) & 1ull) << b;
+        xcol[c] = x;
+    }
+    const uint64_t off_t = func_u_offset<0, 8>(tid, a);
+    const uint32_t col_t = func_col(tid, a);            // column bits among u bits 0..7
+    const uint64_t per = (a.ntiles + gridDim.x - 1) / gridDim.x;
+    uint64_t tile = (uint64_t)blockIdx.x * per;
+    const uint64_t end = min(tile + per, a.ntiles);
+    __syncthreads();
+    for (; tile < end; ++tile) {
+        const uint64_t tb = func_tile_base(tile, a);
+        uint32_t xt = 0;
+        for (int b = 0; b < a.n1; ++b) xt |= (uint32_t)((tb >> a.xpos[b]) & 1ull) << b;
+        int any = 0;
+        for (uint32_t c = tid; c < ncol; c += 256) {
+            const uint32_t y = (uint32_t)table[xt | xcol[c]] & a.ymask;
+            ys[c] = (uint8_t)y;
+            any |= y != 0u;
+        }
+        if (!__syncthreads_or(any)) continue;   // nothing moves in this tile
+        cx<R> v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = amps[tb + off_t + func_u_offset<8, kFuncTileBits>(256u * j, a)];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) t[tid + 256u * j] = v[j];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const uint32_t u = tid + 256u * j;
+            const uint32_t y = ys[col_t | func_col(256u * j, a)];   // < 2^n2
+            uint32_t ry = 0;
+            for (int b = 0; b < a.n2; ++b) ry |= ((y >> b) & 1u) << a.ru[b];
+            v[j] = t[u ^ ry];
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) amps[tb + off_t + func_u_offset<8, kFuncTileBits>(256u * j, a)] = v[j];
+        __syncthreads();   // t and ys are rewritten for the next tile
+    }
+}
+
 namespace {
 
+// `nonzero` = entries of the table that are not 0: where fewer than one x in 64 moves anything, the
+// per-pair kernel (which touches only the moved pairs) runs instead of the tiles.
 static int func_apply_dev(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, int32_t n_reg2, const int32_t *reg2,
-                          const int32_t *d_table) {
+                          const int32_t *d_table, uint64_t nonzero) {
     const int n = s->n;
     if (n_reg1 < 1 || n_reg2 < 1 || n_reg1 + n_reg2 > n)
         return set_error(QIPGPU_EINVAL, "func_apply: register sizes %d + %d for n=%d", n_reg1, n_reg2, n);
@@ -899,6 +1025,37 @@ static int func_apply_dev(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, 
     for (int q = 0; q < n; ++q)
         if (!(used & (1ull << q))) other.push_back(q);
     for (int b = 0; b < a.nother; ++b) a.rpos[b] = (uint8_t)(n - 1 - other[a.nother - 1 - b]);
+    if (n >= kFuncTileBits && n_reg2 <= 8 && n_reg1 <= 31 && nonzero * 64 >= (1ull << n_reg1)) {
+        FuncTileArgs f;
+        std::memset(&f, 0, sizeof(f));
+        f.ntiles = 1ull << (n - kFuncTileBits);
+        f.ymask = a.ymask;
+        f.n1 = n_reg1;
+        f.n2 = n_reg2;
+        f.ncol_bits = kFuncTileBits - n_reg2;
+        std::memcpy(f.xpos, a.xpos, sizeof(f.xpos));
+        // tile bits: the reg2 bits and the lowest other bits
+        uint64_t rmask = 0;
+        for (int b = 0; b < n_reg2; ++b) rmask |= 1ull << a.qpos[b];
+        std::vector<int> pb;
+        for (int b = 0; b < n_reg2; ++b) pb.push_back(a.qpos[b]);
+        for (int b = 0; b < n && (int)pb.size() < kFuncTileBits; ++b)
+            if (!(rmask >> b & 1)) pb.push_back(b);
+        std::sort(pb.begin(), pb.end());
+        for (int i = 0; i < kFuncTileBits; ++i) f.pbits[i] = (uint8_t)pb[i];
+        for (int b = 0; b < n_reg2; ++b)
+            f.ru[b] = (uint8_t)(std::find(pb.begin(), pb.end(), (int)a.qpos[b]) - pb.begin());
+        int nc = 0;
+        for (int i = 0; i < kFuncTileBits; ++i)
+            if (!(rmask >> pb[i] & 1)) f.cu[nc++] = (uint8_t)i;
+        const uint32_t nb = (uint32_t)std::min<uint64_t>(f.ntiles, (uint64_t)device_cus() * 2);
+        if (s->dtype == QIPGPU_C128)
+            hipLaunchKernelGGL(k_func_tile<double>, dim3(nb), dim3(256), 0, s->stream, (cx<double> *)s->amps, d_table, f);
+        else
+            hipLaunchKernelGGL(k_func_tile<float>, dim3(nb), dim3(256), 0, s->stream, (cx<float> *)s->amps, d_table, f);
+        QIP_LAUNCHED();
+        return QIPGPU_OK;
+    }
     const uint32_t nb = capped_grid(blocks_for(a.nthreads, 256));
     if (s->dtype == QIPGPU_C128)
         hipLaunchKernelGGL(k_func_apply<double>, dim3(nb), dim3(256), 0, s->stream, (cx<double> *)s->amps, d_table, a);
@@ -917,6 +1074,7 @@ int qipgpu_table_create(int32_t n_reg1, const int32_t *table, qipgpu_table **out
     if (n_reg1 < 1 || n_reg1 > QIPGPU_MAX_QUBITS) return set_error(QIPGPU_EINVAL, "table for %d-qubit register", n_reg1);
     qipgpu_table *t = new qipgpu_table();
     t->n_reg1 = n_reg1;
+    t->nonzero = count_nonzero(table, 1ull << n_reg1);
     const size_t bytes = sizeof(int32_t) << n_reg1;
     if (hipMalloc(&t->d, bytes) != hipSuccess) {
         (void)hipGetLastError();
@@ -944,7 +1102,7 @@ int qipgpu_func_apply_table(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1
                             const qipgpu_table *t) {
     if (!t || t->n_reg1 != n_reg1)
         return set_error(QIPGPU_EINVAL, "function table is for a %d-qubit register, not %d", t ? t->n_reg1 : -1, n_reg1);
-    return func_apply_dev(s, n_reg1, reg1, n_reg2, reg2, t->d);
+    return func_apply_dev(s, n_reg1, reg1, n_reg2, reg2, t->d, t->nonzero);
 }
 
 int qipgpu_func_apply(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, int32_t n_reg2, const int32_t *reg2,
@@ -955,7 +1113,8 @@ int qipgpu_func_apply(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, int3
     if (rc) return rc;
     QIP_HIP(hipStreamSynchronize(s->stream));
     QIP_HIP(hipMemcpy(s->scratch, table, bytes, hipMemcpyHostToDevice));
-    rc = func_apply_dev(s, n_reg1, reg1, n_reg2, reg2, (const int32_t *)s->scratch);
+    rc = func_apply_dev(s, n_reg1, reg1, n_reg2, reg2, (const int32_t *)s->scratch,
+                        count_nonzero(table, 1ull << n_reg1));
     if (rc) return rc;
     QIP_HIP(hipStreamSynchronize(s->stream));  // scratch may be reused by the next call
     return QIPGPU_OK;

@@ -386,13 +386,7 @@ int compose_base(int32_t k, const int32_t *qubits, int32_t n_gates, const int32_
 }
 
 uint32_t fused_grid(uint64_t ntiles) {
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
-        hipDeviceProp_t p;
-        if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0) cus = p.multiProcessorCount;
-    }
-    (void)hipGetLastError();
-    return (uint32_t)std::min<uint64_t>(ntiles, (uint64_t)cus * kFusedBlocksPerCU);
+    return (uint32_t)std::min<uint64_t>(ntiles, (uint64_t)device_cus() * kFusedBlocksPerCU);
 }
 
 }  // namespace

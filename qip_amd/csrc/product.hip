@@ -181,13 +181,7 @@ __global__ __launch_bounds__(256, 2) void k_product(cx<R> *amps, ProductArgs a) 
 }
 
 uint32_t product_grid(uint64_t ntiles) {
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
-        hipDeviceProp_t p;
-        if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0) cus = p.multiProcessorCount;
-    }
-    (void)hipGetLastError();
-    return (uint32_t)std::min<uint64_t>(ntiles, (uint64_t)cus * kBlocksPerCU);
+    return (uint32_t)std::min<uint64_t>(ntiles, (uint64_t)device_cus() * kBlocksPerCU);
 }
 
 constexpr int kLowBits = kProductLowBits;           // state bits every pass keeps in its tile

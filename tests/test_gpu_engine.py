@@ -155,6 +155,39 @@ def test_func_apply_bit_exact(trial):
     assert numpy.array_equal(be.get_state(), ref)
 
 
+@pytest.mark.parametrize("dtype", [numpy.complex128, numpy.complex64])
+@pytest.mark.parametrize("n,reg1,reg2,kind", [
+    (12, list(range(4)), list(range(4, 12)), "dense"),           # reg2 on the 8 lowest state bits
+    (14, list(range(6, 14)), list(range(6)), "dense"),           # reg2 high, reg1 low
+    (14, [3, 9, 0, 12], [7, 1, 13], "dense"),                    # scattered, 7 rest qubits
+    (16, list(range(15)), [15], "sparse"),                       # Grover-style: one x moves
+    (16, list(range(15)), [15], "block"),                        # f != 0 on the first 1/16 of x only
+    (14, list(range(6, 14)), list(range(6)), "block"),
+    (16, [2, 5, 11, 14, 8], [0, 9, 4, 15, 1, 12, 6, 3], "dense"),
+    (13, [0, 1, 2], [5, 6, 7, 8, 9, 10, 11, 12, 3], "dense"),    # reg2 of 9 qubits: per-pair kernel
+])
+def test_func_apply_tiles(n, reg1, reg2, kind, dtype):
+    """func_apply from 12 qubits on with reg2 of up to 8 qubits (tiles of 4096 amplitudes permuted in
+    LDS, untouched where f is 0 on the whole tile; the per-pair kernel where fewer than one x in 64
+    moves): bit-exact against the oracle for registers on low, high and scattered state bits, with
+    other qubits beside them, for a table that moves one x only and one that is 0 on most tiles."""
+    psi = rand_state(n)
+    be = gpu_from(psi, n, dtype)
+    v = be.get_state().astype(numpy.complex128)
+    if kind == "dense":
+        table = RNG.integers(0, 2 ** 31 - 1, size=2 ** len(reg1)).astype(numpy.int32)
+    elif kind == "block":
+        x = numpy.arange(2 ** len(reg1))
+        table = numpy.where(x < 2 ** len(reg1) // 16, x % 7 + 1, 0).astype(numpy.int32)
+    else:
+        table = numpy.zeros(2 ** len(reg1), dtype=numpy.int32)
+        table[int(RNG.integers(0, 2 ** len(reg1)))] = 1
+    be.func_apply(numpy.array(reg1, dtype=numpy.int32), numpy.array(reg2, dtype=numpy.int32), table)
+    ref = numpy.zeros_like(v)
+    oracle.func_apply(reg1, reg2, None, v, n, ref, faithful=False, table=table)
+    assert numpy.array_equal(be.get_state().astype(numpy.complex128), ref)
+
+
 def test_func_apply_callable_and_cache():
     n = 6
     psi = rand_state(n)
