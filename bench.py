@@ -24,7 +24,8 @@ Beside it, in the same run:
 Other configurations, one JSON line each (--workload): circuit28 (28-qubit depth-20 circuit, plain
 complex128 vs unfused complex64), grover24 (24-qubit Grover search in both forms), qft16 (16-qubit
 QFT graph), batch512 (512 uneven circuits in the batching form), measure28 (marginal probabilities,
-norm and measurement on a 28-qubit complex128 state).
+norm and measurement on a 28-qubit complex128 state), func28 (func_apply on a 28-qubit complex128
+state).
 
 --no-cpu-baseline leaves out the CPU leg (for profiler runs); it does not change the timed work.
 """
@@ -471,6 +472,68 @@ def bench_measure28(args):
     return out
 
 
+def bench_func28(args):
+    """func_apply on a 28-qubit complex128 state (4 GiB; H on every qubit): f(x) = x mod 251 (dense:
+    almost every amplitude moves) with reg1 on the 20 highest state bits and reg2 on the 8 lowest,
+    the reverse, and scattered registers; and a Grover-style oracle (reg1 = 27 qubits, f = [x == x*]:
+    one pair moves). Algorithmic bytes per call as defined for func_apply: 2 S 2^n + 4 2^n1 (no
+    credit for amplitudes that do not move)."""
+    import numpy
+    from qip_amd import GpuBackend
+    from qip_amd.backend import FuncTable
+    n, amp = 28, 16
+    H = numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2)
+    be = GpuBackend.make_state(n, [], [])
+    be.kronselect_dot({q: H for q in range(n)})
+    be.synchronize()
+    ev = HipEvents()
+    reps = max(3, args.steps)
+    cases = {"reg1_high20_reg2_low8": (list(range(20)), list(range(20, 28)), "mod"),
+             "reg1_low20_reg2_high8": (list(range(8, 28)), list(range(8)), "mod"),
+             "scattered": (list(range(4, 24)), list(range(0, 4)) + list(range(24, 28)), "mod"),
+             "grover_oracle_27_1": (list(range(27)), [27], "one")}
+    out_cases = {}
+    for name, (r1, r2, kind) in cases.items():
+        x = numpy.arange(2 ** len(r1), dtype=numpy.int64)
+        vals = x % 251 if kind == "mod" else (x == 12345).astype(numpy.int64)
+        tab = FuncTable(vals.astype(numpy.int32))
+        be.func_apply(r1, r2, tab)
+        be.synchronize()
+        t0, t1 = ev.create(), ev.create()
+        ev.record(t0, be.stream)
+        for _ in range(reps):
+            be.func_apply(r1, r2, tab)
+        ev.record(t1, be.stream)
+        be.synchronize()
+        ms = ev.elapsed_ms(t0, t1) / reps
+        b = 2 * amp * 2 ** n + 4 * 2 ** len(r1)
+        out_cases[name] = {"reg1": r1, "reg2": r2, "ms": ms, "algorithmic_gbs": b / ms / 1e6,
+                           "hbm_frac": b / ms / 1e6 / HBM_PEAK_GBS}
+    dense = [out_cases[c] for c in ("reg1_high20_reg2_low8", "reg1_low20_reg2_high8", "scattered")]
+    value = sum(c["algorithmic_gbs"] for c in dense) / 3
+    cfg = {"workload": "func_apply on a 28-qubit complex128 state: f(x) = x mod 251 with three register layouts "
+                       "(value: their mean), and a Grover-style oracle on 27 + 1 qubits", "n_qubits": n}
+    out = _line("28-qubit c128 func_apply (dense f): algorithmic GB/s", value, "GB/s", reps, 1,
+                sum(c["ms"] for c in dense) / 3, "c128", cfg, cases=out_cases,
+                roofline={"bound": "hbm", "kernel": "k_func_tile (LDS-tiled XOR permutation)", "achieved": value,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": value / HBM_PEAK_GBS, "traffic": None})
+    if not args.no_cpu_baseline:
+        from oracle import oracle
+        r1, r2, _ = cases["reg1_high20_reg2_low8"]
+        v = be.get_state()
+        o = numpy.zeros_like(v)
+        tabv = (numpy.arange(2 ** 20) % 251).astype(numpy.int32)
+        t = time.perf_counter()
+        oracle.func_apply(r1, r2, None, v, n, o, faithful=False, table=tabv, fast=True)
+        cdt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": (2 * amp * 2 ** n + 4 * 2 ** 20) / cdt / 1e9, "unit": "GB/s",
+                               "cores": len(os.sched_getaffinity(0)), "kind": "port",
+                               "sample": "one full-size call of the oracle's func_apply restatement "
+                                         "(func_apply.pyx:11-112, -O3 -fopenmp build, all cores) on the first "
+                                         "layout, {:.2f} s".format(cdt)}
+    return out
+
+
 def bench_batch512(args):
     """512 independent layered random circuits, n ~ U{10..22}, depth ~ U{8..32} (seeded), complex64:
     the batching form (one captured graph per run, autotuned) against the plain complex64 path with
@@ -607,7 +670,7 @@ def main():
                     help="after the benchmark, one device-to-device copy of this many GiB (a kernel of known "
                          "traffic for calibrating hardware byte counters)")
     ap.add_argument("--workload", default="circuit30",
-                    choices=["circuit30", "circuit28", "grover24", "qft16", "batch512", "measure28"],
+                    choices=["circuit30", "circuit28", "grover24", "qft16", "batch512", "measure28", "func28"],
                     help="circuit30 (default, the headline) or one of the other configurations, each printed as "
                          "its own JSON line")
     args = ap.parse_args()
@@ -617,7 +680,8 @@ def main():
         sys.exit(2)
     if args.workload != "circuit30":
         result = {"circuit28": bench_circuit28, "grover24": bench_grover24, "qft16": bench_qft16,
-                  "batch512": bench_batch512, "measure28": bench_measure28}[args.workload](args)
+                  "batch512": bench_batch512, "measure28": bench_measure28,
+                  "func28": bench_func28}[args.workload](args)
         print(json.dumps(result), flush=True)
         if args.copy_calibration_gib > 0:
             copy_calibration(HipEvents(), int(args.copy_calibration_gib * 2 ** 30))
