@@ -942,8 +942,14 @@ __global__ __launch_bounds__(256) void k_func_tile(cx<R> *amps, const int32_t *_
     __shared__ cx<R> t[1 << kFuncTileBits];
     __shared__ uint32_t xcol[1 << (kFuncTileBits - 1)];   // the x bits of each column (at most 2048)
     __shared__ uint8_t ys[1 << (kFuncTileBits - 1)];      // y of each column of the tile (< 2^8)
+    __shared__ uint16_t rowx[256];                        // y -> its bits on the row positions of u
     const uint32_t tid = threadIdx.x;
     const uint32_t ncol = 1u << a.ncol_bits;
+    {
+        uint32_t ry = 0;
+        for (int b = 0; b < a.n2; ++b) ry |= ((tid >> b) & 1u) << a.ru[b];
+        rowx[tid] = (uint16_t)ry;
+    }
     // x = (x bits of the tile) | (x bits of the column): the column part once per kernel
     for (uint32_t c = tid; c < ncol; c += 256) {
         uint32_t u = 0;
@@ -955,6 +961,9 @@ __global__ __launch_bounds__(256) void k_func_tile(cx<R> *amps, const int32_t *_
     }
     const uint64_t off_t = func_u_offset<0, 8>(tid, a);
     const uint32_t col_t = func_col(tid, a);            // column bits among u bits 0..7
+    uint32_t col_j[16];                                 // column bits among u bits 8..11, per j
+#pragma unroll
+    for (int j = 0; j < 16; ++j) col_j[j] = col_t | func_col(256u * j, a);
     const uint64_t per = (a.ntiles + gridDim.x - 1) / gridDim.x;
     uint64_t tile = (uint64_t)blockIdx.x * per;
     const uint64_t end = min(tile + per, a.ntiles);
@@ -977,13 +986,7 @@ __global__ __launch_bounds__(256) void k_func_tile(cx<R> *amps, const int32_t *_
         for (int j = 0; j < 16; ++j) t[tid + 256u * j] = v[j];
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const uint32_t u = tid + 256u * j;
-            const uint32_t y = ys[col_t | func_col(256u * j, a)];   // < 2^n2
-            uint32_t ry = 0;
-            for (int b = 0; b < a.n2; ++b) ry |= ((y >> b) & 1u) << a.ru[b];
-            v[j] = t[u ^ ry];
-        }
+        for (int j = 0; j < 16; ++j) v[j] = t[(tid + 256u * j) ^ rowx[ys[col_j[j]]]];
 #pragma unroll
         for (int j = 0; j < 16; ++j) amps[tb + off_t + func_u_offset<8, kFuncTileBits>(256u * j, a)] = v[j];
         __syncthreads();   // t and ys are rewritten for the next tile
