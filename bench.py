@@ -399,9 +399,9 @@ def bench_measure28(args):
     """Measurement reductions on a 28-qubit complex128 state (4 GiB; H then one Haar unitary on every
     qubit): marginal probabilities over 8 high, 8 low and 8 scattered qubits, 12 low, one high and one
     low qubit (measure_probabilities, one pass over the state each); prob_magnitude (total_prob); and
-    measure on the 8 scattered qubits (norm, marginals, collapse). Algorithmic bytes per call as
-    defined for these functions: S 2^n + 8 2^k for the probabilities, S 2^n for the norm, 4 S 2^n for
-    measure."""
+    measure and then reduce_measure on the 8 scattered qubits (norm, marginals, collapse or tiled
+    reduction). Algorithmic bytes per call as defined for these functions: S 2^n + 8 2^k for the
+    probabilities, S 2^n for the norm, 4 S 2^n for measure and reduce_measure."""
     import random
     import numpy
     from qip_amd import GpuBackend
@@ -444,14 +444,19 @@ def bench_measure28(args):
     ms, wall = timed(lambda: be.measure(cases["scattered8"]))
     meas = {"qubits": cases["scattered8"], "ms": ms, "wall_ms": wall, "algorithmic_gbs": 4 * state_bytes / ms / 1e6,
             "hbm_frac": 4 * state_bytes / ms / 1e6 / HBM_PEAK_GBS}
+    # reduce_measure last (it rewrites the state with the reduced one, tiled over the arena)
+    ms, wall = timed(lambda: be.reduce_measure(cases["scattered8"]))
+    red = {"qubits": cases["scattered8"], "ms": ms, "wall_ms": wall, "algorithmic_gbs": 4 * state_bytes / ms / 1e6,
+           "hbm_frac": 4 * state_bytes / ms / 1e6 / HBM_PEAK_GBS}
     eight = [probs[c] for c in ("high8", "low8", "scattered8")]
     value = sum(c["algorithmic_gbs"] for c in eight) / 3
     avg_ms = sum(c["ms"] for c in eight) / 3
     cfg = {"workload": "28-qubit complex128 state (H, then a Haar unitary on every qubit): measure_probabilities "
                        "over 8 high / 8 low / 8 scattered qubits (value: their mean), 12 low, 1 high, 1 low; "
-                       "total_prob; measure on 8 scattered qubits", "n_qubits": n, "seed": args.seed}
+                       "total_prob; measure and reduce_measure on 8 scattered qubits", "n_qubits": n,
+           "seed": args.seed}
     out = _line("28-qubit c128 marginal probabilities over 8 qubits: algorithmic GB/s", value, "GB/s", reps, 1,
-                avg_ms, "f64", cfg, probabilities=probs, total_prob=norm, measure=meas,
+                avg_ms, "f64", cfg, probabilities=probs, total_prob=norm, measure=meas, reduce_measure=red,
                 roofline={"bound": "hbm", "kernel": "k_marg_pass (one pass over the state, FP64 sums)",
                           "achieved": value, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": value / HBM_PEAK_GBS,
                           "traffic": int((MARG_PMC_28Q["fetch_kib"] * MARG_PMC_28Q["fetch_scale"] +

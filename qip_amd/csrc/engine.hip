@@ -1500,17 +1500,28 @@ __global__ __launch_bounds__(256) void k_collapse(cx<R> *amps, uint64_t len, uin
 
 // reduce_measure (kronprod.pyx:482-489): out[i] = in[measured bits m, unmeasured bits = low
 // n-k bits of i] * mult, out of place.
+// In place, in two steps: k_reduce_gather copies the kept slice, scaled (slice[j] = amps[measured
+// bits | j with zeros inserted at the measured positions] * mult), into a compact buffer of 2^(n-k)
+// amplitudes; k_reduce_tile then writes amps[i] = slice[i mod 2^(n-k)] over the whole state in address
+// order. Peak memory is the state plus the slice, not two states.
 template <typename R>
-__global__ __launch_bounds__(256) void k_reduce_tiled(const cx<R> *__restrict__ in, cx<R> *out, uint64_t len, uint64_t mbits,
-                                                      PosList mpl, uint64_t lowmask, double mult) {
+__global__ __launch_bounds__(256) void k_reduce_gather(const cx<R> *__restrict__ amps, cx<R> *slice, uint64_t count,
+                                                       uint64_t mbits, PosList mpl, double mult) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
-        const cx<R> v = in[mbits | insert_zeros(i & lowmask, mpl)];
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count; j += stride) {
+        const cx<R> v = amps[mbits | insert_zeros(j, mpl)];
         cx<R> o;
         o.re = (R)((double)v.re * mult);
         o.im = (R)((double)v.im * mult);
-        out[i] = o;
+        slice[j] = o;
     }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_reduce_tile(cx<R> *amps, const cx<R> *__restrict__ slice, uint64_t len,
+                                                     uint64_t lowmask) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) amps[i] = slice[i & lowmask];
 }
 
 namespace {
@@ -1673,23 +1684,29 @@ int qipgpu_reduce_measure(qipgpu_state *s, int32_t k, const int32_t *indices, do
     for (int j = 0; j < k; ++j) mpos.push_back(s->n - 1 - indices[j]);
     PosList mpl;
     fill_poslist(mpl, mpos);
-    void *out = nullptr;
-    if (hipMalloc(&out, s->len * s->amp_bytes) != hipSuccess) {
+    const uint64_t count = 1ull << (s->n - k);
+    void *slice = nullptr;
+    if (hipMalloc(&slice, count * s->amp_bytes) != hipSuccess) {
         (void)hipGetLastError();
-        return set_error(QIPGPU_ENOMEM, "reduce_measure: cannot allocate the output arena");
+        return set_error(QIPGPU_ENOMEM, "reduce_measure: cannot allocate the %llu-amplitude slice",
+                         (unsigned long long)count);
     }
-    const uint64_t lowmask = (1ull << (s->n - k)) - 1ull;
-    const uint32_t nb = std::min<uint32_t>(blocks_for(s->len, 256 * 4), 1u << 20);
-    if (s->dtype == QIPGPU_C128)
-        hipLaunchKernelGGL(k_reduce_tiled<double>, dim3(nb), dim3(256), 0, s->stream, (const cx<double> *)s->amps,
-                           (cx<double> *)out, s->len, mbits, mpl, lowmask, mult);
-    else
-        hipLaunchKernelGGL(k_reduce_tiled<float>, dim3(nb), dim3(256), 0, s->stream, (const cx<float> *)s->amps,
-                           (cx<float> *)out, s->len, mbits, mpl, lowmask, mult);
+    const uint32_t ng = std::min<uint32_t>(blocks_for(count, 256 * 4), 1u << 20);
+    const uint32_t nt = std::min<uint32_t>(blocks_for(s->len, 256 * 4), 1u << 20);
+    if (s->dtype == QIPGPU_C128) {
+        hipLaunchKernelGGL(k_reduce_gather<double>, dim3(ng), dim3(256), 0, s->stream, (const cx<double> *)s->amps,
+                           (cx<double> *)slice, count, mbits, mpl, mult);
+        hipLaunchKernelGGL(k_reduce_tile<double>, dim3(nt), dim3(256), 0, s->stream, (cx<double> *)s->amps,
+                           (const cx<double> *)slice, s->len, count - 1);
+    } else {
+        hipLaunchKernelGGL(k_reduce_gather<float>, dim3(ng), dim3(256), 0, s->stream, (const cx<float> *)s->amps,
+                           (cx<float> *)slice, count, mbits, mpl, mult);
+        hipLaunchKernelGGL(k_reduce_tile<float>, dim3(nt), dim3(256), 0, s->stream, (cx<float> *)s->amps,
+                           (const cx<float> *)slice, s->len, count - 1);
+    }
     QIP_LAUNCHED();
     QIP_HIP(hipStreamSynchronize(s->stream));
-    QIP_HIP(hipFree(s->amps));
-    s->amps = out;
+    QIP_HIP(hipFree(slice));
     return QIPGPU_OK;
 }
 
