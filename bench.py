@@ -401,7 +401,9 @@ def bench_measure28(args):
     low qubit (measure_probabilities, one pass over the state each); prob_magnitude (total_prob); and
     measure and then reduce_measure on the 8 scattered qubits (norm, marginals, collapse or tiled
     reduction). Algorithmic bytes per call as defined for these functions: S 2^n + 8 2^k for the
-    probabilities, S 2^n for the norm, 4 S 2^n for measure and reduce_measure."""
+    probabilities, S 2^n for the norm, 4 S 2^n for measure and reduce_measure. Last, the initial state
+    (make_state's product of fed vectors, built on the device): unfed |0...0>, 28 one-qubit feeds, four
+    seven-qubit feeds; S 2^n bytes written."""
     import random
     import numpy
     from qip_amd import GpuBackend
@@ -448,6 +450,20 @@ def bench_measure28(args):
     ms, wall = timed(lambda: be.reduce_measure(cases["scattered8"]))
     red = {"qubits": cases["scattered8"], "ms": ms, "wall_ms": wall, "algorithmic_gbs": 4 * state_bytes / ms / 1e6,
            "hbm_frac": 4 * state_bytes / ms / 1e6 / HBM_PEAK_GBS}
+    # the initial state (make_state's product of fed vectors, built on the device in place)
+    rng = numpy.random.default_rng(args.seed)
+
+    def rand_vec(k):
+        v = rng.standard_normal(2 ** k) + 1j * rng.standard_normal(2 ** k)
+        return v / numpy.linalg.norm(v)
+    inits = {"zero": ([], []),
+             "28_one_qubit_feeds": ([[q] for q in range(n)], [rand_vec(1) for _ in range(n)]),
+             "4_seven_qubit_feeds": ([list(range(7 * g, 7 * g + 7)) for g in range(4)], [rand_vec(7) for _ in range(4)])}
+    init = {}
+    for name, (groups, feeds) in inits.items():
+        ms, wall = timed(lambda groups=groups, feeds=feeds: be.reinit(groups, feeds))
+        init[name] = {"ms": ms, "wall_ms": wall, "algorithmic_gbs": state_bytes / ms / 1e6,
+                      "hbm_frac": state_bytes / ms / 1e6 / HBM_PEAK_GBS}
     eight = [probs[c] for c in ("high8", "low8", "scattered8")]
     value = sum(c["algorithmic_gbs"] for c in eight) / 3
     avg_ms = sum(c["ms"] for c in eight) / 3
@@ -457,6 +473,7 @@ def bench_measure28(args):
            "seed": args.seed}
     out = _line("28-qubit c128 marginal probabilities over 8 qubits: algorithmic GB/s", value, "GB/s", reps, 1,
                 avg_ms, "f64", cfg, probabilities=probs, total_prob=norm, measure=meas, reduce_measure=red,
+                initial_state=init,
                 roofline={"bound": "hbm", "kernel": "k_marg_pass (one pass over the state, FP64 sums)",
                           "achieved": value, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": value / HBM_PEAK_GBS,
                           "traffic": int((MARG_PMC_28Q["fetch_kib"] * MARG_PMC_28Q["fetch_scale"] +
