@@ -234,6 +234,56 @@ struct InitArgs {
 // qip/backend.py:88-101: amplitude = prod_g feed_g[value of the index on group g], computed in
 // group order from 1.0 with numpy's complex product (no fused multiply-add), rounded to the
 // state's precision once at the end.
+// The same product in the same order, chunk by chunk (n >= 12): the leading `m_prefix` groups have all
+// their bits above bit 11, so their partial product is the same for the 4096 amplitudes of a chunk and
+// is formed once per chunk; each amplitude continues it with the other groups in order. Every
+// rounding is the one k_init_product makes.
+template <typename R>
+__global__ __launch_bounds__(256) void k_init_chunked(cx<R> *amps, InitArgs a, int32_t m_prefix) {
+#pragma clang fp contract(off)
+    const uint64_t nchunks = a.len >> 12;
+    for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const uint64_t base = c << 12;
+        double pre = 1.0, pim = 0.0;
+        int pos = 0;
+        for (int g = 0; g < m_prefix; ++g) {
+            const int L = a.glen[g];
+            uint64_t sub = 0;
+            for (int j = 0; j < L; ++j) sub = (sub << 1) | ((base >> a.gpos[pos + j]) & 1ull);
+            pos += L;
+            const cx<double> f = a.feeds[a.goff[g] + (int64_t)sub];
+            const double nre = pre * f.re - pim * f.im;
+            const double nim = pre * f.im + pim * f.re;
+            pre = nre;
+            pim = nim;
+        }
+        for (int e = 0; e < 16; ++e) {
+            const uint64_t i = base + threadIdx.x + 256u * e;
+            double re = 0.0, im = 0.0;
+            if ((i & a.unfed_mask) == 0) {
+                re = pre;
+                im = pim;
+                int p2 = pos;
+                for (int g = m_prefix; g < a.n_groups; ++g) {
+                    const int L = a.glen[g];
+                    uint64_t sub = 0;
+                    for (int j = 0; j < L; ++j) sub = (sub << 1) | ((i >> a.gpos[p2 + j]) & 1ull);
+                    p2 += L;
+                    const cx<double> f = a.feeds[a.goff[g] + (int64_t)sub];
+                    const double nre = re * f.re - im * f.im;
+                    const double nim = re * f.im + im * f.re;
+                    re = nre;
+                    im = nim;
+                }
+            }
+            cx<R> o;
+            o.re = (R)re;
+            o.im = (R)im;
+            amps[i] = o;
+        }
+    }
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_init_product(cx<R> *amps, InitArgs a) {
 #pragma clang fp contract(off)
@@ -407,11 +457,33 @@ int qipgpu_init_product(qipgpu_state *s, int32_t n_groups, const int32_t *group_
     a.gpos = reinterpret_cast<const int32_t *>(d + b_glen);
     a.goff = reinterpret_cast<const int64_t *>(d + o_goff);
     a.feeds = reinterpret_cast<const cx<double> *>(d + o_feed);
-    const uint32_t nb = std::min<uint32_t>(blocks_for(s->len, 256), 1u << 20);
-    if (s->dtype == QIPGPU_C128)
-        hipLaunchKernelGGL(k_init_product<double>, dim3(nb), dim3(256), 0, s->stream, (cx<double> *)s->amps, a);
-    else
-        hipLaunchKernelGGL(k_init_product<float>, dim3(nb), dim3(256), 0, s->stream, (cx<float> *)s->amps, a);
+    // leading groups whose bits are all above bit 11: formed once per 4096-amplitude chunk
+    int32_t m_prefix = 0;
+    if (s->n >= 12) {
+        int64_t pos = 0;
+        for (int g = 0; g < n_groups; ++g) {
+            bool high = true;
+            for (int j = 0; j < group_len[g]; ++j) high = high && gpos[pos + j] >= 12;
+            pos += group_len[g];
+            if (!high) break;
+            ++m_prefix;
+        }
+    }
+    if (m_prefix > 0) {
+        const uint32_t nb = (uint32_t)std::min<uint64_t>(s->len >> 12, (uint64_t)device_cus() * 8);
+        if (s->dtype == QIPGPU_C128)
+            hipLaunchKernelGGL(k_init_chunked<double>, dim3(nb), dim3(256), 0, s->stream, (cx<double> *)s->amps, a,
+                               m_prefix);
+        else
+            hipLaunchKernelGGL(k_init_chunked<float>, dim3(nb), dim3(256), 0, s->stream, (cx<float> *)s->amps, a,
+                               m_prefix);
+    } else {
+        const uint32_t nb = std::min<uint32_t>(blocks_for(s->len, 256), 1u << 20);
+        if (s->dtype == QIPGPU_C128)
+            hipLaunchKernelGGL(k_init_product<double>, dim3(nb), dim3(256), 0, s->stream, (cx<double> *)s->amps, a);
+        else
+            hipLaunchKernelGGL(k_init_product<float>, dim3(nb), dim3(256), 0, s->stream, (cx<float> *)s->amps, a);
+    }
     QIP_LAUNCHED();
     QIP_HIP(hipStreamSynchronize(s->stream));  // host descriptor buffer goes out of scope
     return QIPGPU_OK;

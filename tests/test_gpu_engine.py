@@ -125,6 +125,29 @@ def test_init_product():
     assert numpy.array_equal(be0.get_state(), numpy.eye(16)[0])
 
 
+@pytest.mark.parametrize("dtype", [numpy.complex128, numpy.complex64])
+@pytest.mark.parametrize("groups", [
+    [[q] for q in range(16)],                  # one feed per qubit: qubits 0..3 (bits 15..12) lead
+    [[15, 14], [0, 1, 2], [7], [3, 4]],        # a group on low bits first: no common prefix
+    [[0, 2], [5, 9, 11], [13]],                # qubits 1, 3, 4, ... unfed
+    [[0, 1, 2, 3], [8, 4], [15], [6, 5, 7]],
+])
+def test_init_product_chunks(groups, dtype):
+    """From 12 qubits on, the product of the leading groups on bits above 11 is formed once per
+    4096-amplitude chunk and continued per amplitude in the same group order: the same zeros, and
+    each value within (groups) ulps of the oracle's product, which takes the same order but may fuse
+    a multiply (complex64: within one rounding of the oracle's value rounded)."""
+    n = 16
+    feeds = [rand_state(len(g)) for g in groups]
+    be = GpuBackend.make_state(n, groups, feeds, statetype=dtype)
+    got = be.get_state().astype(numpy.complex128)
+    ref = oracle.make_state_vector(n, groups, feeds)
+    nz = ref != 0
+    assert numpy.array_equal(got != 0, nz)
+    rel = numpy.abs(got[nz] - ref[nz]) / numpy.abs(ref[nz])
+    assert numpy.max(rel) <= (len(groups) * 1.2e-16 if dtype == numpy.complex128 else 1.2e-7)
+
+
 def test_ranges():
     n = 8
     psi = rand_state(n)
