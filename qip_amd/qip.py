@@ -12,7 +12,7 @@ from typing import Any, Callable, Iterable, Optional, Sequence, Tuple, Type, Uni
 import numpy
 
 from .backend import StateType
-from .pipeline import PipelineObject
+from .pipeline import PipelineObject, graph_changed
 from .util import InitialState, flatten
 
 
@@ -73,6 +73,7 @@ class Qubit(PipelineObject):
         return sq, rest
 
     def set_sink(self, sink: "PipelineObject"):
+        graph_changed()
         if type(sink) == Qubit:
             if len(self.sink) == 0 or self.sink[0].qid == sink.qid:
                 self.sink += [sink]
