@@ -139,5 +139,7 @@ def i32(seq) -> numpy.ndarray:
     return numpy.ascontiguousarray(numpy.asarray(seq, dtype=numpy.int64).astype(numpy.int32))
 
 
-def ptr(a: numpy.ndarray):
-    return a.ctypes.data_as(ctypes.c_void_p)
+def ptr(a: numpy.ndarray) -> int:
+    """Address of `a`'s data for a `void*` argument. The int holds no reference to `a`: the caller
+    keeps the array (or the array a view comes from) alive until the call returns."""
+    return a.ctypes.data
