@@ -191,7 +191,9 @@ def tabulate(func, n_reg1: int) -> numpy.ndarray:
     return table
 
 
-def func_table(func, n_reg1: int, cache: bool = True) -> _FuncTable:
+def func_table(func, n_reg1: int, cache: bool = False) -> _FuncTable:
+    """The device table of `func` on an n_reg1-qubit register: a FuncTable's own, a fresh one, or
+    (cache=True) one kept per function object for later calls."""
     if isinstance(func, FuncTable):
         return func.device_table(n_reg1)
     if not cache or isinstance(func, numpy.ndarray):
@@ -233,9 +235,12 @@ class GpuBackend(StateType):
     """Plain MI355X backend: one in-place HIP sweep per connected component of each operator,
     complex128 (the drop-in) or complex64 state, FP64 reductions."""
 
-    # f tables are evaluated once per (function object, register size) and kept on the device;
-    # set to False for functions whose values change between calls.
-    cache_func_tables = True
+    # A plain callable is tabulated again on every func_apply, as the reference does
+    # (func_apply.pyx:66-69), so a function whose values change between calls is followed. Pass a
+    # FuncTable (or an integer ndarray) to keep a table on the device across calls; set this to True
+    # to keep one device table per function object instead (only for functions whose values never
+    # change).
+    cache_func_tables = False
 
     def __init__(self, n: int, handle: ctypes.c_void_p, dtype: int):
         super().__init__(n, None)

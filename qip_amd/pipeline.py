@@ -1,15 +1,27 @@
 """Graph execution (counterpart of qip/pipeline.py).
 
-A circuit is a graph of PipelineObject nodes built by the node constructors of ``qip_amd.qip``
-and ``qip_amd.operators``. ``run`` collects the graph, gives every input node a contiguous range
-of qubit indices in order of its `qid`, builds the state through the backend constructor, and
-feeds every node to the state in dependency order, lowest qid first. The default backend is the
-MI355X ``GpuBackend``; any StateType constructor can be passed as ``backend_constructor``.
+A circuit is a graph of PipelineObject nodes wired through two lists per node: `inputs` (the nodes
+it reads) and `sink` (the nodes that registered themselves as its readers). `run` turns the part of
+the graph its outputs need into a Schedule and plays it against a state backend:
+
+  sources  the nodes the walk starts from, in qid order; each takes the next `n` qubit indices of
+           the state (qip/pipeline.py:111-122). Inside run the sources are the leaves (nodes
+           without inputs): run hands the collector its feed keyed by tuples, which no node equals
+           (qip/pipeline.py:90,100,155).
+  steps    Kahn's algorithm over the graph. A node becomes runnable when the last of its inputs to
+           run lists it as a sink; of the runnable nodes the lowest qid runs first, equal qids in
+           the order they became runnable (qip/pipeline.py:186-222). Each step holds the index
+           groups of the node's inputs at that point of the walk.
+
+A schedule depends only on the graph and on the feed's keys, so run keeps compiled schedules on the
+first output node (at most four feed-key sets per node) and plays one again while no node anywhere
+has gained a sink since it was compiled. Graphs with a node that overrides `feed` or
+`remap_index` see the live index map and are walked on every run.
 """
-import collections
 import heapq
 import itertools
-from typing import AbstractSet, Any, Callable, Mapping, Sequence, Tuple, Type, Union
+import threading
+from typing import AbstractSet, Any, Callable, Dict, List, Mapping, Optional, Sequence, Tuple, Type, Union
 
 import numpy
 
@@ -18,75 +30,219 @@ from .util import InitialState, flatten
 
 
 class PipelineObject(object):
-    """Base graph node (qip/pipeline.py:7-68)."""
+    """Base graph node (qip/pipeline.py:7-68). Subclasses act on the state in `feed_indices`."""
 
     def __init__(self, quantum: bool, default: InitialState = None, n: int = 0):
         self.quantum = quantum
-        self.inputs = []
-        self.sink = []
         self.default = default
         self.n = n
+        self.inputs = []
+        self.sink = []
 
     def run(self, feed: Mapping["PipelineObject", InitialState] = None, **kwargs):
         return run(self, feed=feed, **kwargs)
 
     def feed(self, state: StateType, qbitindex: Mapping["PipelineObject", Sequence[int]],
              n: int) -> Tuple[int, object]:
-        """Apply this node to the state, given the qubit indices of every node."""
+        """Act on the state given every node's qubit indices; returns (qubits removed, annotation)."""
         return self.feed_indices(state, [qbitindex[q] for q in self.inputs], n)
 
     def feed_indices(self, state: StateType, index_groups: Sequence[Sequence[int]],
                      n: int) -> Tuple[int, Any]:
-        """Apply this node given its inputs' index groups; returns (qubits removed, annotation)."""
+        """Act on the state given the index group of each input; the base node is the identity."""
         return 0, 0
 
     def select_index(self, indices: Sequence[int]) -> Sequence[int]:
+        """This node's qubit indices, given its inputs' indices concatenated."""
         return indices
 
     def remap_index(self, index_map: Mapping["PipelineObject", int], n: int) -> Mapping["PipelineObject", int]:
+        """The index map after this node has run."""
         return index_map
 
     def set_sink(self, sink: "PipelineObject"):
-        graph_changed()
-        self.sink += [sink]
+        _GRAPH_EPOCH.advance()
+        self.sink.append(sink)
 
     def get_inputs(self) -> Sequence["PipelineObject"]:
         return self.inputs
 
     # Nodes compare by identity, so an identity hash is consistent with equality. (The reference
-    # hashes repr(self), qip/pipeline.py:67-68, which recurses through every input and makes each
-    # set operation of a run O(graph depth); its string hash is randomised per process, so no
-    # result can depend on the order it gives.)
+    # hashes repr(self), qip/pipeline.py:67-68, which recurses through every input and costs
+    # O(graph depth) per set operation; its string hash is randomised per process, so no result
+    # can depend on the order it gives.)
     __hash__ = object.__hash__
 
 
-# Schedules of recent runs. A run's visiting order and the index groups every node is fed depend only
-# on the graph (nodes, their inputs and sinks) and on the feed's keys, as long as no node overrides
-# `feed` or `remap_index`; a graph run again replays its schedule instead of walking the graph. Every
-# new sink (any node built on existing nodes, in any graph) advances the version and retires every
-# stored schedule. A replay feeds the same nodes, in the same order, with the same index groups as
-# the walk.
-_GRAPH_VERSION = [0]
-_SCHEDULES = collections.OrderedDict()
-_SCHEDULE_SLOTS = 8
+class _Epoch:
+    """A counter advanced whenever any node gains a sink; compiled schedules remember it."""
+
+    def __init__(self):
+        self.value = 0
+
+    def advance(self) -> None:
+        self.value += 1
+
+
+_GRAPH_EPOCH = _Epoch()
+_CACHE_LOCK = threading.Lock()
+_SCHEDULES_PER_NODE = 4
 
 
 def graph_changed() -> None:
-    """Retire every stored schedule (called whenever a node gains a sink)."""
-    _GRAPH_VERSION[0] += 1
+    """Retire every compiled schedule (done by every set_sink; callable directly)."""
+    _GRAPH_EPOCH.advance()
 
 
-def _replayable(graphnodes: AbstractSet[PipelineObject]) -> bool:
-    return all(type(node).feed is PipelineObject.feed and type(node).remap_index is PipelineObject.remap_index
-               for node in graphnodes)
+def _qid(node: PipelineObject) -> int:
+    return getattr(node, "qid", 0)
 
 
-class _ScheduleRecorder:
-    def __init__(self):
-        self.order = []
+def _collect(outputs: Sequence[PipelineObject], stop: Mapping) -> Tuple[List[PipelineObject], List[PipelineObject]]:
+    """(sources, nodes) reachable backwards from `outputs`, both in depth-first discovery order:
+    the walk does not go past a node without inputs or a node that is a key of `stop`."""
+    nodes: List[PipelineObject] = []
+    sources: List[PipelineObject] = []
+    visited = set()
+    pending = list(reversed(outputs))
+    while pending:
+        node = pending.pop()
+        if node in visited:
+            continue
+        visited.add(node)
+        nodes.append(node)
+        if len(node.inputs) == 0 or node in stop:
+            sources.append(node)
+        else:
+            pending.extend(reversed([i for i in node.inputs if i not in visited]))
+    return sources, nodes
 
-    def feed(self, qbitindex: Mapping[PipelineObject, Sequence[int]], node: PipelineObject) -> None:
-        self.order.append((node, [qbitindex[q] for q in node.inputs]))
+
+def _ordered_sources(sources: Sequence[PipelineObject]) -> List[PipelineObject]:
+    # stable: equal qids keep discovery order
+    return sorted(sources, key=_qid)
+
+
+def _walk(sources: Sequence[PipelineObject], nodes: AbstractSet[PipelineObject],
+          visit: Callable[[Dict, PipelineObject], None]) -> None:
+    """Visit every node of `nodes` reachable from `sources` in execution order (module docstring).
+    visit(index_map, node) sees the index map as it stands when the node runs."""
+    start = _ordered_sources(sources)
+    width = sum(s.n for s in start)
+    index_map: Dict = {}
+    base = 0
+    for src in start:
+        index_map[src] = list(range(base, base + src.n))
+        base += src.n
+    source_set = set(start)
+    # readers of each node inside the graph (the inputs side of the edges); sources never wait
+    readers: Dict[PipelineObject, List[PipelineObject]] = {}
+    missing: Dict[PipelineObject, int] = {}
+    for node in nodes:
+        if node in source_set:
+            continue
+        distinct = set(node.inputs)
+        missing[node] = len(distinct)
+        for u in distinct:
+            readers.setdefault(u, []).append(node)
+    arrival = itertools.count()
+    heap = [(_qid(s), next(arrival), s) for s in start]
+    heapq.heapify(heap)
+    finished = set()
+    while heap:
+        node = heapq.heappop(heap)[2]
+        visit(index_map, node)
+        index_map = node.remap_index(index_map, width)
+        finished.add(node)
+        now_ready = set()
+        for r in readers.get(node, ()):
+            missing[r] -= 1
+            if missing[r] == 0:
+                now_ready.add(r)
+        # a node whose last input does not list it as a sink never runs (qip/pipeline.py:213-221)
+        for r in node.sink:
+            if r in now_ready:
+                now_ready.discard(r)
+                index_map[r] = r.select_index(flatten(index_map[i] for i in r.inputs))
+                heapq.heappush(heap, (_qid(r), next(arrival), r))
+
+
+class Schedule:
+    """A graph compiled for one set of outputs and feed keys: the sources (qid order) and, when
+    every node uses the base `feed` and `remap_index`, the steps as (node, index groups)."""
+
+    __slots__ = ("sources", "nodes", "n", "steps", "epoch")
+
+    def __init__(self, outputs: Sequence[PipelineObject]):
+        sources, nodes = _collect(outputs, {})
+        self.sources = _ordered_sources(sources)
+        self.nodes = set(nodes)
+        self.n = sum(s.n for s in self.sources)
+        self.epoch = _GRAPH_EPOCH.value
+        self.steps: Optional[List[Tuple[PipelineObject, Tuple[Tuple[int, ...], ...]]]] = None
+        if all(type(v).feed is PipelineObject.feed and type(v).remap_index is PipelineObject.remap_index
+               for v in nodes):
+            steps = []
+            _walk(self.sources, self.nodes,
+                  lambda imap, v: steps.append((v, tuple(tuple(imap[i]) for i in v.inputs))))
+            self.steps = steps
+
+    def index_ranges(self) -> Dict[PipelineObject, List[int]]:
+        out, base = {}, 0
+        for s in self.sources:
+            out[s] = list(range(base, base + s.n))
+            base += s.n
+        return out
+
+    def play(self, state: StateType) -> Tuple[object, Dict]:
+        """Feed every node to the state; returns (state.get_state(), {node: annotation})."""
+        acc = NodeFeeder(state, self.n)
+        if self.steps is None:
+            _walk(self.sources, self.nodes, acc.feed)
+        else:
+            for node, groups in self.steps:
+                acc.feed_groups(node, [list(g) for g in groups])
+        return acc.state.get_state(), acc.classic_map
+
+
+def _schedule_for(outputs: Tuple[PipelineObject, ...], feed_keys: Tuple) -> Schedule:
+    owner = outputs[0]
+    key = (outputs, feed_keys)
+    with _CACHE_LOCK:
+        table = owner.__dict__.setdefault("_compiled_schedules", {})
+        sched = table.get(key)
+        if sched is not None and sched.epoch == _GRAPH_EPOCH.value:
+            return sched
+    sched = Schedule(outputs)
+    if sched.steps is not None:
+        with _CACHE_LOCK:
+            table.pop(key, None)
+            while len(table) >= _SCHEDULES_PER_NODE:
+                table.pop(next(iter(table)))
+            table[key] = sched
+    return sched
+
+
+def _default_vector(node: PipelineObject):
+    """An unfed source's state from its `default`: an int k > 0 is the basis vector e_k, 0 is left
+    out (|0> anyway), anything else is used as given (qip/pipeline.py:101-109)."""
+    d = node.default
+    if type(d) != int:
+        return d
+    if 0 < d < 2 ** node.n:
+        vec = numpy.zeros((2 ** node.n,))
+        vec[d] = 1.0
+        return vec
+    return None
+
+
+def _given(v) -> bool:
+    # the deprecated whole-state feed counts when non-empty (an ndarray has no plain truth value)
+    if v is None:
+        return False
+    if isinstance(v, numpy.ndarray):
+        return v.size > 0
+    return bool(v)
 
 
 def run(*args: PipelineObject,
@@ -97,132 +253,59 @@ def run(*args: PipelineObject,
         **kwargs):
     """Run the circuit ending in `args` (qip/pipeline.py:71-135).
 
-    feed: {node or tuple of nodes: amplitudes}; unfed input nodes take their `default` (an int
-    becomes a basis vector) or |0>. Returns (state.get_state(), {node: annotation}).
+    feed: {node or tuple of nodes: amplitudes}; a tuple key feeds the joint state of its nodes.
+    Unfed sources take their `default` or |0>. strict=True requires every source in the feed.
+    Returns (state.get_state(), {node: annotation}) for every node that ran.
     """
-    feed = {} if feed is None else {(k if type(k) == tuple else (k,)): feed[k] for k in feed}
-    fed_nodes = flatten(feed.keys())
+    given = {}
+    for k, v in (feed or {}).items():
+        given[k if type(k) == tuple else (k,)] = v
+    fed = set(flatten(given.keys()))
+    sched = _schedule_for(tuple(args), tuple(given.keys()))
+    if strict:
+        missing = [s for s in sched.sources if s not in fed]
+        if missing:
+            raise ValueError("Missing Qubit states for: {}".format(missing))
+    for s in sched.sources:
+        if s not in fed and s.default is not None:
+            vec = _default_vector(s)
+            if vec is not None:
+                given[(s,)] = vec
+    ranges = sched.index_ranges()
+    groups = [[i for node in key for i in ranges[node]] for key in given]
+    vectors = list(given.values())
+    if _given(kwargs.get("state")):
+        # deprecated: one vector for the whole register (qip/pipeline.py:127-131)
+        groups = [tuple(range(sched.n))]
+        vectors = [kwargs["state"]]
     if backend_constructor is None:
         backend_constructor = GpuBackend.make_state
-
-    key = (args, tuple(feed.keys()))
-    stored = _SCHEDULES.get(key)
-    if stored is not None and stored[0] == _GRAPH_VERSION[0]:
-        _SCHEDULES.move_to_end(key)
-        _, frontier, graphnodes, order = stored
-    else:
-        frontier, graphnodes = get_deps(*args, feed=feed)
-        order = None
-        if _replayable(graphnodes):
-            recorder = _ScheduleRecorder()
-            run_graph(frontier, graphnodes, recorder)
-            order = recorder.order
-            _SCHEDULES[key] = (_GRAPH_VERSION[0], frontier, graphnodes, order)
-            while len(_SCHEDULES) > _SCHEDULE_SLOTS:
-                _SCHEDULES.popitem(last=False)
-    for qubit in frontier:
-        if qubit not in fed_nodes and qubit.default is not None:
-            if type(qubit.default) == int:
-                # default 0 is |0>, which an unfed qubit is anyway (qip/pipeline.py:103-107)
-                if 0 < qubit.default < 2 ** qubit.n:
-                    vec = numpy.zeros((2 ** qubit.n,))
-                    vec[qubit.default] = 1.0
-                    feed[(qubit,)] = vec
-            else:
-                feed[(qubit,)] = qubit.default
-
-    frontier = sorted(frontier, key=lambda q: q.qid)
-    if strict:
-        missing = [q for q in frontier if q not in fed_nodes]
-        if len(missing):
-            raise ValueError("Missing Qubit states for: {}".format(missing))
-
-    qbitindex = {}
-    n = 0
-    for qbit in frontier:
-        qbitindex[qbit] = list(range(n, n + qbit.n))
-        n += qbit.n
-
-    feed_index_groups = [sum((qbitindex[q] for q in qs), []) for qs in feed]
-    feed_states = [feed[qs] for qs in feed]
-    if "state" in kwargs and kwargs["state"] is not None and _truthy(kwargs["state"]):
-        # deprecated whole-state feed (qip/pipeline.py:127-131)
-        feed_index_groups = [tuple(range(n))]
-        feed_states = [kwargs["state"]]
-
-    state = backend_constructor(n, feed_index_groups, feed_states, statetype=statetype)
-    if order is None:
-        return feed_forward(frontier, state, graphnodes)
-    acc = NodeFeeder(state, n)
-    for node, index_groups in order:
-        acc.feed_groups(node, index_groups)
-    return acc.state.get_state(), acc.classic_map
-
-
-def _truthy(v) -> bool:
-    # `if kwargs['state']` on an ndarray is ambiguous; the reference only uses vectors here,
-    # which it tests for truth through the list/array it was given.
-    if isinstance(v, numpy.ndarray):
-        return v.size > 0
-    return bool(v)
+    state = backend_constructor(sched.n, groups, vectors, statetype=statetype)
+    return sched.play(state)
 
 
 def get_deps(*args: PipelineObject, feed: Mapping = None) -> Tuple[Sequence[PipelineObject], AbstractSet[PipelineObject]]:
-    """(input nodes, all nodes) of the graph ending in `args` (qip/pipeline.py:138-160): walk
-    inputs back until a node is fed or has no inputs."""
-    if feed is None:
-        feed = {}
-    seen = set()
-    deps = set()
-    todo = set(args)
-    while todo:
-        node = todo.pop()
-        if node in feed or len(node.inputs) == 0:
-            deps.add(node)
-        else:
-            # a node reached again through another path has been walked already (same sets as
-            # walking it again, without the repeated walks of shared sub-graphs)
-            todo.update(i for i in node.inputs if i not in seen)
-        seen.add(node)
-    return list(deps), seen
+    """(sources, all nodes) of the graph ending in `args` (qip/pipeline.py:138-160): the walk back
+    through inputs stops at nodes without inputs and at keys of `feed`."""
+    sources, nodes = _collect(args, feed or {})
+    return sources, set(nodes)
 
 
 class GraphAccumulator:
+    """Receives every node of a walk in execution order (qip/pipeline.py:181-183)."""
+
     def feed(self, qbitindex: Mapping[PipelineObject, Sequence[int]], node: PipelineObject) -> None:
         raise NotImplementedError("Method not implemented.")
 
 
 def run_graph(frontier: Sequence[PipelineObject], graphnodes: AbstractSet[PipelineObject], graphacc: GraphAccumulator):
-    """Visit every node once its inputs are done, lowest qid first (qip/pipeline.py:186-222)."""
-    qbitindex = {}
-    seen = set()
-    n = 0
-    for qbit in sorted(frontier, key=lambda q: q.qid):
-        qbitindex[qbit] = list(range(n, n + qbit.n))
-        n += qbit.n
-    # The reference re-sorts its frontier list by qid (a stable sort) and pops the head: lowest qid
-    # first, equal qids in the order they joined the frontier. A heap keyed (qid, arrival) takes the
-    # same order without re-sorting.
-    arrival = itertools.count()
-    heap = [(node.qid, next(arrival), node) for node in frontier]
-    heapq.heapify(heap)
-    queued = set(frontier)
-    while heap:
-        node = heapq.heappop(heap)[2]
-        queued.discard(node)
-        graphacc.feed(qbitindex, node)
-        qbitindex = node.remap_index(qbitindex, n)
-        seen.add(node)
-        for nxt in node.sink:
-            if nxt in graphnodes and nxt not in seen and nxt not in queued:
-                if all(prev in seen for prev in nxt.inputs):
-                    heapq.heappush(heap, (nxt.qid, next(arrival), nxt))
-                    queued.add(nxt)
-                    qbitindex[nxt] = nxt.select_index(flatten(qbitindex[j] for j in nxt.inputs))
+    """Pass every node to graphacc.feed(index_map, node) in execution order (qip/pipeline.py:186-222)."""
+    _walk(list(frontier), set(graphnodes), lambda imap, node: graphacc.feed(imap, node))
 
 
 class NodeFeeder(GraphAccumulator):
-    """Feeds nodes to a state and records every node's annotation (qip/pipeline.py:225-240)."""
+    """Feeds nodes to a state and keeps each node's annotation (qip/pipeline.py:225-240): every node
+    whose annotation is not None lands in classic_map, 0 for plain gates."""
 
     def __init__(self, state: StateType, n: int):
         self.state = state
@@ -230,35 +313,34 @@ class NodeFeeder(GraphAccumulator):
         self.classic_map = {}
 
     def feed(self, qbitindex: Mapping[PipelineObject, Sequence[int]], node: PipelineObject):
-        self._record(node, *node.feed(self.state, qbitindex, self.n))
+        self._keep(node, node.feed(self.state, qbitindex, self.n))
 
     def feed_groups(self, node: PipelineObject, index_groups: Sequence[Sequence[int]]):
-        self._record(node, *node.feed_indices(self.state, index_groups, self.n))
+        self._keep(node, node.feed_indices(self.state, index_groups, self.n))
 
-    def _record(self, node: PipelineObject, n_bits: int, bits: object):
-        if n_bits > 0 or bits is not None:
-            self.classic_map[node] = bits
-            self.n -= n_bits
+    def _keep(self, node: PipelineObject, result: Tuple[int, object]) -> None:
+        removed, note = result
+        if removed > 0 or note is not None:
+            self.classic_map[node] = note
+            self.n -= removed
 
 
 def feed_forward(frontier: Sequence[PipelineObject], state: StateType, graphnodes: AbstractSet[PipelineObject]):
-    n = sum(f.n for f in frontier)
-    acc = NodeFeeder(state, n)
+    acc = NodeFeeder(state, sum(f.n for f in frontier))
     run_graph(frontier, graphnodes, acc)
     return acc.state.get_state(), acc.classic_map
 
 
 def make_circuit_mat(*args: PipelineObject, backend_constructor: Callable = None) -> numpy.ndarray:
-    """Dense matrix of the quantum part of a circuit, one basis state per column. The reference
-    helper (qip/pipeline.py:163-178) feeds run(state=i) with an int, which its backend cannot
-    index; here column i is the circuit applied to the basis vector e_i."""
-    o, _ = run(*args, backend_constructor=backend_constructor)
-    n = int(numpy.log2(len(o)))
-    mat = numpy.zeros((2 ** n, 2 ** n), dtype=numpy.complex128)
-    mat[:, 0] = o
-    for i in range(1, 2 ** n):
-        e = numpy.zeros(2 ** n, dtype=numpy.complex128)
+    """Dense matrix of the quantum part of a circuit: column i is the circuit applied to the basis
+    vector e_i. (The reference helper, qip/pipeline.py:163-178, feeds run(state=i) with an int,
+    which its backend cannot index.)"""
+    first, _ = run(*args, backend_constructor=backend_constructor)
+    dim = len(first)
+    cols = [numpy.asarray(first, dtype=numpy.complex128)]
+    for i in range(1, dim):
+        e = numpy.zeros(dim, dtype=numpy.complex128)
         e[i] = 1.0
-        o, _ = run(*args, state=e, backend_constructor=backend_constructor)
-        mat[:, i] = o
-    return mat
+        out, _ = run(*args, state=e, backend_constructor=backend_constructor)
+        cols.append(numpy.asarray(out, dtype=numpy.complex128))
+    return numpy.stack(cols, axis=1)

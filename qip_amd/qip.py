@@ -1,93 +1,99 @@
-"""Qubit and measurement nodes (counterpart of qip/qip.py).
+"""Register and measurement nodes (counterpart of qip/qip.py).
 
-Semantics kept from the reference because results depend on them:
-  * every node takes a process-wide increasing `qid` unless given one; input nodes get their
-    qubit index ranges in qid order and execution breaks ties by lowest qid (qip/qip.py:13,35-38);
-  * a Qubit may sink into one Qubit only (no cloning, qip/qip.py:91-99);
-  * Measure collapses through state.measure and keeps every qubit; StochasticMeasure reads
-    state.measure_probabilities without changing the state (qip/qip.py:135-185).
+Rules kept from the reference because results depend on them:
+  * every node draws a qid from one process-wide increasing counter, `Qubit.QID`
+    (qip/qip.py:13,35-38), except that the parts of one split share the qid of the first part and a
+    controlled op shares the qid of the op it controls. Sources take their qubit index ranges in
+    qid order, and the walk breaks ties by lowest qid;
+  * a node may feed into one plain Qubit node only, unless those share a qid (no cloning,
+    qip/qip.py:91-99); gates and measurements may read a node freely;
+  * Measure collapses through state.measure and keeps every qubit, annotating (bits, probability);
+    StochasticMeasure reads state.measure_probabilities and leaves the state alone
+    (qip/qip.py:135-185).
 """
-from typing import Any, Callable, Iterable, Optional, Sequence, Tuple, Type, Union
+import itertools
+from typing import Any, Callable, Iterable, List, Optional, Sequence, Tuple, Type, Union
 
 import numpy
 
 from .backend import StateType
-from .pipeline import PipelineObject, graph_changed
+from .pipeline import PipelineObject
 from .util import InitialState, flatten
 
 
+def _read_by(node: PipelineObject, inputs: Sequence[PipelineObject]) -> None:
+    """Register `node` as a reader (sink) of each of its inputs."""
+    for src in inputs:
+        src.set_sink(node)
+
+
+def _segments(n: int, cuts: Iterable[int]) -> List[Tuple[int, int]]:
+    """[lo, hi) index ranges covering 0..n-1, cut at each point of `cuts` inside (0, n)."""
+    inner = sorted(cuts)
+    if any(c < 0 or c > n for c in inner):
+        raise ValueError("split points must lie within 0..{}".format(n))
+    marks = [0] + [c for c in inner if 0 < c < n] + [n]
+    return list(zip(marks[:-1], marks[1:]))
+
+
 class Qubit(PipelineObject):
-    """A register of `n` qubits; with inputs it is the identity on them (qip/qip.py:8-102)."""
+    """A register of `n` qubits; built on inputs it is the identity on them (qip/qip.py:8-102)."""
     QID = 0
+
+    @staticmethod
+    def _draw_qid() -> int:
+        qid = Qubit.QID
+        Qubit.QID = qid + 1
+        return qid
 
     def __init__(self, *inputs: "Qubit", n: int = None, qid: int = None,
                  nosink: bool = False, default: InitialState = None, quantum: bool = True, **kwargs):
         super().__init__(quantum, default=default)
-        if n is None:
-            n = sum(q.n for q in inputs) if len(inputs) > 0 else 1
-        if type(default) == list and len(default) != 2 ** n:
+        width = n if n is not None else (sum(q.n for q in inputs) if inputs else 1)
+        if type(default) == list and len(default) != 2 ** width:
             raise ValueError("Default state length must be 2**n")
-        if n <= 0:
+        if width <= 0:
             raise Exception("Number of qubits must be greater than 0")
-        self.n = n
-        if qid is None:
-            qid = Qubit.QID
-            Qubit.QID += 1
-        self.qid = qid
+        self.n = width
+        self.qid = self._draw_qid() if qid is None else qid
         self.inputs = inputs
         if not nosink:
-            for item in inputs:
-                item.set_sink(self)
+            _read_by(self, inputs)
 
     def split(self, indices: Optional[Iterable[int]] = None) -> Tuple["Qubit", ...]:
-        """Split into sub-registers at the given qubit offsets (default: one per input)."""
+        """Sub-registers cut at the given qubit offsets; by default one per input, and the node
+        itself when it has exactly one input. All parts share one new qid."""
         if indices is None:
-            cuts = []
-            acc = 0
-            for q in self.inputs:
-                acc += q.n
-                cuts.append(acc)
-            if len(cuts) == 1:
+            if len(self.inputs) == 1:
                 return self,
+            cuts = list(itertools.accumulate(q.n for q in self.inputs))
         else:
-            cuts = sorted(indices)
-        if 0 not in cuts:
-            cuts = [0] + cuts
-        if self.n not in cuts:
-            cuts = cuts + [self.n]
-        parts = []
-        qid = None
-        for lo, hi in zip(cuts[:-1], cuts[1:]):
-            sq = SplitQubit(list(range(lo, hi)), self, qid=qid)
-            parts.append(sq)
-            qid = sq.qid
+            cuts = list(indices)
+        parts: List[SplitQubit] = []
+        for lo, hi in _segments(self.n, cuts):
+            shared = parts[0].qid if parts else None
+            parts.append(SplitQubit(list(range(lo, hi)), self, qid=shared))
         return tuple(parts)
 
     def extract_index(self, indices: Union[int, Iterable[int]]):
-        """(selected, remaining) sub-registers (qip/qip.py:76-87)."""
-        if type(indices) == int:
-            indices = [indices]
-        indices = list(indices)
-        sq = SplitQubit([i for i in indices if 0 <= i < self.n], self)
-        rest = SplitQubit([i for i in range(self.n) if i not in indices], self, qid=sq.qid)
-        return sq, rest
+        """(selected, remaining) sub-registers of this node, sharing one qid."""
+        chosen = [indices] if type(indices) == int else list(indices)
+        picked = SplitQubit([i for i in chosen if 0 <= i < self.n], self)
+        rest = SplitQubit([i for i in range(self.n) if i not in chosen], self, qid=picked.qid)
+        return picked, rest
 
     def set_sink(self, sink: "PipelineObject"):
-        graph_changed()
-        if type(sink) == Qubit:
-            if len(self.sink) == 0 or self.sink[0].qid == sink.qid:
-                self.sink += [sink]
-            else:
-                raise Exception("Qubits may only sink to one output (no cloning)")
-        else:
-            self.sink += [sink]
+        if type(sink) is Qubit and self.sink and self.sink[0].qid != sink.qid:
+            raise Exception("Qubits may only sink to one output (no cloning)")
+        super().set_sink(sink)
 
     def __repr__(self):
         return "Q({})".format(self.qid)
 
 
 class OpConstructor:
-    """Callable wrapper of a node class (qip/qip.py:103-118)."""
+    """A callable building nodes of one kind (qip/qip.py:105-117). Wrappers such as C(...) ask it,
+    through wrap_op_hook, whether it handles being wrapped itself; the base answer is no (None)."""
 
     def __init__(self, op: Union[Type[Qubit], Callable[..., Qubit]]):
         self.op = op
@@ -107,7 +113,7 @@ Q = OpConstructor(Qubit)
 
 
 class SplitQubit(Qubit):
-    """A selection of the qubit indices of its input (qip/qip.py:123-132)."""
+    """The qubits at positions `indices` of its input's register (qip/qip.py:123-132)."""
 
     def __init__(self, indices: Sequence[int], *inputs: Qubit, **kwargs):
         super().__init__(*inputs, n=len(indices), **kwargs)
@@ -120,42 +126,45 @@ class SplitQubit(Qubit):
         return "SplitQubit({})".format(",".join(map(repr, self.inputs)))
 
 
-class Measure(Qubit):
+class _Readout(Qubit):
+    """A node that reads its inputs' qubits, flattened in input order, from the state and annotates
+    the result without removing qubits."""
+
+    def __init__(self, *inputs: Qubit, nosink: bool = False, quantum: bool = True):
+        super().__init__(*inputs, nosink=nosink, quantum=quantum)
+
+    def read(self, state: StateType, indices: numpy.ndarray) -> Any:
+        raise NotImplementedError
+
+    def feed_indices(self, state: StateType, index_groups: Sequence[Sequence[int]], n: int) -> Tuple[int, Any]:
+        return 0, self.read(state, numpy.array(flatten(index_groups), dtype=numpy.int32))
+
+
+class Measure(_Readout):
     """Projective measurement of its inputs; annotation (bits, probability) (qip/qip.py:135-158)."""
 
     def __init__(self, *inputs: Qubit, measure_by: int = None, nosink: bool = False):
         super().__init__(*inputs, nosink=nosink)
-        self.inputs = inputs
-        self.n = sum(q.n for q in self.inputs)
         self.measure = measure_by
-        if not nosink:
-            for item in inputs:
-                item.set_sink(self)
 
-    def feed_indices(self, state: StateType, index_groups: Sequence[Sequence[int]], n: int) -> Tuple[int, Any]:
-        indices = numpy.array(flatten(index_groups), dtype=numpy.int32)
+    def read(self, state: StateType, indices: numpy.ndarray) -> Tuple[int, float]:
         bits, prob = state.measure(indices)
-        return 0, (bits, prob)
+        return bits, prob
 
     def __repr__(self):
-        return "M({})".format(",".join(i.__repr__() for i in self.inputs))
+        return "M({})".format(",".join(repr(i) for i in self.inputs))
 
 
-class StochasticMeasure(Qubit):
-    """Outcome distribution of its inputs without collapse; with top_k, (outcomes, probabilities)
-    of the top_k most likely (qip/qip.py:161-185)."""
+class StochasticMeasure(_Readout):
+    """The outcome distribution of its inputs without collapse (LSB-first, qubit j of the inputs at
+    bit j); with top_k, (outcomes, probabilities) of the top_k most likely (qip/qip.py:161-185)."""
 
     def __init__(self, *inputs: Qubit, nosink: bool = False, top_k: int = 0):
-        super().__init__(*inputs, quantum=False, nosink=nosink)
+        super().__init__(*inputs, nosink=nosink, quantum=False)
         self.top_k = top_k
-        if not nosink:
-            for item in inputs:
-                item.set_sink(self)
 
-    def feed_indices(self, state: StateType, index_groups: Sequence[Sequence[int]], n: int) -> Tuple[int, object]:
-        indices = numpy.array(flatten(index_groups), dtype=numpy.int32)
-        probs = state.measure_probabilities(indices, top_k=self.top_k)[:]
-        return 0, probs
+    def read(self, state: StateType, indices: numpy.ndarray):
+        return state.measure_probabilities(indices, top_k=self.top_k)[:]
 
     def __repr__(self) -> str:
-        return "SM({})".format(",".join(i.__repr__() for i in self.inputs))
+        return "SM({})".format(",".join(repr(i) for i in self.inputs))

@@ -1,123 +1,126 @@
-"""Operator wrappers and control contexts (counterpart of qip/qubit_util.py).
+"""Op wrappers and control contexts (counterpart of qip/qubit_util.py).
 
-QubitOpWrapper reproduces one quirk on purpose because circuit results depend on it: when called,
-it rebuilds its keyword arguments from the ones given at wrap time and drops the caller's
-(qip/qubit_util.py:142-143). So ``Rm(m, q, negate=True)`` builds the +2*pi/2^m phase, which is
-what QFFT's makeR asks for (qip/qfft.py:42-43): QFFT(rev=True) therefore equals
-numpy.fft.ifft(x, norm="ortho"). ``nosplit`` and ``nocontext`` are named parameters and are kept.
+A context entry is (constructor, control qubits, positions). While entries are active, building a
+wrapped op goes through three steps:
+  1. insert: the entries' qubits are placed into the op's inputs at their positions, innermost
+     entry first, so the outermost entry's qubits end up in front;
+  2. wrap: the op constructor is wrapped by every entry's constructor, innermost first
+     (C(C(op)) for two nested C entries);
+  3. release: the wrapped op's outputs at the entries' positions are taken out, outermost entry
+     first, and become the entries' qubits for the next op built in the context.
+Entries live in frames, one per `with QubitWrapperContext(...)` block.
+
+QubitOpWrapper reproduces one quirk on purpose, because circuit results depend on it: a call
+builds the node with the keyword arguments given at wrap time and drops the caller's
+(qip/qubit_util.py:142-143). So `Rm(m, q, negate=True)` builds the +2 pi / 2^m phase, which is what
+QFFT's makeR asks for (qip/qfft.py:42-43): QFFT(rev=True) equals numpy.fft.ifft(x, norm="ortho").
+`nosplit` and `nocontext` are named parameters of the call and are kept.
 """
 import collections.abc
-from typing import Any, Callable, Mapping, Optional, Sequence, Tuple, Type, Union
+from typing import Any, Callable, List, Mapping, Optional, Sequence, Tuple, Type, Union
 
 from .qip import OpConstructor, Qubit
 from .util import flatten
 
 
+class _Entry:
+    __slots__ = ("constructor", "qubits", "positions")
+
+    def __init__(self, constructor, qubits: Sequence[Qubit], positions: Sequence[int]):
+        self.constructor = constructor
+        self.qubits = list(qubits)
+        self.positions = list(positions)
+
+
+_FRAMES: List[List[_Entry]] = []
+
+
+def _active() -> List[_Entry]:
+    return [e for frame in _FRAMES for e in frame]
+
+
+def _insert(entries: Sequence[_Entry], qubits: Sequence[Qubit]) -> List[Qubit]:
+    out = list(qubits)
+    for e in reversed(entries):
+        for q, pos in zip(e.qubits, e.positions):
+            out.insert(pos, q)
+    return out
+
+
+def _wrap(entries: Sequence[_Entry], op: OpConstructor) -> OpConstructor:
+    for e in reversed(entries):
+        op = e.constructor(op)
+    return op
+
+
+def _release(entries: Sequence[_Entry], outputs: Sequence[Qubit]) -> List[Qubit]:
+    rest = list(outputs)
+    for e in entries:
+        e.qubits = [rest.pop(pos) for pos in e.positions]
+    return rest
+
+
+def _entry_groups(args: Sequence[Any]) -> List[List[Any]]:
+    """Split context arguments into (constructor, qubits[, positions]) groups: every argument that
+    is not iterable starts a new group, iterables join the current one as lists."""
+    groups: List[List[Any]] = []
+    for a in args:
+        if isinstance(a, collections.abc.Iterable):
+            if not groups:
+                groups.append([])
+            groups[-1].append(list(a))
+        else:
+            groups.append([a])
+    return groups
+
+
 class QubitWrapperContext:
-    """Within `with QubitWrapperContext(C, [q]):` every wrapped op is built as C(op) with q
-    inserted as its control (qip/qubit_util.py:7-128)."""
-    TOTAL_CONTEXT_QUBITS = 0
-    CONTEXT_STACK = []
+    """`with QubitWrapperContext(C, [q]):` builds every wrapped op inside as C(op) with q inserted as
+    its first input; arguments are constructors, each followed by its qubits and optionally the
+    positions they take (default 0..len-1) (qip/qubit_util.py:7-128)."""
 
     def __init__(self, *args: Any):
-        super().__init__()
-        self.context = []
+        self.context: List[_Entry] = []
         self.context_size = 0
-        buff = []
-        for arg in args:
-            if isinstance(arg, collections.abc.Iterable):
-                buff.append(list(arg))
-            else:
-                if len(buff) > 0:
-                    self.add_context(*buff)
-                    buff = []
-                buff.append(arg)
-        if len(buff) > 0:
-            self.add_context(*buff)
+        for group in _entry_groups(args):
+            self.add_context(*group)
 
     def add_context(self, constructor: OpConstructor, qubits: Sequence[Qubit], indices: Optional[Sequence[int]] = None):
-        if indices is None:
-            indices = list(range(len(qubits)))
-        self.context.append((constructor, qubits, indices))
+        positions = list(range(len(qubits))) if indices is None else list(indices)
+        if len(positions) != len(qubits):
+            raise ValueError("Size of qubits_list incorrect: {} is not required {}".format(len(positions), len(qubits)))
+        self.context.append(_Entry(constructor, qubits, positions))
         self.context_size += len(qubits)
 
     @staticmethod
     def apply_to_qubits_in_context(op_func: OpConstructor, *qubits: Qubit,
                                    op_args: Optional[Sequence[Any]] = None,
                                    op_kwargs: Optional[Mapping[str, Any]] = None) -> Tuple[Qubit, ...]:
-        qubits_list = list(qubits)
-        in_context = len(QubitWrapperContext.CONTEXT_STACK) > 0
-        if in_context:
-            qubits_list = QubitWrapperContext.put_qubits_in_context_order(*qubits_list)
-            op_func = QubitWrapperContext.make_context_constructor(op_func)
-        kwargs = dict(op_kwargs) if op_kwargs else {}
-        kwargs["nocontext"] = True
-        kwargs["nosplit"] = True
-        qubits_list = op_func(*qubits_list, **kwargs).split()
-        if in_context:
-            applied, qubits_list = QubitWrapperContext.split_context_and_circuit_qubits(*qubits_list)
-            QubitWrapperContext.set_qubits(applied)
-        return qubits_list
+        """Build op_func on `qubits` inside the active contexts; returns the op's split outputs
+        with the context qubits taken out."""
+        entries = _active()
+        if entries:
+            qubits = _insert(entries, qubits)
+            op_func = _wrap(entries, op_func)
+        outputs = op_func(*qubits, nocontext=True, nosplit=True).split()
+        return tuple(_release(entries, outputs)) if entries else outputs
 
     def put_qubits_in_local_context_order(self, *qubits: Qubit) -> Tuple[Qubit, ...]:
-        out = list(qubits)
-        for _, cq, ci in reversed(self.context):
-            for q, i in zip(cq, ci):
-                out.insert(i, q)
-        return tuple(out)
-
-    @staticmethod
-    def put_qubits_in_context_order(*qubits: Qubit) -> Tuple[Qubit, ...]:
-        out = list(qubits)
-        for _, cq, ci in reversed(QubitWrapperContext.get_context()):
-            for q, i in zip(cq, ci):
-                out.insert(i, q)
-        return tuple(out)
-
-    @staticmethod
-    def split_context_and_circuit_qubits(*qubits: Qubit) -> Tuple[Sequence[Qubit], Sequence[Qubit]]:
-        rest = list(qubits)
-        applied = []
-        for ctx in QubitWrapperContext.get_context():
-            for i in ctx[2]:
-                applied.append(rest.pop(i))
-        return applied, rest
-
-    @staticmethod
-    def make_context_constructor(op_func: OpConstructor, op_args=None, op_kwargs=None) -> OpConstructor:
-        for constructor, _, _ in reversed(QubitWrapperContext.get_context()):
-            op_func = constructor(op_func)
-        return op_func
-
-    @staticmethod
-    def get_context():
-        return flatten(QubitWrapperContext.CONTEXT_STACK)
-
-    @staticmethod
-    def set_qubits(qubits_list: Sequence[Qubit]):
-        if len(qubits_list) != QubitWrapperContext.TOTAL_CONTEXT_QUBITS:
-            raise ValueError("Size of qubits_list incorrect: {} is not required {}"
-                             .format(len(qubits_list), QubitWrapperContext.TOTAL_CONTEXT_QUBITS))
-        pos = 0
-        for context_set in QubitWrapperContext.CONTEXT_STACK:
-            for i, ctx in enumerate(context_set):
-                new_qubits = qubits_list[pos:pos + len(ctx[1])]
-                context_set[i] = (ctx[0], new_qubits, ctx[2])
-                pos += len(new_qubits)
+        """`qubits` with this context's qubits put back at their positions."""
+        return tuple(_insert(self.context, qubits))
 
     def __enter__(self):
-        QubitWrapperContext.CONTEXT_STACK.append(self.context)
-        QubitWrapperContext.TOTAL_CONTEXT_QUBITS += self.context_size
+        _FRAMES.append(self.context)
         return self
 
     def __exit__(self, exc_type, exc_val, exc_tb):
-        QubitWrapperContext.CONTEXT_STACK.pop()
-        QubitWrapperContext.TOTAL_CONTEXT_QUBITS -= self.context_size
+        _FRAMES.pop()
 
 
 class QubitOpWrapper(OpConstructor):
-    """Wraps an op class: calling it builds the node (inside any active contexts) and returns its
-    outputs split per input (qip/qubit_util.py:131-168)."""
+    """Calling it builds the wrapped node inside any active contexts and returns the node's outputs,
+    one per input (qip/qubit_util.py:131-168). Call-time keyword arguments are dropped (module
+    docstring)."""
 
     def __init__(self, op: Union[Type[Qubit], Callable[..., Qubit]], *args, **kwargs):
         super().__init__(op)
@@ -125,56 +128,49 @@ class QubitOpWrapper(OpConstructor):
         self.kwargs = kwargs
 
     def __call__(self, *inputs: Qubit, nosplit: bool = False, nocontext: bool = False,
-                 **kwargs) -> Union[Qubit, Tuple[Qubit, ...]]:
-        # the caller's keyword arguments are replaced by the wrap-time ones (module docstring)
-        kwargs = dict(self.kwargs)
-        if nocontext:
-            q = self.op(*self.args, *inputs, nocontext=nocontext, **kwargs)
-            if len(q.inputs) == 1 or nosplit:
-                return q
-            return q.split()
-        outs = QubitWrapperContext.apply_to_qubits_in_context(self, *inputs)
-        if len(outs) > 1:
-            return outs
-        if len(outs) == 1:
-            return outs[0]
-        raise ValueError("Somehow ended up with zero output qubits.")
-
-    def wrap_op_hook(self, opconstructor, consumed_inputs=None) -> Optional[OpConstructor]:
-        return None
+                 **dropped) -> Union[Qubit, Tuple[Qubit, ...]]:
+        if not nocontext:
+            outs = QubitWrapperContext.apply_to_qubits_in_context(self, *inputs)
+            if not outs:
+                raise ValueError("Somehow ended up with zero output qubits.")
+            return outs[0] if len(outs) == 1 else outs
+        node = self.op(*self.args, *inputs, nocontext=True, **self.kwargs)
+        return node if (nosplit or len(node.inputs) <= 1) else node.split()
 
     def __repr__(self):
-        if len(self.args) == 0:
-            return "OpWrap({})".format(self.op)
-        return "OpWrap({},+{} args)".format(self.op, len(self.args))
+        if self.args:
+            return "OpWrap({},+{} args)".format(self.op, len(self.args))
+        return "OpWrap({})".format(self.op)
 
 
 class QubitFuncWrapper(OpConstructor):
-    """Turns a circuit-building function into an op that C(...) can wrap: the wrappers applied to
-    it become a context around the function body (qip/qubit_util.py:171-205)."""
+    """Turns a circuit-building function into an op that wrappers such as C(...) can take: each
+    wrapper is remembered with the input positions it consumes, and a call runs the function inside
+    a context made of them (qip/qubit_util.py:171-205)."""
 
     def __init__(self, func: Callable[..., Union[Qubit, Tuple[Qubit, ...]]]):
         super().__init__(func)
-        self.wrapper_funcs = []
+        self.wrapper_funcs: List[Tuple[Any, Sequence[int]]] = []
 
     def __call__(self, *inputs: Qubit, **kwargs) -> Union[Qubit, Tuple[Qubit, ...]]:
         remaining = list(inputs)
-        taken = []
-        for opconstructor, consumed in reversed(self.wrapper_funcs):
-            consumed_qubits = reversed([remaining.pop(i) for i in reversed(consumed)])
-            taken.append((opconstructor, consumed_qubits, consumed))
-        taken = list(reversed(taken))
-        with QubitWrapperContext(*flatten(taken)) as context:
+        peeled = []
+        # the last wrapper consumed its inputs from what the earlier ones left: undo in reverse
+        for constructor, consumed in reversed(self.wrapper_funcs):
+            taken = [remaining.pop(i) for i in sorted(consumed, reverse=True)]
+            peeled.append((constructor, taken[::-1], list(consumed)))
+        ctx = QubitWrapperContext()
+        for constructor, taken, consumed in reversed(peeled):
+            ctx.add_context(constructor, taken, consumed)
+        with ctx:
             outputs = self.op(*remaining, **kwargs)
             if isinstance(outputs, Qubit):
                 outputs = (outputs,)
-            ordered = context.put_qubits_in_local_context_order(*outputs)
-            if len(ordered) == 1:
-                return ordered[0]
-            return ordered
+            ordered = ctx.put_qubits_in_local_context_order(*outputs)
+        return ordered[0] if len(ordered) == 1 else ordered
 
     def wrap_op_hook(self, opconstructor, consumed_inputs=None) -> OpConstructor:
-        self.wrapper_funcs.append((opconstructor, consumed_inputs))
+        self.wrapper_funcs.append((opconstructor, list(consumed_inputs or [])))
         return self
 
     @staticmethod

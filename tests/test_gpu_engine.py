@@ -217,12 +217,47 @@ def test_func_apply_callable_and_cache():
     be = gpu_from(psi, n)
     f = lambda x: (3 * x + 1) % 8  # noqa: E731
     be.func_apply([0, 1, 2], [3, 4, 5], f)
-    be.func_apply([0, 1, 2], [3, 4, 5], f)      # second call uses the cached device table
+    be.func_apply([0, 1, 2], [3, 4, 5], f)
     ref = numpy.zeros_like(psi)
     oracle.func_apply([0, 1, 2], [3, 4, 5], f, psi, n, ref)
     ref2 = numpy.zeros_like(psi)
     oracle.func_apply([0, 1, 2], [3, 4, 5], f, ref, n, ref2)
     assert numpy.array_equal(be.get_state(), ref2)
+
+
+def test_func_apply_follows_a_changing_closure():
+    """A closure whose values change between two func_apply calls: each call uses the values the
+    function has at that call, as the reference's per-call tabulation does (func_apply.pyx:66-69).
+    With cache_func_tables the first table is kept, which is the documented opt-in."""
+    n = 7
+    psi = rand_state(n)
+    shift = [1]
+    f = lambda x: (x + shift[0]) % 8  # noqa: E731
+    want = psi
+    be = gpu_from(psi, n)
+    for s in (1, 5):
+        shift[0] = s
+        be.func_apply([0, 1, 2, 3], [4, 5, 6], f)
+        out = numpy.zeros_like(want)
+        oracle.func_apply([0, 1, 2, 3], [4, 5, 6], f, want, n, out)
+        want = out
+    assert numpy.array_equal(be.get_state(), want)
+    be.close()
+
+    class Cached(GpuBackend):
+        cache_func_tables = True
+    c = Cached.make_state(n, [list(range(n))], [psi])
+    shift[0] = 1
+    c.func_apply([0, 1, 2, 3], [4, 5, 6], f)
+    shift[0] = 5
+    c.func_apply([0, 1, 2, 3], [4, 5, 6], f)           # the table of shift 1 again
+    shift[0] = 1
+    once = numpy.zeros_like(psi)
+    oracle.func_apply([0, 1, 2, 3], [4, 5, 6], f, psi, n, once)
+    twice = numpy.zeros_like(psi)
+    oracle.func_apply([0, 1, 2, 3], [4, 5, 6], f, once, n, twice)
+    assert numpy.array_equal(c.get_state(), twice)
+    c.close()
 
 
 @pytest.mark.parametrize("trial", range(6))
