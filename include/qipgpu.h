@@ -40,8 +40,8 @@ enum {
     QIPGPU_EUNSUP = -5    /* operator outside what the engine applies: NotImplementedError */
 };
 
-/* Largest number of (non-control) target qubits of one dense operator. */
-#define QIPGPU_MAX_DENSE 6
+/* Largest number of (non-control) target qubits of one dense or diagonal operator. */
+#define QIPGPU_MAX_DENSE 12
 /* Largest number of qubits of a state. */
 #define QIPGPU_MAX_QUBITS 34
 
@@ -75,10 +75,24 @@ int qipgpu_add_range(qipgpu_state *s, int64_t start, int64_t end, const void *ho
  * Applies, in place, the operator that is `mat` on the `targets` qubits, conditioned on every
  * `controls` qubit being 1 (the C_MAT structure diag(I, M), kronprod.pyx:215-225). The engine
  * finds further control qubits and diagonal structure inside `mat` itself, so a caller may pass
- * a whole controlled operator as one dense matrix. k <= 12 before that reduction and at most
- * QIPGPU_MAX_DENSE targets after it. */
+ * a whole controlled operator as one dense matrix. k <= QIPGPU_MAX_DENSE. Up to 3 targets are
+ * applied from registers, 4..6 through LDS tiles, 7..12 as an FP64 GEMM out of place (a temporary
+ * of 2^(n - controls) amplitudes). */
 int qipgpu_apply_matrix(qipgpu_state *s, int32_t k, const int32_t *targets, const double *mat,
                         int32_t n_ctrl, const int32_t *controls);
+
+/* Any kronselect_dot descriptor component, by the reference's own formula (cdot_loop,
+ * qip/ext/kronprod.pyx:137-197): out[r] = sum over the 2^u columns c that agree with r outside
+ * the union (u qubits) of all groups' indices, in increasing order, of
+ * prod_g M_g[r|_g, c|_g] * in[c], with M_g's entries by calc_mat_entry (kronprod.pyx:203-235).
+ * Group g has group_len[g] indices (concatenated in group_idx, first = MSB); its first levels[g]
+ * indices are C_MAT levels (controls); the inner matrix is kinds[g] = 1 (NUMPY: a dense
+ * 2^(len - levels) square matrix, row-major complex128, the inner matrices concatenated in `mats`)
+ * or 3 (SWAP_MAT of block width params[g]). Groups may share indices. Costs 2^(n+u) entry
+ * products and a temporary state (out of place, as the reference); u <= 30. For descriptors
+ * no specialised sweep covers. */
+int qipgpu_apply_kron(qipgpu_state *s, int32_t n_groups, const int32_t *group_len, const int32_t *group_idx,
+                      const int32_t *levels, const int32_t *kinds, const int32_t *params, const double *mats);
 
 /* Product of single-qubit operators, mats[4j..4j+3] (row-major 2x2 complex128) on targets[j], for
  * k <= QIPGPU_MAX_PRODUCT distinct qubits, in one sweep: the uncontrolled single-qubit factors of

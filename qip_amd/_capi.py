@@ -38,6 +38,7 @@ SIGNATURES = {
     "qipgpu_apply_matrix": (ctypes.c_int, [_P, _I32, _P, _P, _I32, _P]),
     "qipgpu_apply_product": (ctypes.c_int, [_P, _I32, _P, _P]),
     "qipgpu_apply_diagonal": (ctypes.c_int, [_P, _I32, _P, _P, _I32, _P]),
+    "qipgpu_apply_kron": (ctypes.c_int, [_P, _I32, _P, _P, _P, _P, _P, _P]),
     "qipgpu_func_apply": (ctypes.c_int, [_P, _I32, _P, _I32, _P, _P]),
     "qipgpu_table_create": (ctypes.c_int, [_I32, _P, _PP]),
     "qipgpu_table_destroy": (ctypes.c_int, [_P]),

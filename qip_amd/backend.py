@@ -348,8 +348,18 @@ class GpuBackend(StateType):
         for call in planner.plan(mats):
             if isinstance(call, planner.ProductCall):
                 self.apply_product(call.targets, call.mats)
+            elif isinstance(call, planner.GeneralCall):
+                self.apply_general(call)
             else:
                 self.apply_matrix(call.targets, call.mat, call.controls)
+
+    def apply_general(self, call: "planner.GeneralCall") -> None:
+        """A descriptor component by the reference's column-sum formula (qipgpu_apply_kron)."""
+        lens, idx, levels, kinds, params, mats = call.packed()
+        a = [_capi.i32(v) for v in (lens, idx, levels, kinds, params)]
+        m = numpy.ascontiguousarray(mats, dtype=numpy.complex128)
+        _capi.check(self._lib.qipgpu_apply_kron(self._h, len(lens), *[_capi.ptr(v) for v in a],
+                                                _capi.ptr(m.view(numpy.float64))), "apply_kron")
 
     def apply_product(self, targets: Sequence[int], mats: Sequence[numpy.ndarray]) -> None:
         """Single-qubit operators mats[j] on distinct qubits targets[j], up to 12 per sweep

@@ -97,6 +97,9 @@ def descriptor_gates(mats) -> List[Tuple[Tuple[int, ...], numpy.ndarray]]:
             for q, m in zip(call.targets, call.mats):
                 out.append(((q,), numpy.ascontiguousarray(m, dtype=numpy.complex128)))
             continue
+        if isinstance(call, planner.GeneralCall):
+            raise NotImplementedError("batched gates act on at most {} qubits, controls included "
+                                      "(got {})".format(MAX_GATE_QUBITS, len(call.targets)))
         gq, G = controlled_dense(call)
         if len(gq) > MAX_GATE_QUBITS:
             raise NotImplementedError("batched gates act on at most {} qubits, controls included "

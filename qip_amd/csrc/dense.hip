@@ -1,0 +1,570 @@
+// dense.hip -- dense operators on 4..12 target qubits, and any kronselect_dot descriptor component
+// (qip/ext/kronprod.pyx:41-235) that no specialised sweep covers.
+//
+// Terms: a dense operator U (D x D, D = 2^K, row-major, first target = MSB of the row index) acts
+// on the state's "bases": a base is a state index with every target bit 0 and every control bit 1;
+// the D amplitudes base + off[j] form one closed group, and out[base + off[i]] =
+// sum_j U[i][j] * in[base + off[j]]. That is the matrix product Y = U X with X[j][b] the amplitudes
+// of base b. FP64 arithmetic for both state precisions, rounded once on store.
+//
+//   k_dense_tile<R, K>   K = 4..6, in place. A persistent workgroup keeps U in LDS; per tile of B
+//                        bases it stages X (D x B = 4096 amplitudes, 64 KB in FP64) in LDS, each
+//                        thread forms 4 rows x 4 bases of Y in registers, and the next tile's
+//                        amplitudes are loaded into registers while this one is multiplied. A tile
+//                        holds every amplitude of its bases before any is written, so in place is safe.
+//   k_dense_gemm<R>      K = 7..12, out of place: a 64 x 64 output tile per workgroup over 16-column
+//                        steps of j through LDS, into a compact [D][nbase] buffer; k_dense_scatter
+//                        writes it back to the bases' positions.
+//   k_kron_general<R>    the reference's own formula, out of place: for every output row r, the sum
+//                        over the 2^u columns c that agree with r outside the union of the groups'
+//                        qubits, in increasing order, of prod_g entry_g(r|_g, c|_g) * in[c] (the
+//                        product stops at a zero factor), with entry_g following C_MAT levels and a
+//                        NUMPY or SWAP_MAT inner matrix (calc_mat_entry, kronprod.pyx:203-235).
+#include "engine.h"
+
+#include <algorithm>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+using namespace qipgpu;
+
+namespace {
+
+template <typename R>
+__device__ __forceinline__ cx<double> to_d(cx<R> v) {
+    cx<double> o;
+    o.re = (double)v.re;
+    o.im = (double)v.im;
+    return o;
+}
+
+template <typename R>
+__device__ __forceinline__ cx<R> from_d(cx<double> v) {
+    cx<R> o;
+    o.re = (R)v.re;
+    o.im = (R)v.im;
+    return o;
+}
+
+__device__ __forceinline__ void mac(cx<double> &acc, const cx<double> a, const cx<double> b) {
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(-a.im, b.im, acc.re);
+    acc.im = fma(a.re, b.im, acc.im);
+    acc.im = fma(a.im, b.re, acc.im);
+}
+
+__device__ __forceinline__ uint64_t base_of(uint64_t g, const PosList &pl, uint64_t ctrl_mask) {
+    return insert_zeros(g, pl) | ctrl_mask;
+}
+
+// ---- K = 4..6: LDS tiles, in place ---------------------------------------------------------------
+
+constexpr int kTileThreads = 256;
+constexpr int kTileElems = 4096;        // D x B amplitudes per tile
+
+template <int K>
+struct TileShape {
+    static constexpr int D = 1 << K;
+    static constexpr int RB = 4;                       // rows per thread
+    static constexpr int BB = 4;                       // bases per thread
+    static constexpr int RG = D / RB;                  // row groups
+    static constexpr int BG = kTileThreads / RG;       // base groups
+    static constexpr int B = BG * BB;                  // bases per tile
+    static constexpr int LOADS = kTileElems / kTileThreads;
+    static_assert(D * B == kTileElems, "tile shape");
+};
+
+template <typename R, int K>
+__global__ __launch_bounds__(kTileThreads) void k_dense_tile(cx<R> *amps, uint64_t nbase, uint64_t ctrl_mask,
+                                                             PosList pl, const uint64_t *__restrict__ off_g,
+                                                             const cx<double> *__restrict__ U_g, uint64_t ntiles) {
+    using S = TileShape<K>;
+    constexpr int D = S::D, B = S::B;
+    __shared__ cx<double> sU[D * D];
+    __shared__ cx<double> sX[kTileElems];
+    __shared__ uint64_t sOff[D];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < D * D; e += kTileThreads) sU[e] = U_g[e];
+    for (int e = tid; e < D; e += kTileThreads) sOff[e] = off_g[e];
+    __syncthreads();
+
+    // load role: one base (b_ld) and rows j = j0 + i * (256 / B)
+    const int b_ld = tid % B;
+    const int j0 = tid / B;
+    constexpr int JSTEP = kTileThreads / B;
+    // compute role: rows rg + RG * r, bases bg + BG * q
+    const int rg = tid / S::BG;
+    const int bg = tid % S::BG;
+
+    cx<R> pre[S::LOADS];
+    auto load_tile = [&](uint64_t t) {
+        const uint64_t g = t * B + b_ld;
+        if (g < nbase) {
+            const uint64_t base = base_of(g, pl, ctrl_mask);
+#pragma unroll
+            for (int i = 0; i < S::LOADS; ++i) pre[i] = amps[base + sOff[j0 + i * JSTEP]];
+        }
+    };
+    uint64_t t = blockIdx.x;
+    if (t < ntiles) load_tile(t);
+    for (; t < ntiles; t += gridDim.x) {
+        __syncthreads();                       // the previous tile's products have read sX
+#pragma unroll
+        for (int i = 0; i < S::LOADS; ++i) sX[(j0 + i * JSTEP) * B + b_ld] = to_d(pre[i]);
+        __syncthreads();
+        if (t + gridDim.x < ntiles) load_tile(t + gridDim.x);   // in flight during the products
+        cx<double> acc[S::RB][S::BB];
+#pragma unroll
+        for (int r = 0; r < S::RB; ++r)
+#pragma unroll
+            for (int q = 0; q < S::BB; ++q) acc[r][q].re = acc[r][q].im = 0.0;
+        for (int j = 0; j < D; ++j) {
+            cx<double> u[S::RB], x[S::BB];
+#pragma unroll
+            for (int r = 0; r < S::RB; ++r) u[r] = sU[(rg + S::RG * r) * D + j];
+#pragma unroll
+            for (int q = 0; q < S::BB; ++q) x[q] = sX[j * B + bg + S::BG * q];
+#pragma unroll
+            for (int r = 0; r < S::RB; ++r)
+#pragma unroll
+                for (int q = 0; q < S::BB; ++q) mac(acc[r][q], u[r], x[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < S::BB; ++q) {
+            const uint64_t g = t * B + bg + S::BG * q;
+            if (g < nbase) {
+                const uint64_t base = base_of(g, pl, ctrl_mask);
+#pragma unroll
+                for (int r = 0; r < S::RB; ++r) amps[base + sOff[rg + S::RG * r]] = from_d<R>(acc[r][q]);
+            }
+        }
+    }
+}
+
+// ---- K = 7..12: tiled GEMM out of place ---------------------------------------------------------
+
+constexpr int kGemmTile = 64;
+constexpr int kGemmStep = 16;
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_dense_gemm(const cx<R> *__restrict__ amps, cx<R> *__restrict__ out,
+                                                    uint64_t nbase, int D, uint64_t ctrl_mask, PosList pl,
+                                                    const uint64_t *__restrict__ off,
+                                                    const cx<double> *__restrict__ U) {
+    __shared__ cx<double> sU[kGemmTile][kGemmStep + 1];
+    __shared__ cx<double> sX[kGemmStep][kGemmTile];
+    const int tid = threadIdx.x;
+    const uint64_t b0 = (uint64_t)blockIdx.x * kGemmTile;   // first base of the tile
+    const int i0 = blockIdx.y * kGemmTile;                    // first row of the tile
+    // loader roles: X column b_ld with steps jj = tid / 64 + 4 s; U row u_row, columns u_col + 4 s
+    const int b_ld = tid % kGemmTile;
+    const int jj0 = tid / kGemmTile;
+    const bool have_base = b0 + b_ld < nbase;
+    const uint64_t base = have_base ? base_of(b0 + b_ld, pl, ctrl_mask) : 0;
+    const int u_row = tid / 4, u_col = tid % 4;
+    // compute roles: rows ty * 4 + r, bases tx + 16 q
+    const int ty = tid / 16, tx = tid % 16;
+    cx<double> acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[r][q].re = acc[r][q].im = 0.0;
+    for (int jc = 0; jc < D; jc += kGemmStep) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int jj = jj0 + 4 * s;
+            cx<double> v;
+            v.re = v.im = 0.0;
+            if (have_base) v = to_d(amps[base + off[jc + jj]]);
+            sX[jj][b_ld] = v;
+            sU[u_row][u_col + 4 * s] = U[(size_t)(i0 + u_row) * D + jc + u_col + 4 * s];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int jj = 0; jj < kGemmStep; ++jj) {
+            cx<double> u[4], x[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) u[r] = sU[ty * 4 + r][jj];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) x[q] = sX[jj][tx + 16 * q];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) mac(acc[r][q], u[r], x[q]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint64_t b = b0 + tx + 16 * q;
+        if (b < nbase) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) out[(size_t)(i0 + ty * 4 + r) * nbase + b] = from_d<R>(acc[r][q]);
+        }
+    }
+}
+
+// out[i][b] (compact) -> amps[base(b) + off[i]]
+template <typename R>
+__global__ __launch_bounds__(256) void k_dense_scatter(cx<R> *amps, const cx<R> *__restrict__ out, uint64_t nbase,
+                                                       int D, uint64_t ctrl_mask, PosList pl,
+                                                       const uint64_t *__restrict__ off) {
+    const uint64_t total = nbase * (uint64_t)D;
+    for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (uint64_t)gridDim.x * 256) {
+        const uint64_t i = e / nbase, b = e % nbase;
+        amps[base_of(b, pl, ctrl_mask) + off[i]] = out[e];
+    }
+}
+
+// ---- any descriptor component: the reference's column sum ---------------------------------------
+
+struct KronGroupDev {
+    int32_t len;        // indices in the group
+    int32_t levels;     // C_MAT levels: the first `levels` indices are controls
+    int32_t kind;       // inner matrix: 1 NUMPY (dense), 3 SWAP_MAT
+    int32_t param;      // SWAP_MAT block width
+    int32_t pos_at;     // first entry in the positions array (state bits, first = MSB of the index)
+    int32_t pad_;
+    uint64_t mat_at;    // first complex entry of a NUMPY inner matrix
+};
+
+struct KronArgs {
+    int32_t n_groups;
+    int32_t u;                  // union size
+    uint8_t upos[QIPGPU_MAX_QUBITS];   // union state bits, enumeration bit u-1-j -> upos[j]
+    uint64_t umask;
+};
+
+__device__ __forceinline__ cx<double> group_entry(const KronGroupDev &g, const int32_t *__restrict__ pos,
+                                                 const cx<double> *__restrict__ mats, uint64_t row, uint64_t col) {
+    uint64_t ri = 0, ci = 0;
+    for (int m = 0; m < g.len; ++m) {
+        const int p = pos[g.pos_at + m];
+        ri = (ri << 1) | ((row >> p) & 1ull);
+        ci = (ci << 1) | ((col >> p) & 1ull);
+    }
+    cx<double> one, zero;
+    one.re = 1.0;
+    one.im = 0.0;
+    zero.re = zero.im = 0.0;
+    int len = g.len;
+    for (int lv = 0; lv < g.levels; ++lv) {
+        const uint64_t top = 1ull << (len - 1);
+        const bool rb = (ri & top) != 0, cb = (ci & top) != 0;
+        if (rb != cb) return zero;
+        ri &= top - 1;
+        ci &= top - 1;
+        --len;
+        if (!rb) return ri == ci ? one : zero;
+    }
+    if (g.kind == 3) {
+        const uint64_t half = 1ull << g.param;
+        const uint64_t partner = (ri % half) * half + ri / half;
+        return partner == ci ? one : zero;
+    }
+    return mats[g.mat_at + (ri << len) + ci];
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_kron_general(const cx<R> *__restrict__ in, cx<R> *__restrict__ out,
+                                                      uint64_t len, KronArgs a,
+                                                      const KronGroupDev *__restrict__ groups,
+                                                      const int32_t *__restrict__ pos,
+                                                      const cx<double> *__restrict__ mats) {
+    const uint64_t cols = 1ull << a.u;
+    for (uint64_t row = (uint64_t)blockIdx.x * 256 + threadIdx.x; row < len; row += (uint64_t)gridDim.x * 256) {
+        cx<double> s;
+        s.re = s.im = 0.0;
+        const uint64_t keep = row & ~a.umask;
+        for (uint64_t e = 0; e < cols; ++e) {
+            uint64_t col = keep;
+            for (int j = 0; j < a.u; ++j) col |= ((e >> (a.u - 1 - j)) & 1ull) << a.upos[j];
+            cx<double> p;
+            p.re = 1.0;
+            p.im = 0.0;
+            for (int gi = 0; gi < a.n_groups; ++gi) {
+                const cx<double> m = group_entry(groups[gi], pos, mats, row, col);
+                cx<double> np;
+                np.re = p.re * m.re - p.im * m.im;
+                np.im = p.re * m.im + p.im * m.re;
+                p = np;
+                if (p.re == 0.0 && p.im == 0.0) break;
+            }
+            const cx<double> v = to_d(in[col]);
+            s.re += p.re * v.re - p.im * v.im;
+            s.im += p.re * v.im + p.im * v.re;
+        }
+        out[row] = from_d<R>(s);
+    }
+}
+
+// Diagonal on up to QIPGPU_MAX_DENSE targets, entries in device memory; one item per amplitude of the
+// controlled subspace.
+template <typename R>
+__global__ __launch_bounds__(256) void k_diag_table(cx<R> *amps, uint64_t count, uint64_t ctrl_mask, PosList pl,
+                                                    int K, const uint8_t *__restrict__ tpos_g,
+                                                    const cx<double> *__restrict__ d) {
+    __shared__ uint8_t tpos[QIPGPU_MAX_DENSE];
+    if (threadIdx.x < K) tpos[threadIdx.x] = tpos_g[threadIdx.x];
+    __syncthreads();
+    for (uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x; g < count; g += (uint64_t)gridDim.x * 256) {
+        const uint64_t i = base_of(g, pl, ctrl_mask);
+        uint32_t j = 0;
+        for (int b = 0; b < K; ++b) j = (j << 1) | (uint32_t)((i >> tpos[b]) & 1ull);
+        const cx<double> v = to_d(amps[i]), f = d[j];
+        cx<double> o;
+        o.re = f.re * v.re - f.im * v.im;
+        o.im = f.re * v.im + f.im * v.re;
+        amps[i] = from_d<R>(o);
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_copy_amps(cx<R> *dst, const cx<R> *__restrict__ src, uint64_t len) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < len; i += (uint64_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+// ---- host side -----------------------------------------------------------------------------------
+
+// Temporary device buffer of one call: freed on every exit path, after the state's stream has
+// finished with it.
+struct StreamBuffer {
+    hipStream_t stream;
+    void *p = nullptr;
+    explicit StreamBuffer(hipStream_t st) : stream(st) {}
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes); }
+    ~StreamBuffer() {
+        if (p) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipFree(p);
+        }
+    }
+};
+
+std::vector<uint64_t> offsets(const std::vector<int> &tbits) {
+    const int K = (int)tbits.size();
+    std::vector<uint64_t> off((size_t)1 << K);
+    for (size_t j = 0; j < off.size(); ++j) {
+        uint64_t o = 0;
+        for (int t = 0; t < K; ++t)
+            if ((j >> (K - 1 - t)) & 1) o |= 1ull << tbits[t];
+        off[j] = o;
+    }
+    return off;
+}
+
+PosList poslist(std::vector<int> pos) {
+    std::sort(pos.begin(), pos.end());
+    PosList pl;
+    pl.count = (int32_t)pos.size();
+    for (size_t i = 0; i < pos.size(); ++i) pl.p[i] = (uint8_t)pos[i];
+    return pl;
+}
+
+// Upload [off | U] (U as interleaved FP64) to a stream-ordered buffer; the host data is consumed
+// before return.
+int upload_operator(qipgpu_state *s, StreamBuffer &buf, const std::vector<uint64_t> &off,
+                    const std::complex<double> *U, size_t D) {
+    const size_t ob = off.size() * sizeof(uint64_t), ub = D * D * sizeof(cx<double>);
+    QIP_HIP(buf.alloc(ob + ub));
+    QIP_HIP(hipMemcpyAsync(buf.p, off.data(), ob, hipMemcpyHostToDevice, s->stream));
+    QIP_HIP(hipMemcpyAsync((char *)buf.p + ob, U, ub, hipMemcpyHostToDevice, s->stream));
+    QIP_HIP(hipStreamSynchronize(s->stream));   // the host copies may be pageable
+    return QIPGPU_OK;
+}
+
+template <typename R, int K>
+int launch_tile(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mask, const std::vector<int> &allpos,
+                const std::complex<double> *U) {
+    constexpr int D = 1 << K;
+    const std::vector<uint64_t> off = offsets(tbits);
+    std::vector<char> host(sizeof(uint64_t) * D + sizeof(cx<double>) * D * D);
+    std::memcpy(host.data(), off.data(), sizeof(uint64_t) * D);
+    std::memcpy(host.data() + sizeof(uint64_t) * D, U, sizeof(cx<double>) * D * D);
+    void *d = nullptr;
+    int rc = ring_upload(s, host.data(), host.size(), &d);
+    if (rc) return rc;
+    const uint64_t nbase = s->len >> allpos.size();
+    const uint64_t ntiles = (nbase + TileShape<K>::B - 1) / TileShape<K>::B;
+    const int per_cu = K == 6 ? 1 : 2;       // LDS: 128 KB at K = 6, 80 KB and 68 KB below
+    const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)device_cus() * per_cu);
+    hipLaunchKernelGGL((k_dense_tile<R, K>), dim3((uint32_t)grid), dim3(kTileThreads), 0, s->stream,
+                       (cx<R> *)s->amps, nbase, ctrl_mask, poslist(allpos), (const uint64_t *)d,
+                       (const cx<double> *)((char *)d + sizeof(uint64_t) * D), ntiles);
+    QIP_LAUNCHED();
+    return QIPGPU_OK;
+}
+
+template <typename R>
+int launch_gemm(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mask, const std::vector<int> &allpos,
+                const std::complex<double> *U) {
+    const int K = (int)tbits.size();
+    const int D = 1 << K;
+    const std::vector<uint64_t> off = offsets(tbits);
+    StreamBuffer op(s->stream), tmp(s->stream);
+    int rc = upload_operator(s, op, off, U, (size_t)D);
+    if (rc) return rc;
+    const uint64_t nbase = s->len >> allpos.size();
+    QIP_HIP(tmp.alloc(nbase * (size_t)D * sizeof(cx<R>)));
+    const PosList pl = poslist(allpos);
+    const uint64_t *doff = (const uint64_t *)op.p;
+    const cx<double> *dU = (const cx<double> *)((char *)op.p + off.size() * sizeof(uint64_t));
+    const dim3 grid((uint32_t)((nbase + kGemmTile - 1) / kGemmTile), (uint32_t)(D / kGemmTile));
+    hipLaunchKernelGGL((k_dense_gemm<R>), grid, dim3(256), 0, s->stream, (const cx<R> *)s->amps, (cx<R> *)tmp.p,
+                       nbase, D, ctrl_mask, pl, doff, dU);
+    QIP_LAUNCHED();
+    const uint64_t total = nbase * (uint64_t)D;
+    const uint32_t sb = (uint32_t)std::min<uint64_t>((total + 255) / 256, (uint64_t)device_cus() * 64);
+    hipLaunchKernelGGL((k_dense_scatter<R>), dim3(sb), dim3(256), 0, s->stream, (cx<R> *)s->amps,
+                       (const cx<R> *)tmp.p, nbase, D, ctrl_mask, pl, doff);
+    QIP_LAUNCHED();
+    return QIPGPU_OK;
+}
+
+template <typename R>
+int kron_general(qipgpu_state *s, int32_t n_groups, const int32_t *group_len, const int32_t *group_idx,
+                 const int32_t *levels, const int32_t *kinds, const int32_t *params, const double *mats) {
+    const int n = s->n;
+    std::vector<KronGroupDev> groups((size_t)n_groups);
+    std::vector<int32_t> pos;
+    std::vector<int32_t> uq;
+    size_t mat_entries = 0;
+    int32_t at = 0;
+    for (int g = 0; g < n_groups; ++g) {
+        KronGroupDev &G = groups[g];
+        G.len = group_len[g];
+        G.levels = levels[g];
+        G.kind = kinds[g];
+        G.param = params[g];
+        G.pos_at = (int32_t)pos.size();
+        G.pad_ = 0;
+        G.mat_at = mat_entries;
+        for (int m = 0; m < G.len; ++m) {
+            const int32_t q = group_idx[at + m];
+            pos.push_back(n - 1 - q);
+            if (std::find(uq.begin(), uq.end(), q) == uq.end()) uq.push_back(q);
+        }
+        at += G.len;
+        if (G.kind == 1) {
+            const int inner = G.len - G.levels;
+            mat_entries += (size_t)1 << (2 * inner);
+        }
+    }
+    std::sort(uq.begin(), uq.end());    // ascending qubits: the enumeration's MSB is the smallest qubit
+    KronArgs a;
+    a.n_groups = n_groups;
+    a.u = (int32_t)uq.size();
+    a.umask = 0;
+    for (size_t j = 0; j < uq.size(); ++j) {
+        a.upos[j] = (uint8_t)(n - 1 - uq[j]);
+        a.umask |= 1ull << (n - 1 - uq[j]);
+    }
+    const size_t gb = groups.size() * sizeof(KronGroupDev), pb = std::max<size_t>(pos.size(), 1) * sizeof(int32_t);
+    const size_t mb = std::max<size_t>(mat_entries, 1) * sizeof(cx<double>);
+    StreamBuffer desc(s->stream), tmp(s->stream);
+    QIP_HIP(desc.alloc(gb + pb + mb));
+    char *d = (char *)desc.p;
+    QIP_HIP(hipMemcpyAsync(d, groups.data(), gb, hipMemcpyHostToDevice, s->stream));
+    if (!pos.empty()) QIP_HIP(hipMemcpyAsync(d + gb, pos.data(), pos.size() * sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
+    if (mat_entries) QIP_HIP(hipMemcpyAsync(d + gb + pb, mats, mat_entries * sizeof(cx<double>), hipMemcpyHostToDevice, s->stream));
+    QIP_HIP(tmp.alloc(s->len * sizeof(cx<R>)));
+    const uint32_t nb = (uint32_t)std::min<uint64_t>((s->len + 255) / 256, (uint64_t)device_cus() * 16);
+    hipLaunchKernelGGL((k_kron_general<R>), dim3(nb), dim3(256), 0, s->stream, (const cx<R> *)s->amps,
+                       (cx<R> *)tmp.p, s->len, a, (const KronGroupDev *)d, (const int32_t *)(d + gb),
+                       (const cx<double> *)(d + gb + pb));
+    QIP_LAUNCHED();
+    const uint32_t cb = (uint32_t)std::min<uint64_t>((s->len + 255) / 256, (uint64_t)device_cus() * 64);
+    hipLaunchKernelGGL((k_copy_amps<R>), dim3(cb), dim3(256), 0, s->stream, (cx<R> *)s->amps, (const cx<R> *)tmp.p, s->len);
+    QIP_LAUNCHED();
+    QIP_HIP(hipStreamSynchronize(s->stream));   // the host descriptor arrays may be pageable
+    return QIPGPU_OK;
+}
+
+template <typename R>
+int launch_diag_table(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mask, const std::vector<int> &cpos,
+                      const std::complex<double> *d) {
+    const int K = (int)tbits.size();
+    const size_t D = (size_t)1 << K;
+    std::vector<char> host(D * sizeof(cx<double>) + 64);
+    std::memcpy(host.data(), d, D * sizeof(cx<double>));
+    uint8_t *tp = (uint8_t *)(host.data() + D * sizeof(cx<double>));
+    for (int b = 0; b < K; ++b) tp[b] = (uint8_t)tbits[b];
+    StreamBuffer buf(s->stream);
+    QIP_HIP(buf.alloc(host.size()));
+    QIP_HIP(hipMemcpyAsync(buf.p, host.data(), host.size(), hipMemcpyHostToDevice, s->stream));
+    const uint64_t count = s->len >> cpos.size();
+    const uint32_t nb = (uint32_t)std::min<uint64_t>((count + 255) / 256, (uint64_t)device_cus() * 64);
+    hipLaunchKernelGGL((k_diag_table<R>), dim3(nb), dim3(256), 0, s->stream, (cx<R> *)s->amps, count, ctrl_mask,
+                       poslist(cpos), K, (const uint8_t *)(host.size() ? (char *)buf.p + D * sizeof(cx<double>) : nullptr),
+                       (const cx<double> *)buf.p);
+    QIP_LAUNCHED();
+    QIP_HIP(hipStreamSynchronize(s->stream));   // the host copy may be pageable
+    return QIPGPU_OK;
+}
+
+}  // namespace
+
+namespace qipgpu {
+
+int diag_large(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mask, const std::vector<int> &cpos,
+               const std::complex<double> *d) {
+    if ((int)tbits.size() > QIPGPU_MAX_DENSE)
+        return set_error(QIPGPU_EUNSUP, "diagonal operator on %d qubits exceeds %d", (int)tbits.size(), QIPGPU_MAX_DENSE);
+    return s->dtype == QIPGPU_C128 ? launch_diag_table<double>(s, tbits, ctrl_mask, cpos, d)
+                                   : launch_diag_table<float>(s, tbits, ctrl_mask, cpos, d);
+}
+
+int dense_large(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mask, const std::vector<int> &allpos,
+                const std::complex<double> *U) {
+    const int K = (int)tbits.size();
+    const bool c128 = s->dtype == QIPGPU_C128;
+    switch (K) {
+        case 4: return c128 ? launch_tile<double, 4>(s, tbits, ctrl_mask, allpos, U) : launch_tile<float, 4>(s, tbits, ctrl_mask, allpos, U);
+        case 5: return c128 ? launch_tile<double, 5>(s, tbits, ctrl_mask, allpos, U) : launch_tile<float, 5>(s, tbits, ctrl_mask, allpos, U);
+        case 6: return c128 ? launch_tile<double, 6>(s, tbits, ctrl_mask, allpos, U) : launch_tile<float, 6>(s, tbits, ctrl_mask, allpos, U);
+        default: break;
+    }
+    if (K >= 7 && K <= QIPGPU_MAX_DENSE)
+        return c128 ? launch_gemm<double>(s, tbits, ctrl_mask, allpos, U) : launch_gemm<float>(s, tbits, ctrl_mask, allpos, U);
+    return set_error(QIPGPU_EUNSUP, "dense operator on %d target qubits exceeds %d", K, QIPGPU_MAX_DENSE);
+}
+
+}  // namespace qipgpu
+
+extern "C" {
+
+int qipgpu_apply_kron(qipgpu_state *s, int32_t n_groups, const int32_t *group_len, const int32_t *group_idx,
+                      const int32_t *levels, const int32_t *kinds, const int32_t *params, const double *mats) {
+    if (!s) return set_error(QIPGPU_EINVAL, "null state");
+    if (n_groups < 1) return set_error(QIPGPU_EINVAL, "a descriptor needs at least one group");
+    std::vector<int32_t> all;
+    int32_t at = 0;
+    for (int g = 0; g < n_groups; ++g) {
+        const int32_t L = group_len[g];
+        if (L < 1 || L > s->n) return set_error(QIPGPU_EINVAL, "group %d has %d indices", g, L);
+        if (levels[g] < 0 || levels[g] > L) return set_error(QIPGPU_EINVAL, "group %d: %d control levels", g, levels[g]);
+        const int inner = L - levels[g];
+        if (kinds[g] == 1) {
+            if (inner > 14) return set_error(QIPGPU_EUNSUP, "group %d: a %d-qubit dense matrix", g, inner);
+        } else if (kinds[g] == 3) {
+            if (params[g] < 0 || 2 * params[g] != inner)
+                return set_error(QIPGPU_EINVAL, "group %d: SwapMat(%d) on %d indices", g, params[g], inner);
+        } else {
+            return set_error(QIPGPU_EINVAL, "group %d: matrix kind %d", g, kinds[g]);
+        }
+        // indices within one group must differ; groups may share indices (kronprod.pyx:140-141)
+        int rc = check_qubits(s, L, group_idx + at, "group indices");
+        if (rc) return rc;
+        for (int m = 0; m < L; ++m)
+            if (std::find(all.begin(), all.end(), group_idx[at + m]) == all.end()) all.push_back(group_idx[at + m]);
+        at += L;
+    }
+    if ((int)all.size() > 30) return set_error(QIPGPU_EUNSUP, "descriptor spans %d qubits", (int)all.size());
+    int rc = flush_recorded(s);
+    if (rc) return rc;
+    return s->dtype == QIPGPU_C128 ? kron_general<double>(s, n_groups, group_len, group_idx, levels, kinds, params, mats)
+                                   : kron_general<float>(s, n_groups, group_len, group_idx, levels, kinds, params, mats);
+}
+
+}  // extern "C"

@@ -5,8 +5,10 @@
 
 #include <cstdarg>
 #include <cstdint>
+#include <complex>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "qipgpu.h"
 
@@ -121,5 +123,14 @@ void recorder_free(qipgpu_state *s);
 int settle(qipgpu_state *s);
 // Sum of |psi|^2 in FP64, result left on the device at `dst` (enqueued on s->stream).
 int norm_to_device(qipgpu_state *s, double *dst);
+// Dense operator U (2^K x 2^K, row-major) on 4 <= K <= QIPGPU_MAX_DENSE targets at state bits `tbits`
+// (first = MSB of U's index), where the bits of ctrl_mask are 1; `allpos` = control and target
+// bits (dense.hip).
+int dense_large(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mask, const std::vector<int> &allpos,
+                const std::complex<double> *U);
+// Diagonal d (2^K entries) on 7 <= K <= QIPGPU_MAX_DENSE targets, where the bits of ctrl_mask are 1;
+// `cpos` = control bits (dense.hip).
+int diag_large(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mask, const std::vector<int> &cpos,
+               const std::complex<double> *d);
 
 }  // namespace qipgpu

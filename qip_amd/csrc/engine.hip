@@ -599,27 +599,6 @@ __global__ __launch_bounds__(256) void k_dense(const DenseArgs<R, K> a) {
     }
 }
 
-// Dense operator on 4..6 target qubits: one wave per block, each lane owns one base; the lane's
-// 2^K amplitudes are staged in LDS ([j][lane], conflict-free) and the matrix is read from the
-// device with wave-uniform addresses.
-template <typename R, int K>
-__global__ __launch_bounds__(64) void k_dense_lds(cx<R> *amps, uint64_t nbase, uint64_t ctrl_mask, PosList pl,
-                                                  const uint64_t *__restrict__ off, const cx<double> *__restrict__ U) {
-    constexpr int D = 1 << K;
-    __shared__ cx<double> v[D][64];
-    const uint64_t g = (uint64_t)blockIdx.x * 64 + threadIdx.x;
-    if (g >= nbase) return;
-    const uint64_t base = insert_zeros(g, pl) | ctrl_mask;
-    for (int j = 0; j < D; ++j) v[j][threadIdx.x] = widen(amps[base + off[j]]);
-    for (int i = 0; i < D; ++i) {
-        cx<double> acc;
-        acc.re = 0;
-        acc.im = 0;
-        for (int j = 0; j < D; ++j) cfma(acc, U[i * D + j], v[j][threadIdx.x]);
-        amps[base + off[i]] = narrow<R>(acc);
-    }
-}
-
 // Diagonal operator on K <= 6 targets (K = 0: a phase on the controlled subspace). One item per
 // amplitude of the controlled subspace.
 template <typename R, int K>
@@ -768,37 +747,6 @@ static int launch_dense_reg(qipgpu_state *s, const std::vector<int> &tbits, uint
 }
 
 template <typename R, int K>
-static int launch_dense_lds(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mask,
-                            const std::vector<int> &allpos, const std::vector<cd> &U) {
-    constexpr int D = 1 << K;
-    // upload [off (D uint64) | U (D*D cx<R>)] through the staging ring
-    std::vector<char> buf(sizeof(uint64_t) * D + sizeof(cx<double>) * D * D);
-    uint64_t *off = reinterpret_cast<uint64_t *>(buf.data());
-    cx<double> *Uh = reinterpret_cast<cx<double> *>(buf.data() + sizeof(uint64_t) * D);
-    for (int j = 0; j < D; ++j) {
-        uint64_t o = 0;
-        for (int t = 0; t < K; ++t)
-            if ((j >> (K - 1 - t)) & 1) o |= 1ull << tbits[t];
-        off[j] = o;
-    }
-    for (int i = 0; i < D * D; ++i) {
-        Uh[i].re = U[i].real();
-        Uh[i].im = U[i].imag();
-    }
-    void *d = nullptr;
-    int rc = ring_upload(s, buf.data(), buf.size(), &d);
-    if (rc) return rc;
-    PosList pl;
-    fill_poslist(pl, allpos);
-    const uint64_t nbase = s->len >> allpos.size();
-    const uint32_t nb = blocks_for(nbase, 64);
-    hipLaunchKernelGGL((k_dense_lds<R, K>), dim3(nb), dim3(64), 0, s->stream, (cx<R> *)s->amps, nbase, ctrl_mask, pl,
-                       (const uint64_t *)d, (const cx<double> *)((char *)d + sizeof(uint64_t) * D));
-    QIP_LAUNCHED();
-    return QIPGPU_OK;
-}
-
-template <typename R, int K>
 static int launch_diag(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mask,
                        const std::vector<int> &cpos, const std::vector<cd> &d) {
     DiagArgs<R, K> a;
@@ -854,18 +802,15 @@ static int dispatch(qipgpu_state *s, const Analysed &an, const std::vector<int32
             case 6: return launch_diag<R, 6>(s, tbits, ctrl_mask, cpos, d);
             default: break;
         }
-        return set_error(QIPGPU_EUNSUP, "diagonal operator on %d qubits exceeds %d", k, QIPGPU_MAX_DENSE);
+        return diag_large(s, tbits, ctrl_mask, cpos, d.data());
     }
     switch (k) {
         case 1: return launch_dense_reg<R, 1>(s, tbits, ctrl_mask, allpos, an.U);
         case 2: return launch_dense_reg<R, 2>(s, tbits, ctrl_mask, allpos, an.U);
         case 3: return launch_dense_reg<R, 3>(s, tbits, ctrl_mask, allpos, an.U);
-        case 4: return launch_dense_lds<R, 4>(s, tbits, ctrl_mask, allpos, an.U);
-        case 5: return launch_dense_lds<R, 5>(s, tbits, ctrl_mask, allpos, an.U);
-        case 6: return launch_dense_lds<R, 6>(s, tbits, ctrl_mask, allpos, an.U);
         default: break;
     }
-    return set_error(QIPGPU_EUNSUP, "dense operator on %d target qubits exceeds %d", k, QIPGPU_MAX_DENSE);
+    return dense_large(s, tbits, ctrl_mask, allpos, an.U.data());
 }
 
 static int check_disjoint(const qipgpu_state *s, int32_t k, const int32_t *t, int32_t c, const int32_t *ctl) {

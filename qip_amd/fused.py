@@ -99,7 +99,7 @@ def descriptor_gates(mats) -> List[Tuple[Tuple[int, ...], numpy.ndarray]]:
     for call in planner.plan(mats):
         if isinstance(call, planner.ProductCall):
             out.extend(((q,), m) for q, m in zip(call.targets, call.mats))
-        elif len(call.controls) + len(call.targets) > 6:
+        elif isinstance(call, planner.GeneralCall) or len(call.controls) + len(call.targets) > 6:
             raise ValueError("a call on {} qubits is applied unfused".format(len(call.controls) + len(call.targets)))
         else:
             out.append(controlled_dense(call))
@@ -338,6 +338,9 @@ class FusedBackend(GpuBackend):
             if isinstance(call, planner.ProductCall):
                 for q, m in zip(call.targets, call.mats):
                     rec.add((q,), m)
+            elif isinstance(call, planner.GeneralCall):
+                rec.record(self)
+                GpuBackend.apply_general(self, call)       # launches the record first
             elif len(call.controls) + len(call.targets) > self.fusion_width:
                 rec.record(self)
                 GpuBackend.apply_matrix(self, call.targets, call.mat, call.controls)   # launches the record first

@@ -8,16 +8,24 @@ Its operator (kronprod.pyx:164-196) is
 
 with U the union of all index groups. Groups that share no qubit give a Kronecker product, i.e.
 independent commuting factors; the planner splits the descriptor into such connected components
-and emits one in-place engine call per component:
+and emits in-place engine calls per component:
 
   * a C_MAT chain (operators.py:233-268, kronprod.pyx:215-225) becomes control qubits around its
     innermost matrix;
   * a SWAP_MAT(m) (operators.py:157-179, kronprod.pyx:227-231) over registers a, b becomes m
     two-qubit swaps a[i] <-> b[i] (same significance), each under the group's controls;
-  * a numpy matrix is applied densely on its qubits;
-  * a component of several groups sharing qubits (e.g. C(Not) on a 2-qubit register gives
-    {(c,t1): CMat(X), (c,t2): CMat(X)}, operators.py:41-43,221-227) is expanded into its dense
-    operator over the union by the product formula above; the engine then recovers its controls.
+  * a numpy matrix is applied densely on its qubits (up to MAX_DENSE_TARGETS);
+  * a component whose groups all begin with the same C_MAT control q (C(Op) on a register gives
+    {(c,t1): CMat(X), (c,t2): CMat(X)}, operators.py:41-43,221-227) is |0><0|_q (x) I +
+    |1><1|_q (x) O', O' the descriptor of the groups with that level peeled: q becomes a control of
+    every call planned for O', whose groups may now fall apart into components;
+  * any other component of several groups sharing qubits is expanded into its dense operator over
+    the union by the product formula above (up to MAX_EXPANDED_QUBITS); the engine then recovers
+    its controls;
+  * what is left, and numpy matrices wider than MAX_DENSE_TARGETS, go to the engine's restatement
+    of the formula itself (GeneralCall).
+Every call of one descriptor acts on qubits no other call of it changes under different controls,
+so the calls commute and their order is free.
 """
 from typing import Dict, List, Mapping, Sequence, Tuple
 
@@ -25,6 +33,7 @@ import numpy
 
 KRON_NUMPY, KRON_CMAT, KRON_SWAP = 1, 2, 3          # qip/ext/kronprod.pyx:18-19
 MAX_EXPANDED_QUBITS = 10                             # dense expansion of shared-qubit components
+MAX_DENSE_TARGETS = 12                               # QIPGPU_MAX_DENSE
 KRON_MERGE_WIDTH = 3                                 # single-qubit factors merged per sweep
 
 _SWAP4 = numpy.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], dtype=numpy.complex128)
@@ -136,17 +145,47 @@ def _expand_component(groups: List[Tuple[Tuple[int, ...], object]]) -> Tuple[Lis
     return union, out
 
 
-def plan(mats: Mapping) -> List[MatrixCall]:
-    """Engine calls equivalent to one kronselect_dot(mats)."""
-    newmats = normalise(mats)
-    items = []
-    for key, m in newmats.items():
-        if type(m) == numpy.ndarray:
-            m = m.astype(numpy.complex128)      # qip/util.py:63
-        _kind(m)
-        items.append((key, m))
-    # connected components over shared qubits (union-find), in first-appearance order
-    parent = {}
+class GeneralCall:
+    """A descriptor component applied by the engine's restatement of the reference's formula:
+    `groups` as (indices, matrix) with C_MAT chains and SWAP_MAT kept."""
+    __slots__ = ("groups",)
+    controls = ()
+
+    def __init__(self, groups: Sequence[Tuple[Tuple[int, ...], object]]):
+        self.groups = list(groups)
+
+    def packed(self):
+        """(group lengths, indices, C_MAT levels, inner kinds, SWAP_MAT widths, inner NUMPY matrices
+        concatenated as complex128) for qipgpu_apply_kron."""
+        lens, idx, levels, kinds, params, mats = [], [], [], [], [], []
+        for key, m in self.groups:
+            peeled, inner_idx, kind, inner = _peel(tuple(key), m)
+            lens.append(len(key))
+            idx.extend(key)
+            levels.append(len(peeled))
+            kinds.append(kind)
+            if kind == KRON_SWAP:
+                if len(inner_idx) != 2 * inner.n:
+                    raise Exception("Shape of square submatrix must equal 2**(number of indices): "
+                                    "{}: {}".format(key, m))
+                params.append(inner.n)
+            else:
+                params.append(0)
+                mats.append(numpy.ascontiguousarray(inner, dtype=numpy.complex128).ravel())
+        flat = numpy.concatenate(mats) if mats else numpy.zeros(1, dtype=numpy.complex128)
+        return lens, idx, levels, kinds, params, flat
+
+    @property
+    def targets(self) -> List[int]:
+        return sorted(set(q for idx, _ in self.groups for q in idx))
+
+    def __repr__(self):
+        return "GeneralCall(qubits={})".format(self.targets)
+
+
+def _components(items: List[Tuple[Tuple[int, ...], object]]) -> List[List[Tuple[Tuple[int, ...], object]]]:
+    """Groups joined by shared qubits (union-find), components in first-appearance order."""
+    parent: Dict[int, int] = {}
 
     def find(q):
         while parent[q] != q:
@@ -161,33 +200,74 @@ def plan(mats: Mapping) -> List[MatrixCall]:
             a, b = find(key[0]), find(q)
             if a != b:
                 parent[b] = a
-    comps: Dict[int, List[Tuple[Tuple[int, ...], object]]] = {}
-    order = []
-    for key, m in items:
-        root = find(key[0])
-        if root not in comps:
-            comps[root] = []
-            order.append(root)
-        comps[root].append((key, m))
+    comps: Dict[object, List[Tuple[Tuple[int, ...], object]]] = {}
+    for n, (key, m) in enumerate(items):
+        # a group without indices (a 1x1 matrix under C_MAT levels) is a scalar: its own component
+        comps.setdefault(find(key[0]) if key else ("scalar", n), []).append((key, m))
+    return list(comps.values())
 
-    calls: List[MatrixCall] = []
-    for root in order:
-        groups = comps[root]
-        if len(groups) == 1:
-            key, m = groups[0]
-            controls, idx, kind, inner = _peel(key, m)
-            if kind == KRON_SWAP:
-                w = inner.n
-                if len(idx) != 2 * w:
-                    raise Exception("Shape of square submatrix must equal 2**(number of indices): "
-                                    "{}: {}".format(key, m))
-                for a, b in zip(idx[:w], idx[w:]):
-                    calls.append(MatrixCall([a, b], controls, _SWAP4))
-            else:
-                calls.append(MatrixCall(idx, controls, numpy.ascontiguousarray(inner, dtype=numpy.complex128)))
-        else:
-            union, dense = _expand_component(groups)
-            calls.append(MatrixCall(union, [], dense))
+
+def _wrap_controls(controls: Sequence[int], groups):
+    """The groups with the peeled controls put back as C_MAT levels (for a GeneralCall)."""
+    out = []
+    for key, m in groups:
+        for c in reversed(controls):
+            key = (c,) + tuple(key)
+            m = _CMatLevel(m)
+        out.append((tuple(key), m))
+    return out
+
+
+class _CMatLevel:
+    """A C_MAT level around m (same _kron_struct and fields as operators.CMat)."""
+    _kron_struct = KRON_CMAT
+
+    def __init__(self, m):
+        self.m = m
+
+
+def _plan_component(groups, controls: List[int]) -> list:
+    if len(groups) == 1:
+        key, m = groups[0]
+        peeled, idx, kind, inner = _peel(key, m)
+        ctl = controls + peeled
+        if kind == KRON_SWAP:
+            w = inner.n
+            if len(idx) != 2 * w:
+                raise Exception("Shape of square submatrix must equal 2**(number of indices): "
+                                "{}: {}".format(key, m))
+            return [MatrixCall([a, b], ctl, _SWAP4) for a, b in zip(idx[:w], idx[w:])]
+        if len(idx) <= MAX_DENSE_TARGETS:
+            return [MatrixCall(idx, ctl, numpy.ascontiguousarray(inner, dtype=numpy.complex128))]
+        return [GeneralCall(_wrap_controls(controls, groups))]
+    heads = set(key[0] if _kind(m) == KRON_CMAT else None for key, m in groups)
+    if len(heads) == 1 and None not in heads:
+        q = heads.pop()
+        if all(q not in key[1:] for key, _ in groups):
+            inner = [(tuple(key[1:]), m.m) for key, m in groups]
+            calls = []
+            for comp in _components(inner):
+                calls.extend(_plan_component(comp, controls + [q]))
+            return calls
+    union = set(q for key, _ in groups for q in key)
+    if len(union) <= MAX_EXPANDED_QUBITS:
+        qubits, dense = _expand_component(groups)
+        return [MatrixCall(qubits, controls, dense)]
+    return [GeneralCall(_wrap_controls(controls, groups))]
+
+
+def plan(mats: Mapping) -> list:
+    """Engine calls equivalent to one kronselect_dot(mats): MatrixCall, ProductCall and GeneralCall."""
+    newmats = normalise(mats)
+    items = []
+    for key, m in newmats.items():
+        if type(m) == numpy.ndarray:
+            m = m.astype(numpy.complex128)      # qip/util.py:63
+        _kind(m)
+        items.append((key, m))
+    calls: list = []
+    for comp in _components(items):
+        calls.extend(_plan_component(comp, []))
     return merge_single_qubit(calls)
 
 
@@ -215,12 +295,33 @@ class ProductCall:
         return "ProductCall(t={})".format(self.targets)
 
 
-def merge_single_qubit(calls: List[MatrixCall]) -> list:
+def merge_single_qubit(calls: list) -> list:
     """Collect the uncontrolled single-qubit factors of one descriptor into one ProductCall (one
     sweep for all of them; the reference applies the whole descriptor in one sweep too,
-    kronprod.pyx:157-197)."""
-    singles = [c for c in calls if len(c.targets) == 1 and not c.controls]
-    if len(singles) < 2:
-        return calls
-    rest = [c for c in calls if not (len(c.targets) == 1 and not c.controls)]
-    return rest + [ProductCall([c.targets[0] for c in singles], [c.mat for c in singles])]
+    kronprod.pyx:157-197), and the single-qubit factors under one control set into Kronecker
+    blocks of up to KRON_MERGE_WIDTH qubits under those controls."""
+    singles, by_ctl, rest = [], {}, []
+    for c in calls:
+        if isinstance(c, MatrixCall) and len(c.targets) == 1:
+            if not c.controls:
+                singles.append(c)
+            else:
+                by_ctl.setdefault(tuple(sorted(c.controls)), []).append(c)
+        else:
+            rest.append(c)
+    out = rest
+    for ctl, group in by_ctl.items():
+        if len(group) == 1:
+            out.append(group[0])
+            continue
+        for i in range(0, len(group), KRON_MERGE_WIDTH):
+            part = group[i:i + KRON_MERGE_WIDTH]
+            mat = part[0].mat
+            for c in part[1:]:
+                mat = numpy.kron(mat, c.mat)
+            out.append(MatrixCall([c.targets[0] for c in part], list(ctl), mat))
+    if len(singles) >= 2:
+        out.append(ProductCall([c.targets[0] for c in singles], [c.mat for c in singles]))
+    else:
+        out.extend(singles)
+    return out
