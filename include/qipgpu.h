@@ -130,7 +130,9 @@ int qipgpu_probabilities(qipgpu_state *s, int32_t k, const int32_t *indices, dou
 int qipgpu_marginals(qipgpu_state *s, int32_t k, const int32_t *indices, double *out);
 
 /* measure_top_probabilities (kronprod.pyx:264-315): the top_k largest marginals (measurement
- * order), descending, ties by ascending m. Writes *count = min(top_k, 2^k) entries. */
+ * order), descending, ties by ascending m. Writes *count = min(top_k, 2^k) entries. The marginals
+ * stay on the device and are selected there for min(top_k, 2^k) <= 1024; wider selections copy
+ * the 2^k marginals to the host. */
 int qipgpu_top_probabilities(qipgpu_state *s, int32_t k, const int32_t *indices, int32_t top_k,
                              int64_t *ms, double *ps, int32_t *count);
 
@@ -150,6 +152,14 @@ int qipgpu_measure(qipgpu_state *s, int32_t k, const int32_t *indices, double u,
  * reduced state appears tiled 2^k times (the reference keeps its 2^n arena, backend.py:139-142). */
 int qipgpu_reduce_measure(qipgpu_state *s, int32_t k, const int32_t *indices, double u,
                           int64_t measured, double measured_prob, int64_t *m, double *p);
+
+/* reduce_measure with the state shrunk to the n-k unmeasured qubits (kronprod.pyx:449-490 with
+ * the arena resized, the intended reduce of backend.py:136-151): afterwards the handle holds
+ * 2^(n-k) amplitudes, amplitude j being the collapsed amplitude with measured bits m and
+ * unmeasured bits j (their order kept). Needs k < n. The old buffer is freed after the slice is
+ * gathered (peak memory: the state plus the slice). */
+int qipgpu_reduce_measure_shrink(qipgpu_state *s, int32_t k, const int32_t *indices, double u,
+                                 int64_t measured, double measured_prob, int64_t *m, double *p);
 
 /* ---- precision form: fused complex64 blocks on MFMA (no reference counterpart: it applies the
  * product of several consecutive cdot_loop operators at once) ---------------------------------- */

@@ -50,6 +50,7 @@ SIGNATURES = {
     "qipgpu_soft_measure": (ctypes.c_int, [_P, _I32, _P, _D, _I64, _PI64, _PD]),
     "qipgpu_measure": (ctypes.c_int, [_P, _I32, _P, _D, _I64, _D, _PI64, _PD]),
     "qipgpu_reduce_measure": (ctypes.c_int, [_P, _I32, _P, _D, _I64, _D, _PI64, _PD]),
+    "qipgpu_reduce_measure_shrink": (ctypes.c_int, [_P, _I32, _P, _D, _I64, _D, _PI64, _PD]),
     "qipgpu_fused_apply": (ctypes.c_int, [_P, _I32, _P, _P]),
     "qipgpu_fused_pattern_sweep": (ctypes.c_int, [_P, _I32, _P]),
     "qipgpu_compose": (ctypes.c_int, [_I32, _P, _I32, _P, _P, _P, _P]),

@@ -444,16 +444,23 @@ class GpuBackend(StateType):
 
     def reduce_measure(self, indices: Sequence[int], measured: Optional[int] = None,
                        measured_prob: Optional[float] = None,
-                       input_offset: int = 0, output_offset: int = 0) -> Tuple[int, float]:
+                       input_offset: int = 0, output_offset: int = 0, *, shrink: bool = False) -> Tuple[int, float]:
+        """Measure, then drop the measured qubits. By default the result is the reference's: the
+        reduced state repeated 2^k times over the 2^n buffer (qip/backend.py:139-142). With
+        shrink=True the state becomes the 2^(n-k) amplitudes of the unmeasured qubits and `n`
+        drops by k."""
         idx = _capi.i32(indices)
         validate_measured(len(idx), measured, measured_prob)
         u = random.random() if (measured is None or measured_prob is None) else 0.0
         m = ctypes.c_int64()
         p = ctypes.c_double()
-        _capi.check(self._lib.qipgpu_reduce_measure(self._h, len(idx), _capi.ptr(idx), u,
-                                                    -1 if measured is None else int(measured),
-                                                    -1.0 if measured_prob is None else float(measured_prob),
-                                                    ctypes.byref(m), ctypes.byref(p)), "reduce_measure")
+        fn = self._lib.qipgpu_reduce_measure_shrink if shrink else self._lib.qipgpu_reduce_measure
+        _capi.check(fn(self._h, len(idx), _capi.ptr(idx), u,
+                       -1 if measured is None else int(measured),
+                       -1.0 if measured_prob is None else float(measured_prob),
+                       ctypes.byref(m), ctypes.byref(p)), "reduce_measure")
+        if shrink:
+            self.n -= len(idx)
         return int(m.value), float(p.value)
 
     def soft_measure(self, indices: Sequence[int], measured: Optional[int] = None,

@@ -327,20 +327,7 @@ __global__ __launch_bounds__(256) void k_copy_amps(cx<R> *dst, const cx<R> *__re
 
 // ---- host side -----------------------------------------------------------------------------------
 
-// Temporary device buffer of one call: freed on every exit path, after the state's stream has
-// finished with it.
-struct StreamBuffer {
-    hipStream_t stream;
-    void *p = nullptr;
-    explicit StreamBuffer(hipStream_t st) : stream(st) {}
-    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes); }
-    ~StreamBuffer() {
-        if (p) {
-            (void)hipStreamSynchronize(stream);
-            (void)hipFree(p);
-        }
-    }
-};
+using StreamBuffer = TempBuffer;
 
 std::vector<uint64_t> offsets(const std::vector<int> &tbits) {
     const int K = (int)tbits.size();
@@ -496,8 +483,7 @@ int launch_diag_table(qipgpu_state *s, const std::vector<int> &tbits, uint64_t c
     const uint64_t count = s->len >> cpos.size();
     const uint32_t nb = (uint32_t)std::min<uint64_t>((count + 255) / 256, (uint64_t)device_cus() * 64);
     hipLaunchKernelGGL((k_diag_table<R>), dim3(nb), dim3(256), 0, s->stream, (cx<R> *)s->amps, count, ctrl_mask,
-                       poslist(cpos), K, (const uint8_t *)(host.size() ? (char *)buf.p + D * sizeof(cx<double>) : nullptr),
-                       (const cx<double> *)buf.p);
+                       poslist(cpos), K, (const uint8_t *)buf.p + D * sizeof(cx<double>), (const cx<double> *)buf.p);
     QIP_LAUNCHED();
     QIP_HIP(hipStreamSynchronize(s->stream));   // the host copy may be pageable
     return QIPGPU_OK;

@@ -105,6 +105,28 @@ int set_error(int code, const char *fmt, ...);
                                        hipGetErrorString(e_));                             \
     } while (0)
 
+// A device buffer for one call, freed on every exit path once the state's stream is done with it.
+struct TempBuffer {
+    hipStream_t stream;
+    void *p = nullptr;
+    explicit TempBuffer(hipStream_t st) : stream(st) {}
+    TempBuffer(const TempBuffer &) = delete;
+    TempBuffer &operator=(const TempBuffer &) = delete;
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes); }
+    // hand the buffer over (it is not freed here any more)
+    void *release() {
+        void *q = p;
+        p = nullptr;
+        return q;
+    }
+    ~TempBuffer() {
+        if (p) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipFree(p);
+        }
+    }
+};
+
 // Make sure s->scratch holds at least `bytes`.
 int ensure_scratch(qipgpu_state *s, size_t bytes);
 // Compute units of the current device, queried once per device (persistent grids are sized by it).

@@ -1370,9 +1370,10 @@ namespace {
 
 enum class Order { LsbFirst, Entwine };
 
-// Every outcome's marginal in one pass over the state (k_marg_pass), into host `out` (2^k values in
-// the output order given by opos).
-static int marginals_pass(qipgpu_state *s, const std::vector<int> &mpos, const uint8_t *opos, double *out) {
+// Every outcome's marginal in one pass over the state (k_marg_pass), left on the device at the start
+// of the scratch (2^k values in the output order given by opos). The scratch also holds `extra` bytes
+// from the first 256-byte boundary after the results on, free once this returns.
+static int marginals_pass(qipgpu_state *s, const std::vector<int> &mpos, const uint8_t *opos, size_t extra) {
     const int n = s->n, k = (int)mpos.size();
     MargPassArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -1408,7 +1409,7 @@ static int marginals_pass(qipgpu_state *s, const std::vector<int> &mpos, const u
     if (nblk > 0x7fffffffull) return set_error(QIPGPU_EUNSUP, "marginal grid too large");
     const size_t out_bytes = sizeof(double) * a.m_total;
     const size_t part_off = ((out_bytes + 255) / 256) * 256;
-    int rc = ensure_scratch(s, part_off + sizeof(double) * a.m_total * nparts);
+    int rc = ensure_scratch(s, part_off + std::max(sizeof(double) * a.m_total * nparts, extra));
     if (rc) return rc;
     double *d_out = static_cast<double *>(s->scratch);
     double *d_part = reinterpret_cast<double *>(static_cast<char *>(s->scratch) + part_off);
@@ -1424,14 +1425,13 @@ static int marginals_pass(qipgpu_state *s, const std::vector<int> &mpos, const u
     else
         hipLaunchKernelGGL(k_marg_pass_final, dim3(blocks_for(a.m_total, 256)), dim3(256), 0, s->stream, d_part, a, d_out);
     QIP_LAUNCHED();
-    QIP_HIP(hipMemcpyAsync(out, d_out, out_bytes, hipMemcpyDeviceToHost, s->stream));
-    QIP_HIP(hipStreamSynchronize(s->stream));
     return QIPGPU_OK;
 }
 
-// Compute marginals over outcomes [m_lo, m_lo + m_count) into host `out`.
-static int marginals_host(qipgpu_state *s, int32_t k, const int32_t *indices, Order order, uint64_t m_lo,
-                          uint64_t m_count, double *out) {
+// Marginals over outcomes [m_lo, m_lo + m_count), left on the device at the start of the scratch;
+// the scratch holds `extra` more bytes from the first 256-byte boundary after them.
+static int marginals_dev(qipgpu_state *s, int32_t k, const int32_t *indices, Order order, uint64_t m_lo,
+                         uint64_t m_count, size_t extra) {
     int rc = check_qubits(s, k, indices, "measured qubits");
     if (rc) return rc;
     rc = settle(s);
@@ -1457,14 +1457,14 @@ static int marginals_host(qipgpu_state *s, int32_t k, const int32_t *indices, Or
     fill_poslist(a.mpl, mpos);
     // every outcome of up to 2^26: one pass over the state (its partials stay within 1 GiB)
     if (n >= kMargChunkBits && k >= 1 && k <= 26 && m_lo == 0 && m_count == (1ull << k))
-        return marginals_pass(s, mpos, a.opos, out);
+        return marginals_pass(s, mpos, a.opos, extra);
     const bool serial = a.terms <= 64;
     a.chunk = 256 * 64;
     a.nchunks = serial ? 1 : (a.terms + a.chunk - 1) / a.chunk;
     const size_t out_bytes = sizeof(double) * m_count;
     const size_t part_bytes = serial ? 0 : sizeof(double) * m_count * a.nchunks;
     const size_t part_off = ((out_bytes + 255) / 256) * 256;
-    rc = ensure_scratch(s, part_off + part_bytes);
+    rc = ensure_scratch(s, part_off + std::max(part_bytes, extra));
     if (rc) return rc;
     double *d_out = static_cast<double *>(s->scratch);
     double *d_part = reinterpret_cast<double *>(static_cast<char *>(s->scratch) + part_off);
@@ -1489,7 +1489,15 @@ static int marginals_host(qipgpu_state *s, int32_t k, const int32_t *indices, Or
                            d_out);
         QIP_LAUNCHED();
     }
-    QIP_HIP(hipMemcpyAsync(out, d_out, out_bytes, hipMemcpyDeviceToHost, s->stream));
+    return QIPGPU_OK;
+}
+
+// The same into host `out`.
+static int marginals_host(qipgpu_state *s, int32_t k, const int32_t *indices, Order order, uint64_t m_lo,
+                          uint64_t m_count, double *out) {
+    int rc = marginals_dev(s, k, indices, order, m_lo, m_count, 0);
+    if (rc) return rc;
+    QIP_HIP(hipMemcpyAsync(out, s->scratch, sizeof(double) * m_count, hipMemcpyDeviceToHost, s->stream));
     QIP_HIP(hipStreamSynchronize(s->stream));
     return QIPGPU_OK;
 }
@@ -1541,6 +1549,60 @@ __global__ __launch_bounds__(256) void k_reduce_tile(cx<R> *amps, const cx<R> *_
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) amps[i] = slice[i & lowmask];
 }
 
+// ---- top-k on the device --------------------------------------------------------------------------
+// Order: larger probability first, equal probabilities by ascending outcome (a strict total order on
+// distinct outcomes). A round sorts chunks of kTopChunk candidates in LDS (bitonic network) and keeps
+// the first T of each; rounds repeat until one chunk is left. Slots past the end hold (-1, ~0), which
+// sort after every probability.
+constexpr int kTopChunk = 2048;
+constexpr int kTopMaxDevice = 1024;
+
+__device__ __forceinline__ bool top_before(double va, uint64_t ia, double vb, uint64_t ib) {
+    return va > vb || (va == vb && ia < ib);
+}
+
+__global__ __launch_bounds__(256) void k_topk_round(const double *__restrict__ vin, const uint64_t *__restrict__ iin,
+                                                    uint64_t len, uint32_t T, double *__restrict__ vout,
+                                                    uint64_t *__restrict__ iout) {
+    __shared__ double sv[kTopChunk];
+    __shared__ uint64_t si[kTopChunk];
+    const uint64_t c0 = (uint64_t)blockIdx.x * kTopChunk;
+    for (int e = threadIdx.x; e < kTopChunk; e += 256) {
+        const uint64_t g = c0 + e;
+        if (g < len) {
+            sv[e] = vin[g];
+            si[e] = iin ? iin[g] : g;
+        } else {
+            sv[e] = -1.0;
+            si[e] = ~0ull;
+        }
+    }
+    __syncthreads();
+    for (int size = 2; size <= kTopChunk; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = threadIdx.x; t < kTopChunk / 2; t += 256) {
+                const int lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
+                const bool asc = (lo & size) == 0;
+                const bool out_of_order = asc ? top_before(sv[hi], si[hi], sv[lo], si[lo])
+                                              : top_before(sv[lo], si[lo], sv[hi], si[hi]);
+                if (out_of_order) {
+                    const double tv = sv[lo];
+                    sv[lo] = sv[hi];
+                    sv[hi] = tv;
+                    const uint64_t ti = si[lo];
+                    si[lo] = si[hi];
+                    si[hi] = ti;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t e = threadIdx.x; e < T; e += 256) {
+        vout[(uint64_t)blockIdx.x * T + e] = sv[e];
+        iout[(uint64_t)blockIdx.x * T + e] = si[e];
+    }
+}
+
 namespace {
 
 static uint64_t measured_bits(int n, int32_t k, const int32_t *indices, uint64_t m, uint64_t *mask_out) {
@@ -1564,12 +1626,9 @@ static int soft_measure_impl(qipgpu_state *s, int32_t k, const int32_t *indices,
     if (k < 1) return set_error(QIPGPU_EINVAL, "measure needs at least one qubit");
     const uint64_t nm = 1ull << k;
     if (measured >= (int64_t)nm) return set_error(QIPGPU_EINVAL, "Measured value must be less than 2**len(indices)");
-    // r = u * prob_magnitude(vec) (kronprod.pyx:362-365)
-    double total = 0.0;
-    rc = qipgpu_total_prob(s, &total);
-    if (rc) return rc;
-    double r = u * total;
     if (measured >= 0) {
+        // the draw is made by the caller and unused (kronprod.pyx:362); only the outcome's probability
+        // is needed, which reads the 2^(n-k) amplitudes of that outcome
         double acc = 0.0;
         rc = marginals_host(s, k, indices, Order::Entwine, (uint64_t)measured, 1, &acc);
         if (rc) return rc;
@@ -1577,9 +1636,14 @@ static int soft_measure_impl(qipgpu_state *s, int32_t k, const int32_t *indices,
         *p_out = acc;
         return QIPGPU_OK;
     }
+    // one pass over the state for every outcome's probability; prob_magnitude (kronprod.pyx:362-365)
+    // is their sum, in ascending outcome order, in FP64
     std::vector<double> marg(nm);
     rc = marginals_host(s, k, indices, Order::Entwine, 0, nm, marg.data());
     if (rc) return rc;
+    double total = 0.0;
+    for (uint64_t i = 0; i < nm; ++i) total += marg[i];
+    double r = u * total;
     // kronprod.pyx:371-381: sequential subtraction, first outcome with r <= 0; if none, the last.
     uint64_t p_index = 0;
     double acc = 0.0;
@@ -1643,20 +1707,60 @@ int qipgpu_top_probabilities(qipgpu_state *s, int32_t k, const int32_t *indices,
     if (k < 0 || k > s->n) return set_error(QIPGPU_EINVAL, "bad qubit count %d", k);
     if (top_k < 0) return set_error(QIPGPU_EINVAL, "negative top_k");
     const uint64_t nm = 1ull << k;
-    std::vector<double> marg(nm);
-    int rc = marginals_host(s, k, indices, Order::Entwine, 0, nm, marg.data());
-    if (rc) return rc;
     const uint64_t kk = std::min<uint64_t>((uint64_t)top_k, nm);
-    std::vector<uint64_t> order(nm);
-    for (uint64_t i = 0; i < nm; ++i) order[i] = i;
-    std::partial_sort(order.begin(), order.begin() + kk, order.end(), [&](uint64_t a, uint64_t b) {
-        return marg[a] > marg[b] || (marg[a] == marg[b] && a < b);
-    });
-    for (uint64_t i = 0; i < kk; ++i) {
-        ms[i] = (int64_t)order[i];
-        ps[i] = marg[order[i]];
-    }
     *count = (int32_t)kk;
+    if (kk == 0) return check_qubits(s, k, indices, "measured qubits");
+    if (kk > (uint64_t)kTopMaxDevice) {
+        // wide selections: every marginal to the host, selected there
+        std::vector<double> marg(nm);
+        int rc = marginals_host(s, k, indices, Order::Entwine, 0, nm, marg.data());
+        if (rc) return rc;
+        std::vector<uint64_t> order(nm);
+        for (uint64_t i = 0; i < nm; ++i) order[i] = i;
+        std::partial_sort(order.begin(), order.begin() + kk, order.end(), [&](uint64_t a, uint64_t b) {
+            return marg[a] > marg[b] || (marg[a] == marg[b] && a < b);
+        });
+        for (uint64_t i = 0; i < kk; ++i) {
+            ms[i] = (int64_t)order[i];
+            ps[i] = marg[order[i]];
+        }
+        return QIPGPU_OK;
+    }
+    // candidates of the first round: ceil(2^k / chunk) * T pairs, ping-pong with the next round's
+    const uint32_t T = (uint32_t)kk;
+    const uint64_t blocks0 = (nm + kTopChunk - 1) / kTopChunk;
+    const size_t pairs = (size_t)blocks0 * T;
+    const size_t region = pairs * (sizeof(double) + sizeof(uint64_t));
+    const size_t out_off = ((sizeof(double) * nm + 255) / 256) * 256;
+    int rc = marginals_dev(s, k, indices, Order::Entwine, 0, nm, 2 * region + 256);
+    if (rc) return rc;
+    double *marg = static_cast<double *>(s->scratch);
+    char *base = static_cast<char *>(s->scratch) + out_off;
+    double *va = reinterpret_cast<double *>(base);
+    uint64_t *ia = reinterpret_cast<uint64_t *>(base + pairs * sizeof(double));
+    double *vb = reinterpret_cast<double *>(base + region);
+    uint64_t *ib = reinterpret_cast<uint64_t *>(base + region + pairs * sizeof(double));
+    const double *vin = marg;
+    const uint64_t *iin = nullptr;
+    uint64_t len = nm;
+    bool into_a = true;
+    for (;;) {
+        const uint64_t blocks = (len + kTopChunk - 1) / kTopChunk;
+        double *vo = into_a ? va : vb;
+        uint64_t *io = into_a ? ia : ib;
+        hipLaunchKernelGGL(k_topk_round, dim3((uint32_t)blocks), dim3(256), 0, s->stream, vin, iin, len, T, vo, io);
+        QIP_LAUNCHED();
+        vin = vo;
+        iin = io;
+        into_a = !into_a;
+        if (blocks == 1) break;
+        len = blocks * T;
+    }
+    std::vector<uint64_t> idx(T);
+    QIP_HIP(hipMemcpyAsync(ps, vin, sizeof(double) * T, hipMemcpyDeviceToHost, s->stream));
+    QIP_HIP(hipMemcpyAsync(idx.data(), iin, sizeof(uint64_t) * T, hipMemcpyDeviceToHost, s->stream));
+    QIP_HIP(hipStreamSynchronize(s->stream));
+    for (uint32_t i = 0; i < T; ++i) ms[i] = (int64_t)idx[i];
     return QIPGPU_OK;
 }
 
@@ -1686,8 +1790,10 @@ int qipgpu_measure(qipgpu_state *s, int32_t k, const int32_t *indices, double u,
     return QIPGPU_OK;
 }
 
-int qipgpu_reduce_measure(qipgpu_state *s, int32_t k, const int32_t *indices, double u, int64_t measured,
-                          double measured_prob, int64_t *m, double *p) {
+// reduce_measure's kept slice: outcome, then slice[j] = amps[measured bits | j with zeros inserted at
+// the measured positions] * sqrt(1/p) in a new buffer of 2^(n-k) amplitudes (handed to the caller).
+static int reduced_slice(qipgpu_state *s, int32_t k, const int32_t *indices, double u, int64_t measured,
+                         double measured_prob, int64_t *m, double *p, TempBuffer &slice) {
     int rc = check_qubits(s, k, indices, "measured qubits");
     if (rc) return rc;
     rc = resolve_outcome(s, k, indices, u, measured, measured_prob, m, p);
@@ -1702,28 +1808,53 @@ int qipgpu_reduce_measure(qipgpu_state *s, int32_t k, const int32_t *indices, do
     PosList mpl;
     fill_poslist(mpl, mpos);
     const uint64_t count = 1ull << (s->n - k);
-    void *slice = nullptr;
-    if (hipMalloc(&slice, count * s->amp_bytes) != hipSuccess) {
+    if (slice.alloc(count * s->amp_bytes) != hipSuccess) {
         (void)hipGetLastError();
         return set_error(QIPGPU_ENOMEM, "reduce_measure: cannot allocate the %llu-amplitude slice",
                          (unsigned long long)count);
     }
     const uint32_t ng = std::min<uint32_t>(blocks_for(count, 256 * 4), 1u << 20);
-    const uint32_t nt = std::min<uint32_t>(blocks_for(s->len, 256 * 4), 1u << 20);
-    if (s->dtype == QIPGPU_C128) {
+    if (s->dtype == QIPGPU_C128)
         hipLaunchKernelGGL(k_reduce_gather<double>, dim3(ng), dim3(256), 0, s->stream, (const cx<double> *)s->amps,
-                           (cx<double> *)slice, count, mbits, mpl, mult);
-        hipLaunchKernelGGL(k_reduce_tile<double>, dim3(nt), dim3(256), 0, s->stream, (cx<double> *)s->amps,
-                           (const cx<double> *)slice, s->len, count - 1);
-    } else {
+                           (cx<double> *)slice.p, count, mbits, mpl, mult);
+    else
         hipLaunchKernelGGL(k_reduce_gather<float>, dim3(ng), dim3(256), 0, s->stream, (const cx<float> *)s->amps,
-                           (cx<float> *)slice, count, mbits, mpl, mult);
+                           (cx<float> *)slice.p, count, mbits, mpl, mult);
+    QIP_LAUNCHED();
+    return QIPGPU_OK;
+}
+
+int qipgpu_reduce_measure(qipgpu_state *s, int32_t k, const int32_t *indices, double u, int64_t measured,
+                          double measured_prob, int64_t *m, double *p) {
+    TempBuffer slice(s->stream);
+    int rc = reduced_slice(s, k, indices, u, measured, measured_prob, m, p, slice);
+    if (rc) return rc;
+    // the reference's arena keeps its 2^n size: the slice repeats over it (backend.py:139-142)
+    const uint64_t count = 1ull << (s->n - k);
+    const uint32_t nt = std::min<uint32_t>(blocks_for(s->len, 256 * 4), 1u << 20);
+    if (s->dtype == QIPGPU_C128)
+        hipLaunchKernelGGL(k_reduce_tile<double>, dim3(nt), dim3(256), 0, s->stream, (cx<double> *)s->amps,
+                           (const cx<double> *)slice.p, s->len, count - 1);
+    else
         hipLaunchKernelGGL(k_reduce_tile<float>, dim3(nt), dim3(256), 0, s->stream, (cx<float> *)s->amps,
-                           (const cx<float> *)slice, s->len, count - 1);
-    }
+                           (const cx<float> *)slice.p, s->len, count - 1);
     QIP_LAUNCHED();
     QIP_HIP(hipStreamSynchronize(s->stream));
-    QIP_HIP(hipFree(slice));
+    return QIPGPU_OK;
+}
+
+int qipgpu_reduce_measure_shrink(qipgpu_state *s, int32_t k, const int32_t *indices, double u, int64_t measured,
+                                 double measured_prob, int64_t *m, double *p) {
+    if (k >= s->n) return set_error(QIPGPU_EINVAL, "shrinking by %d of %d qubits leaves none", k, s->n);
+    TempBuffer slice(s->stream);
+    int rc = reduced_slice(s, k, indices, u, measured, measured_prob, m, p, slice);
+    if (rc) return rc;
+    QIP_HIP(hipStreamSynchronize(s->stream));
+    QIP_HIP(hipFree(s->amps));
+    s->amps = slice.release();
+    s->n -= k;
+    s->len = 1ull << s->n;
+    recorder_discard(s);      // recorded nothing (settled); the record now counts n - k qubits
     return QIPGPU_OK;
 }
 
