@@ -125,6 +125,16 @@ int qipgpu_total_prob(qipgpu_state *s, double *out);
 /* measure_probabilities (kronprod.pyx:238-261): out[2^k], qubit indices[j] on bit j (LSB-first) */
 int qipgpu_probabilities(qipgpu_state *s, int32_t k, const int32_t *indices, double *out);
 
+/* qipgpu_probabilities without any device allocation: no counterpart in the reference; for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The reduction's results and partials live in a 16 MiB device array of the library,
+ * one per GPU, held until the process ends; k <= 20 (larger k: QIPGPU_EINVAL). The state must have
+ * no recorded, unlaunched gates (call qipgpu_fused_flush first; QIPGPU_EINVAL otherwise) and the
+ * call's device must be the state's. Calls that overlap on one GPU (several threads or streams)
+ * race on the shared array. Same results as qipgpu_probabilities to FP64 rounding (the partial sums
+ * may be grouped differently when the fixed array is the limit). */
+int qipgpu_probabilities_noalloc(qipgpu_state *s, int32_t k, const int32_t *indices, double *out);
+
 /* Marginals in measurement order: out[m] = sum of |psi_i|^2 over i whose measured bits read m,
  * the smallest measured qubit being the most significant bit of m (entwine_bit, util.pyx:2-23). */
 int qipgpu_marginals(qipgpu_state *s, int32_t k, const int32_t *indices, double *out);

@@ -45,6 +45,7 @@ SIGNATURES = {
     "qipgpu_func_apply_table": (ctypes.c_int, [_P, _I32, _P, _I32, _P, _P]),
     "qipgpu_total_prob": (ctypes.c_int, [_P, _PD]),
     "qipgpu_probabilities": (ctypes.c_int, [_P, _I32, _P, _P]),
+    "qipgpu_probabilities_noalloc": (ctypes.c_int, [_P, _I32, _P, _P]),
     "qipgpu_marginals": (ctypes.c_int, [_P, _I32, _P, _P]),
     "qipgpu_top_probabilities": (ctypes.c_int, [_P, _I32, _P, _I32, _P, _P, _PI32]),
     "qipgpu_soft_measure": (ctypes.c_int, [_P, _I32, _P, _D, _I64, _PI64, _PD]),

@@ -140,6 +140,8 @@ int flush_recorded(qipgpu_state *s);
 // Drop recorded gates (the state is about to be overwritten) / free the recorder.
 void recorder_discard(qipgpu_state *s);
 void recorder_free(qipgpu_state *s);
+// Gates recorded and not launched yet.
+int64_t recorder_pending(const qipgpu_state *s);
 // Launch recorded gates, then apply a fused path's pending renormalisation factor: before the state
 // is read or written from the host.
 int settle(qipgpu_state *s);

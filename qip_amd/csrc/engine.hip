@@ -15,6 +15,8 @@
 
 #include "engine.h"
 
+#include <mutex>
+
 using qipgpu::cx;
 using qipgpu::insert_zeros;
 using qipgpu::PosList;
@@ -1370,10 +1372,31 @@ namespace {
 
 enum class Order { LsbFirst, Entwine };
 
+// Where a reduction keeps its results and partials: the state's growable scratch (area == nullptr),
+// or a fixed device area that is never reallocated.
+struct FixedArea {
+    char *base;
+    size_t cap;
+};
+
+static int reduction_space(qipgpu_state *s, const FixedArea *area, size_t bytes, char **out) {
+    if (!area) {
+        int rc = ensure_scratch(s, bytes);
+        if (rc) return rc;
+        *out = static_cast<char *>(s->scratch);
+        return QIPGPU_OK;
+    }
+    if (bytes > area->cap)
+        return set_error(QIPGPU_EINVAL, "the reduction needs %zu bytes of a %zu-byte fixed area", bytes, area->cap);
+    *out = area->base;
+    return QIPGPU_OK;
+}
+
 // Every outcome's marginal in one pass over the state (k_marg_pass), left on the device at the start
 // of the scratch (2^k values in the output order given by opos). The scratch also holds `extra` bytes
 // from the first 256-byte boundary after the results on, free once this returns.
-static int marginals_pass(qipgpu_state *s, const std::vector<int> &mpos, const uint8_t *opos, size_t extra) {
+static int marginals_pass(qipgpu_state *s, const std::vector<int> &mpos, const uint8_t *opos, size_t extra,
+                          const FixedArea *area) {
     const int n = s->n, k = (int)mpos.size();
     MargPassArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -1397,22 +1420,24 @@ static int marginals_pass(qipgpu_state *s, const std::vector<int> &mpos, const u
     // partials (2^k per part, read back strided) stay within 8 MiB unless that leaves fewer than two
     // blocks per CU, and within 256 MiB.
     const int nfree_hi = n - kMargChunkBits - a.khi;
+    const size_t res_bytes = ((sizeof(double) * a.m_total + 255) / 256) * 256;
+    const size_t part_cap = area ? (area->cap > res_bytes ? area->cap - res_bytes : 0) : (256ull << 20);
     uint64_t nparts = 1;
     auto blocks_of = [&](uint64_t parts) { return (1ull << a.khi) * parts; };
     while (nparts < (1ull << nfree_hi) && blocks_of(nparts) < 2048 &&
            (sizeof(double) * a.m_total * nparts * 2 <= (8ull << 20) || blocks_of(nparts) < 512) &&
-           sizeof(double) * a.m_total * nparts * 2 <= (256ull << 20))
+           sizeof(double) * a.m_total * nparts * 2 <= part_cap)
         nparts *= 2;
     a.nparts = (uint32_t)nparts;
     a.per = (1ull << nfree_hi) / nparts;
     const uint64_t nblk = (1ull << a.khi) * nparts;
     if (nblk > 0x7fffffffull) return set_error(QIPGPU_EUNSUP, "marginal grid too large");
-    const size_t out_bytes = sizeof(double) * a.m_total;
-    const size_t part_off = ((out_bytes + 255) / 256) * 256;
-    int rc = ensure_scratch(s, part_off + std::max(sizeof(double) * a.m_total * nparts, extra));
+    const size_t part_off = res_bytes;
+    char *space = nullptr;
+    int rc = reduction_space(s, area, part_off + std::max(sizeof(double) * a.m_total * nparts, extra), &space);
     if (rc) return rc;
-    double *d_out = static_cast<double *>(s->scratch);
-    double *d_part = reinterpret_cast<double *>(static_cast<char *>(s->scratch) + part_off);
+    double *d_out = reinterpret_cast<double *>(space);
+    double *d_part = reinterpret_cast<double *>(space + part_off);
     if (s->dtype == QIPGPU_C128)
         hipLaunchKernelGGL(k_marg_pass<double>, dim3((uint32_t)nblk), dim3(256), 0, s->stream, (const cx<double> *)s->amps,
                            a, d_part);
@@ -1431,7 +1456,7 @@ static int marginals_pass(qipgpu_state *s, const std::vector<int> &mpos, const u
 // Marginals over outcomes [m_lo, m_lo + m_count), left on the device at the start of the scratch;
 // the scratch holds `extra` more bytes from the first 256-byte boundary after them.
 static int marginals_dev(qipgpu_state *s, int32_t k, const int32_t *indices, Order order, uint64_t m_lo,
-                         uint64_t m_count, size_t extra) {
+                         uint64_t m_count, size_t extra, const FixedArea *area = nullptr) {
     int rc = check_qubits(s, k, indices, "measured qubits");
     if (rc) return rc;
     rc = settle(s);
@@ -1457,17 +1482,18 @@ static int marginals_dev(qipgpu_state *s, int32_t k, const int32_t *indices, Ord
     fill_poslist(a.mpl, mpos);
     // every outcome of up to 2^26: one pass over the state (its partials stay within 1 GiB)
     if (n >= kMargChunkBits && k >= 1 && k <= 26 && m_lo == 0 && m_count == (1ull << k))
-        return marginals_pass(s, mpos, a.opos, extra);
+        return marginals_pass(s, mpos, a.opos, extra, area);
     const bool serial = a.terms <= 64;
     a.chunk = 256 * 64;
     a.nchunks = serial ? 1 : (a.terms + a.chunk - 1) / a.chunk;
     const size_t out_bytes = sizeof(double) * m_count;
     const size_t part_bytes = serial ? 0 : sizeof(double) * m_count * a.nchunks;
     const size_t part_off = ((out_bytes + 255) / 256) * 256;
-    rc = ensure_scratch(s, part_off + std::max(part_bytes, extra));
+    char *space = nullptr;
+    rc = reduction_space(s, area, part_off + std::max(part_bytes, extra), &space);
     if (rc) return rc;
-    double *d_out = static_cast<double *>(s->scratch);
-    double *d_part = reinterpret_cast<double *>(static_cast<char *>(s->scratch) + part_off);
+    double *d_out = reinterpret_cast<double *>(space);
+    double *d_part = reinterpret_cast<double *>(space + part_off);
     if (serial) {
         const uint32_t nb = blocks_for(m_count, 256);
         if (s->dtype == QIPGPU_C128)
@@ -1855,6 +1881,67 @@ int qipgpu_reduce_measure_shrink(qipgpu_state *s, int32_t k, const int32_t *indi
     s->n -= k;
     s->len = 1ull << s->n;
     recorder_discard(s);      // recorded nothing (settled); the record now counts n - k qubits
+    return QIPGPU_OK;
+}
+
+}  // extern "C"
+
+// ---- measure_probabilities without allocation ------------------------------------------------------
+// The reduction's results and partials live in `scratch`, a file-scope device array of
+// kNoAllocScratchBytes that every GPU the library runs on holds until the process ends. Host code
+// reaches it only through the address hipGetSymbolAddress gives on each device (the array's name is
+// a host-side stand-in), fetched on the first call with that device current.
+constexpr size_t kNoAllocScratchBytes = 16ull << 20;
+constexpr int kNoAllocMaxK = 20;   // 2^20 results (8 MiB) plus at least one part of partials
+
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+namespace {
+std::mutex g_noalloc_mu;
+double *g_noalloc_area[64] = {};
+
+int noalloc_area(int device, double **out) {
+    if (device < 0 || device >= 64) return set_error(QIPGPU_EINVAL, "device %d", device);
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_area[device]) {
+        int cur = -1;
+        QIP_HIP(hipGetDevice(&cur));
+        if (cur != device)
+            return set_error(QIPGPU_EINVAL, "call with the state's device (%d) current, not %d", device, cur);
+        void *p = nullptr;
+        QIP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_noalloc_area[device] = static_cast<double *>(p);
+    }
+    *out = g_noalloc_area[device];
+    return QIPGPU_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int qipgpu_probabilities_noalloc(qipgpu_state *s, int32_t k, const int32_t *indices, double *out) {
+    if (!s) return set_error(QIPGPU_EINVAL, "null state");
+    if (k < 0 || k > s->n) return set_error(QIPGPU_EINVAL, "bad qubit count %d", k);
+    if (k > kNoAllocMaxK)
+        return set_error(QIPGPU_EINVAL, "%d measured qubits exceed the no-allocation limit of %d", k, kNoAllocMaxK);
+    int rc = check_qubits(s, k, indices, "measured qubits");
+    if (rc) return rc;
+    // launching recorded gates may allocate the fused path's buffers: the caller flushes them first
+    if (recorder_pending(s) > 0)
+        return set_error(QIPGPU_EINVAL, "%lld recorded gates are not launched yet (qipgpu_fused_flush first)",
+                         (long long)recorder_pending(s));
+    double *area = nullptr;
+    rc = noalloc_area(s->device, &area);
+    if (rc) return rc;
+    rc = settle(s);                 // no recorded gates: at most the pending scale, no allocation
+    if (rc) return rc;
+    // every result and partial is written before it is read, so nothing needs clearing between calls
+    const FixedArea fixed = {reinterpret_cast<char *>(area), kNoAllocScratchBytes};
+    const uint64_t nm = 1ull << k;
+    rc = marginals_dev(s, k, indices, Order::LsbFirst, 0, nm, 0, &fixed);
+    if (rc) return rc;
+    QIP_HIP(hipMemcpyAsync(out, area, sizeof(double) * nm, hipMemcpyDeviceToHost, s->stream));
+    QIP_HIP(hipStreamSynchronize(s->stream));
     return QIPGPU_OK;
 }
 

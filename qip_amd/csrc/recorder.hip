@@ -400,6 +400,8 @@ void recorder_free(qipgpu_state *s) {
     s->recorder = nullptr;
 }
 
+int64_t recorder_pending(const qipgpu_state *s) { return s->recorder ? s->recorder->pending() : 0; }
+
 }  // namespace qipgpu
 
 using namespace qipgpu;
