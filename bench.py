@@ -21,11 +21,14 @@ Beside it, in the same run:
     the reference's -O3 -ffast-math -fopenmp flags, all visible cores) on a time-bounded sample of
     the circuit's own gates at full size, extrapolated to the circuit's gate mix.
 
-Other configurations, one JSON line each (--workload): circuit28 (28-qubit depth-20 circuit, plain
-complex128 vs unfused complex64), grover24 (24-qubit Grover search in both forms), qft16 (16-qubit
-QFT graph), batch512 (512 uneven circuits in the batching form), measure28 (marginal probabilities,
-norm and measurement on a 28-qubit complex128 state), func28 (func_apply on a 28-qubit complex128
-state).
+The default run then measures every other configuration in the same process and nests each under
+"configs" in the same JSON line, with its own rate, the roofline of its dominant kernel, a bounded
+CPU baseline (--config-cpu-seconds of work) and its error figures: qft16 (16-qubit QFT graph),
+circuit28 (28-qubit depth-20 circuit, plain complex128 vs unfused complex64), grover24 (24-qubit
+Grover search in both forms), batch512 (512 uneven circuits in the batching form), measure28
+(marginal probabilities, norm and measurement on a 28-qubit complex128 state), func28 (func_apply
+on a 28-qubit complex128 state) and dense28 (dense operators on 4..8 targets). --workload NAME runs one of them alone as its own JSON line
+(circuit30: the headline alone).
 
 --no-cpu-baseline leaves out the CPU leg (for profiler runs); it does not change the timed work.
 """
@@ -40,15 +43,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, published
+MFMA_BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 matrix, published
+FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 (vector and matrix), published
 
 # HBM traffic of one k_fused6 launch on a 30-qubit complex64 state (8 GiB), measured by two
-# rocprofv3 --pmc runs of `bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-plain
-# --copy-calibration-gib 1` (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch, mean over the 198
-# block launches of the run). Each counter is scaled by what it reads for the 1 GiB device-to-device
-# copy in the same run: FETCH_SIZE reads 524314.1 KiB for 1048576 KiB moved (x2.0), WRITE_SIZE
-# 1048576 KiB (x1.0). Commands and per-kernel figures: profiles/k_fused6_circuit30_depth40.txt.
-FUSED6_PMC_30Q = {"fetch_kib": 4195892.4, "fetch_scale": 1048576.0 / 524314.125,
-                  "write_kib": 8388629.0, "write_scale": 1.0}
+# rocprofv3 --pmc runs of `bench.py --workload circuit30 --steps 1 --warmup 0 --no-cpu-baseline
+# --no-plain --copy-calibration-gib 1` on the current kernel (FETCH_SIZE, then WRITE_SIZE; KiB per
+# dispatch, mean over the 198 block launches of the run). Each counter is scaled by what it reads for
+# the 1 GiB device-to-device copy in the same run: FETCH_SIZE reads 524314.6 KiB for 1048576 KiB moved
+# (x2.0), WRITE_SIZE 1048576 KiB (x1.0). A PMC pass cannot run inside the timed process, so the
+# figure is a constant here. Commands and per-kernel figures: profiles/k_fused6_circuit30_depth40.txt.
+FUSED6_PMC_30Q = {"fetch_kib": 4196076.3, "fetch_scale": 1048576.0 / 524314.6,
+                  "write_kib": 8388628.6, "write_scale": 1.0}
 
 
 # HBM traffic of one k_marg_pass launch on a 28-qubit complex128 state (4 GiB), measured the same way
@@ -145,7 +151,7 @@ def fused_pattern_floor_ms(be, ops, ev):
     return ev.elapsed_ms(t0, t1) / len(blocks)
 
 
-def plain_dominant_roofline(be, n, ev):
+def plain_dominant_roofline(be, n, ev, amp: int = 16):
     """Average duration of the plain single-qubit dense sweep (the plain path's dominant kernel:
     1200 of 1780 gates) over one gate per qubit, from HIP events on the state's stream."""
     import numpy
@@ -158,8 +164,9 @@ def plain_dominant_roofline(be, n, ev):
         be.kronselect_dot(d)
     ev.record(b, be.stream)
     ms = ev.elapsed_ms(a, b) / len(ops)
-    bytes_per = 2 * 16 * 2 ** n
-    return {"kernel": "k_dense<double,1,1> (single-qubit dense, complex128)", "avg_ms": ms,
+    bytes_per = 2 * amp * 2 ** n
+    kind = "double, complex128" if amp == 16 else "float, complex64"
+    return {"kernel": "k_dense<{}> (single-qubit dense sweep)".format(kind), "avg_ms": ms,
             "bytes_per_launch": bytes_per, "achieved_gbs": bytes_per / (ms * 1e-3) / 1e9}
 
 
@@ -249,12 +256,21 @@ def bench_circuit28(args):
         states[name] = be
     res["infidelity_c64_vs_c128"] = infidelity(states["c128"], states["c64"])
     res["max_abs_err_c64_vs_c128"] = max_abs_diff(states["c128"], states["c64"])
+    ev = HipEvents()
+    dom = {name: plain_dominant_roofline(states[name], n, ev, amp) for name, amp in (("c128", 16), ("c64", 8))}
+    for be in states.values():
+        be.close()
     cfg = {"workload": "28-qubit depth-20 layered random circuit, plain complex128 vs plain complex64 (one sweep "
                        "per gate)", "n_qubits": n, "depth": depth, "gates": len(ops), "seed": args.seed}
+    d = dom["c128"]
     out = _line("28-qubit gates/s, plain c128 vs unfused c64", res["c128"]["gates_per_s"], "gates/s", 1, 1,
                 res["c128"]["s_per_circuit"] * 1e3, "c128", cfg, plain_c128=res["c128"], plain_c64=res["c64"],
                 infidelity_c64_vs_c128=res["infidelity_c64_vs_c128"],
-                max_abs_err_c64_vs_c128=res["max_abs_err_c64_vs_c128"])
+                max_abs_err_c64_vs_c128=res["max_abs_err_c64_vs_c128"],
+                roofline={"bound": "hbm", "kernel": d["kernel"], "achieved": d["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": d["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
+                          "bytes_per_launch": d["bytes_per_launch"], "avg_launch_ms": d["avg_ms"]},
+                roofline_c64=dict(dom["c64"], frac=dom["c64"]["achieved_gbs"] / HBM_PEAK_GBS))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ops, n, args.cpu_seconds)
     return out
@@ -296,16 +312,46 @@ def bench_grover24(args):
                      "max_abs_err_vs_closed_form": err, "measured": m, "measured_prob": p,
                      "found_marked": m == x_star,
                      "algorithmic_gbs": alg * t_iter / dt / 1e9, "hbm_frac": alg * t_iter / dt / 1e9 / HBM_PEAK_GBS,
-                     "touched_gbs": touched * t_iter / dt / 1e9}
+                     "touched_gbs": touched * t_iter / dt / 1e9,
+                     "product_pass": product_pass_roofline(be, ns, 16 if name == "plain_c128" else 8)}
         be.close()
     cfg = {"workload": "24-qubit iterative Grover: 23 search qubits + |-> ancilla, F(x==x*) oracle, {} iterations, "
                        "seeded measurement".format(t_iter), "n_qubits": ns + 1, "iterations": t_iter,
            "x_star": x_star, "seed": args.seed}
+    r = res["plain_c128"]["product_pass"]
     out = _line("24-qubit Grover iterations/s (plain c128, H/X/MCZ diffusion)", res["plain_c128"]["iterations_per_s"],
-                "iterations/s", 1, 1, res["plain_c128"]["s_total"] * 1e3, "c128", cfg, **res)
+                "iterations/s", 1, 1, res["plain_c128"]["s_total"] * 1e3, "c128", cfg, roofline=dict(r, traffic=None),
+                **res)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_grover(ns, x_star, args.cpu_seconds)
     return out
+
+
+def product_pass_roofline(be, ns: int, amp: int):
+    """The Grover iteration's dominant kernel, the product sweep (k_product): an H layer on the ns
+    search qubits is three passes over the whole state; HIP events on the state's stream around ten
+    such layers (an even count, so the state is unchanged)."""
+    import numpy
+    from qip_amd import GpuBackend
+    H = numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2)
+    targets, mats = list(range(ns)), [H] * ns
+    ev = HipEvents()
+    be.flush() if hasattr(be, "flush") else None
+    GpuBackend.apply_product(be, targets, mats)        # launched directly, also on a recording backend
+    GpuBackend.apply_product(be, targets, mats)
+    be.synchronize()
+    a, b = ev.create(), ev.create()
+    ev.record(a, be.stream)
+    for _ in range(10):
+        GpuBackend.apply_product(be, targets, mats)
+    ev.record(b, be.stream)
+    passes = 10 * -(-(ns - 3) // 8)
+    ms = ev.elapsed_ms(a, b) / passes
+    bytes_per = 2 * amp * 2 ** (ns + 1)
+    gbs = bytes_per / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": "k_product<{}> (single-qubit factors of an H layer, up to 12 per pass)".format(
+                "double" if amp == 16 else "float"), "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS, "bytes_per_launch": bytes_per, "avg_launch_ms": ms}
 
 
 def grover_iteration_bytes(ns: int, amp: int, diffusion: str):
@@ -385,15 +431,38 @@ def bench_qft16(args):
     walk_ms = (time.perf_counter() - t) * 1e3
     expect = numpy.fft.ifft(psi, norm="ortho")
     err = float(numpy.max(numpy.abs(state - expect)))
+    # the circuit's most frequent kernel: 120 controlled phases (k_diag on the quarter of the state with
+    # both qubits 1), launched back to back between HIP events on a state of the same size
+    be = GpuBackend.make_state(n, [list(range(n))], [psi])
+    ev = HipEvents()
+    from qip_amd import CMat
+    descs = [{(i, t): CMat(numpy.diag([1.0, numpy.exp(2j * numpy.pi / 2.0 ** (1 + i - t))]))}
+             for t in range(n) for i in range(t + 1, n)]
+    for d in descs[:8]:
+        be.kronselect_dot(d)
+    be.synchronize()
+    a, b = ev.create(), ev.create()
+    ev.record(a, be.stream)
+    for d in descs:
+        be.kronselect_dot(d)
+    ev.record(b, be.stream)
+    diag_ms = ev.elapsed_ms(a, b) / len(descs)
+    be.close()
+    diag_bytes = 2 * 16 * 2 ** (n - 2)
     perm = [int(format(i, "016b")[::-1], 2) for i in range(2 ** n)]
     perr = float(numpy.max(numpy.abs(classic[sm] - (numpy.abs(expect) ** 2)[perm])))
     cfg = {"workload": "16-qubit QFT (qip/qfft.py: 16 H, 120 controlled phases, 8 swaps) + StochasticMeasure, graph "
                        "frontend, seeded random complex128 state", "n_qubits": n, "gates": 144, "seed": args.seed,
            "frontend": "visiting order and index groups stored by the warm-up run and replayed; every gate, the "
                        "measurement and the read-back run each time"}
+    diag_gbs = diag_bytes / (diag_ms * 1e-3) / 1e9
     out = _line("16-qubit QFT circuits/s (graph run incl. state upload and read-back)", 1.0 / dt, "circuits/s", reps,
                 1, dt * 1e3, "c128", cfg, max_abs_err_vs_ifft=err, max_abs_err_probabilities=perr,
-                ms_first_run_walking_graph=walk_ms)
+                ms_first_run_walking_graph=walk_ms,
+                roofline={"bound": "launch", "kernel": "k_diag<double,0> (controlled phase, 120 of the 144 gates)",
+                          "achieved": diag_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": diag_gbs / HBM_PEAK_GBS,
+                          "traffic": None, "bytes_per_launch": diag_bytes, "avg_launch_ms": diag_ms,
+                          "note": "a 1 MiB state stays in cache: each launch is bound by launch latency, not HBM"})
     if not args.no_cpu_baseline:
         from oracle.oracle_backend import OracleBackend
         t = time.perf_counter()
@@ -565,6 +634,89 @@ def bench_func28(args):
     return out
 
 
+def bench_dense28(args):
+    """Dense operators on 4..6 targets (k_dense_tile) on a 28-qubit complex128 state (4 GiB): the
+    targets on the lowest, the highest and scattered state bits, and one 6-target operator under a
+    control; then 8 targets (the FP64 GEMM, k_dense_gemm + scatter) on 26 qubits. Per call: HIP-event
+    time on the state's stream, algorithmic bytes 2 S 2^(n-c), FP64 flops 8 2^K 2^(n-c)."""
+    import numpy
+    from qip_amd import CMat, GpuBackend
+    ev = HipEvents()
+    rng = numpy.random.default_rng(args.seed)
+
+    def haar(d):
+        z = (rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))) / numpy.sqrt(2)
+        q, r = numpy.linalg.qr(z)
+        return q * (numpy.diagonal(r) / numpy.abs(numpy.diagonal(r)))
+
+    reps = max(3, args.steps)
+    cases = {}
+
+    def run_case(be, n, name, qubits, controls, K):
+        m = haar(2 ** K)
+        for _ in range(controls):
+            m = CMat(m)
+        d = {tuple(qubits): m}
+        be.kronselect_dot(d)
+        be.synchronize()
+        a, b = ev.create(), ev.create()
+        ev.record(a, be.stream)
+        for _ in range(reps):
+            be.kronselect_dot(d)
+        ev.record(b, be.stream)
+        ms = ev.elapsed_ms(a, b) / reps
+        nb = 2 * 16 * 2 ** (n - controls)
+        fl = 8 * 2 ** K * 2 ** (n - controls)
+        cases[name] = {"n": n, "qubits": list(qubits), "controls": controls, "targets": K, "ms": ms,
+                       "algorithmic_gbs": nb / ms / 1e6, "hbm_frac": nb / ms / 1e6 / HBM_PEAK_GBS,
+                       "fp64_tflops": fl / ms / 1e9, "fp64_frac": fl / ms / 1e9 / FP64_PEAK_TFLOPS}
+
+    n = 28
+    be = GpuBackend.make_state(n, [], [])
+    be.kronselect_dot({q: numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2) for q in range(n)})
+    for K in (4, 5, 6):
+        run_case(be, n, "k{}_low".format(K), list(range(n - K, n)), 0, K)
+        run_case(be, n, "k{}_high".format(K), list(range(K)), 0, K)
+        run_case(be, n, "k{}_scattered".format(K), [int(q) for q in numpy.linspace(1, n - 2, K).round()], 0, K)
+    run_case(be, n, "k6_controlled", [3, 7, 11, 15, 19, 23, 26], 1, 6)
+    be.close()
+    be = GpuBackend.make_state(26, [], [])
+    be.kronselect_dot({q: numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2) for q in range(26)})
+    run_case(be, 26, "k8_gemm_scattered", [0, 3, 6, 9, 12, 15, 18, 25], 0, 8)
+    be.close()
+    tiles = [c for k, c in cases.items() if k.startswith(("k4", "k5", "k6")) and c["controls"] == 0]
+    value = sum(c["algorithmic_gbs"] for c in tiles) / len(tiles)
+    cfg = {"workload": "dense operators on a 28-qubit complex128 state: 4, 5 and 6 targets on low, high and "
+                       "scattered qubits (value: their mean), 6 targets under one control; 8 targets on 26 qubits",
+           "seed": args.seed}
+    k6 = [cases[k] for k in ("k6_low", "k6_high", "k6_scattered")]
+    k6_gbs = sum(c["algorithmic_gbs"] for c in k6) / 3
+    out = _line("28-qubit c128 dense 4-6 target operators: algorithmic GB/s", value, "GB/s", reps, 1,
+                sum(c["ms"] for c in tiles) / len(tiles), "c128", cfg, cases=cases,
+                roofline={"bound": "hbm", "kernel": "k_dense_tile<double,K> (K = 4..6, LDS tiles)", "achieved": value,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": value / HBM_PEAK_GBS, "traffic": None,
+                          "k6_fp64": {"achieved_gbs": k6_gbs,
+                                      "fp64_tflops": sum(c["fp64_tflops"] for c in k6) / 3,
+                                      "fp64_frac": sum(c["fp64_frac"] for c in k6) / 3,
+                                      "note": "at 6 targets the FP64 arithmetic (256 flop per 32 B moved) is the "
+                                              "bound, not HBM"}})
+    if not args.no_cpu_baseline:
+        from oracle import oracle
+        nc = 22
+        v = numpy.zeros(2 ** nc, dtype=numpy.complex128)
+        v[0] = 1.0
+        o = numpy.zeros_like(v)
+        d = {tuple(range(nc - 4, nc)): haar(16)}
+        cores = len(os.sched_getaffinity(0))
+        t = time.perf_counter()
+        oracle.kronselect_dot(d, v, nc, o, fast=True, nthreads=cores)
+        cdt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": 2 * 16 * 2 ** nc / cdt / 1e9, "unit": "GB/s", "cores": cores, "kind": "port",
+                               "sample": "one 4-target dense descriptor on a 22-qubit state on the oracle's cdot_loop "
+                                         "restatement (-O3 -ffast-math -fopenmp, all cores), {:.2f} s".format(cdt)}
+    return out
+
+
 def bench_batch512(args):
     """512 independent layered random circuits, n ~ U{10..22}, depth ~ U{8..32} (seeded), complex64:
     the batching form (one captured graph per run, autotuned) against the plain complex64 path with
@@ -645,7 +797,12 @@ def bench_batch512(args):
                 batch={"circuits_per_s": circuits_s, "gpu_ms_per_run": run_ms, "algorithmic_bytes_per_run":
                        info["algorithmic_bytes"], "effective_gbs": eff_gbs, "hbm_frac": eff_gbs / HBM_PEAK_GBS,
                        "generate_s": gen_s, "load_plan_capture_s": load_s, "autotune_s": tune_s,
-                       "autotuned_run_ms": tuned_ms})
+                       "autotuned_run_ms": tuned_ms},
+                roofline={"bound": "hbm", "kernel": "one batch run (k_batch6 waves + LDS-resident small states)",
+                          "achieved": eff_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": eff_gbs / HBM_PEAK_GBS,
+                          "traffic": None, "bytes_per_launch": info["algorithmic_bytes"], "avg_launch_ms": run_ms,
+                          "note": "the whole captured graph as one unit: algorithmic bytes of every state's passes "
+                                  "over the run's GPU time (per-kernel figures: profiles/k_batch6_batch512.txt)"})
     if plain:
         out.update(plain_per_gate_c64={"circuits_per_s": count / plain["c64"], "s_per_batch": plain["c64"]},
                    plain_per_gate_c128={"circuits_per_s": count / plain["c128"], "s_per_batch": plain["c128"]},
@@ -686,38 +843,19 @@ def cpu_baseline_batch(ns, gates, seconds: float):
                       "whole batch".format(circuits, spent)}
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=30)
-    ap.add_argument("--depth", type=int, default=40)
-    ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-plain", action="store_true", help="leave out the plain complex128 comparison run")
-    ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
-                    help="after the benchmark, one device-to-device copy of this many GiB (a kernel of known "
-                         "traffic for calibrating hardware byte counters)")
-    ap.add_argument("--workload", default="circuit30",
-                    choices=["circuit30", "circuit28", "grover24", "qft16", "batch512", "measure28", "func28"],
-                    help="circuit30 (default, the headline) or one of the other configurations, each printed as "
-                         "its own JSON line")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py runs on one GPU only: the engine holds the whole state on a single MI355X; "
-              "a multi-GPU run is not part of this build (--gpus {} requested)".format(args.gpus), file=sys.stderr)
-        sys.exit(2)
-    if args.workload != "circuit30":
-        result = {"circuit28": bench_circuit28, "grover24": bench_grover24, "qft16": bench_qft16,
-                  "batch512": bench_batch512, "measure28": bench_measure28,
-                  "func28": bench_func28}[args.workload](args)
-        print(json.dumps(result), flush=True)
-        if args.copy_calibration_gib > 0:
-            copy_calibration(HipEvents(), int(args.copy_calibration_gib * 2 ** 30))
-        return
+def mfma_fraction(n: int, block_ms: float):
+    """The fused kernel against the dense BF16 MFMA peak: algorithmic flops of a 6-qubit block, 8 * 64
+    real flops per amplitude (a 64 x 64 complex matrix per group of 64 amplitudes), and the flops the
+    split-BF16 3-product form executes (three times as many)."""
+    flops = 8 * 64 * 2 ** n
+    tf = flops / (block_ms * 1e-3) / 1e12
+    return {"bound": "mfma (secondary: the kernel is HBM-bound)", "achieved": tf, "peak": MFMA_BF16_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": tf / MFMA_BF16_PEAK_TFLOPS, "flops_per_launch": flops,
+            "executed_tflops": 3 * tf, "executed_frac": 3 * tf / MFMA_BF16_PEAK_TFLOPS}
 
+
+def bench_circuit30(args):
+    """The headline (module docstring): fused complex64 against plain complex128 at 30 qubits."""
     import numpy
     from qip_amd import GpuBackend, infidelity, max_abs_diff
     from qip_amd.fused import FusedBackend
@@ -776,6 +914,7 @@ def main():
             "achieved": fused_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": fused_gbs / HBM_PEAK_GBS,
             "traffic": fused6_traffic_bytes(n), "bytes_per_launch": block_bytes, "avg_launch_ms": avg_block_ms,
         },
+        "roofline_mfma": mfma_fraction(n, avg_block_ms),
         "fused_c64": {"gates_per_s": fused_gates_s, "s_per_circuit": elapsed / args.steps,
                       "gpu_ms_per_circuit": avg_block_ms * (nblocks + 1), "blocks": nblocks,
                       "algorithmic_gb_per_circuit": block_bytes * (nblocks + 1) / 1e9,
@@ -817,9 +956,64 @@ def main():
         log("cpu baseline: sampling")
         result["cpu_baseline"] = cpu_baseline(ops, n, args.cpu_seconds)
 
-    if args.copy_calibration_gib > 0:
-        copy_calibration(ev, int(args.copy_calibration_gib * 2 ** 30))
+    return result
 
+
+# the other workloads, run after the headline in the default invocation, nested under "configs"
+SECONDARY = (("qft16", "bench_qft16"), ("circuit28", "bench_circuit28"), ("grover24", "bench_grover24"),
+             ("batch512", "bench_batch512"), ("measure28", "bench_measure28"), ("func28", "bench_func28"),
+             ("dense28", "bench_dense28"))
+_LINE_ONLY = ("n_gpus", "higher_is_better", "scaling", "vs_baseline", "data")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=30)
+    ap.add_argument("--depth", type=int, default=40)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--config-cpu-seconds", type=float, default=5.0,
+                    help="CPU-baseline work per secondary configuration in the default run")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="leave out the plain complex128 comparison run")
+    ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
+                    help="after the benchmark, one device-to-device copy of this many GiB (a kernel of known "
+                         "traffic for calibrating hardware byte counters)")
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "circuit30", "circuit28", "grover24", "qft16", "batch512", "measure28", "func28",
+                             "dense28"],
+                    help="all (default): the headline, then every other configuration nested under 'configs' in the "
+                         "same JSON line; or one workload alone, as its own JSON line")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the engine holds the whole state on a single MI355X; "
+              "a multi-GPU run is not part of this build (--gpus {} requested)".format(args.gpus), file=sys.stderr)
+        sys.exit(2)
+    log = lambda msg: print("[bench] " + msg, file=sys.stderr, flush=True)  # noqa: E731
+    funcs = dict(SECONDARY, circuit30="bench_circuit30")
+    if args.workload != "all":
+        result = globals()[funcs[args.workload]](args)
+    else:
+        t = time.perf_counter()
+        result = bench_circuit30(args)
+        result["wall_s"] = time.perf_counter() - t
+        configs = {}
+        sub = argparse.Namespace(**vars(args))
+        sub.cpu_seconds = args.config_cpu_seconds
+        for name, fn in SECONDARY:
+            log("config {}: start".format(name))
+            t = time.perf_counter()
+            line = globals()[fn](sub)
+            entry = {k: v for k, v in line.items() if k not in _LINE_ONLY}
+            entry["wall_s"] = time.perf_counter() - t
+            configs[name] = entry
+            log("config {}: {:.4g} {} in {:.1f} s".format(name, line["value"], line["unit"], entry["wall_s"]))
+        result["configs"] = configs
+    if args.copy_calibration_gib > 0:
+        copy_calibration(HipEvents(), int(args.copy_calibration_gib * 2 ** 30))
     print(json.dumps(result), flush=True)
 
 
