@@ -75,35 +75,51 @@ struct TileShape {
     static_assert(D * B == kTileElems, "tile shape");
 };
 
-template <typename R, int K>
+template <typename R, int K, bool ROWS_FAST>
 __global__ __launch_bounds__(kTileThreads) void k_dense_tile(cx<R> *amps, uint64_t nbase, uint64_t ctrl_mask,
                                                              PosList pl, const uint64_t *__restrict__ off_g,
                                                              const cx<double> *__restrict__ U_g, uint64_t ntiles) {
     using S = TileShape<K>;
     constexpr int D = S::D, B = S::B;
-    __shared__ cx<double> sU[D * D];
-    __shared__ cx<double> sX[kTileElems];
+    // Row strides of X and U: with rows across lanes, consecutive lanes write consecutive rows of X
+    // and read consecutive rows of U, which one slot of padding spreads over the banks
+    constexpr int XS = ROWS_FAST ? B + 1 : B;
+    constexpr int US = ROWS_FAST ? D + 1 : D;
+    __shared__ cx<double> sU[D * US];
+    __shared__ cx<double> sX[D * XS];
     __shared__ uint64_t sOff[D];
     const int tid = threadIdx.x;
-    for (int e = tid; e < D * D; e += kTileThreads) sU[e] = U_g[e];
+    for (int e = tid; e < D * D; e += kTileThreads) sU[(e / D) * US + e % D] = U_g[e];
     for (int e = tid; e < D; e += kTileThreads) sOff[e] = off_g[e];
     __syncthreads();
 
-    // load role: one base (b_ld) and rows j = j0 + i * (256 / B)
-    const int b_ld = tid % B;
-    const int j0 = tid / B;
+    // Element i of this thread is (row j, base b) of the tile, e = tid + 256 i. Bases across lanes
+    // (b = e mod B): a thread keeps one base. Rows across lanes (ROWS_FAST, j = e mod D; used when
+    // state bit 0 is a target, so a base's amplitudes are contiguous): a wave reads whole bases.
+    constexpr int BSTEP = kTileThreads / D;
     constexpr int JSTEP = kTileThreads / B;
-    // compute role: rows rg + RG * r, bases bg + BG * q
-    const int rg = tid / S::BG;
-    const int bg = tid % S::BG;
+    const int b_ld = ROWS_FAST ? tid / D : tid % B;
+    const int j_ld = ROWS_FAST ? tid % D : tid / B;
+    // compute role: rows rg + RG * r, bases bg + BG * q; with ROWS_FAST the row group varies fastest
+    // across lanes, so the stores of one instruction cover whole runs of rows
+    const int rg = ROWS_FAST ? tid % S::RG : tid / S::BG;
+    const int bg = ROWS_FAST ? tid / S::RG : tid % S::BG;
 
     cx<R> pre[S::LOADS];
     auto load_tile = [&](uint64_t t) {
-        const uint64_t g = t * B + b_ld;
-        if (g < nbase) {
-            const uint64_t base = base_of(g, pl, ctrl_mask);
+        // bases past the end load a valid amplitude (their products are not stored): unconditional
+        // loads stay in flight together
+        if constexpr (ROWS_FAST) {
 #pragma unroll
-            for (int i = 0; i < S::LOADS; ++i) pre[i] = amps[base + sOff[j0 + i * JSTEP]];
+            for (int i = 0; i < S::LOADS; ++i) {
+                const uint64_t g = t * B + b_ld + BSTEP * i;
+                pre[i] = amps[base_of(g < nbase ? g : nbase - 1, pl, ctrl_mask) + sOff[j_ld]];
+            }
+        } else {
+            const uint64_t g = t * B + b_ld;
+            const uint64_t base = base_of(g < nbase ? g : nbase - 1, pl, ctrl_mask);
+#pragma unroll
+            for (int i = 0; i < S::LOADS; ++i) pre[i] = amps[base + sOff[j_ld + i * JSTEP]];
         }
     };
     uint64_t t = blockIdx.x;
@@ -111,7 +127,10 @@ __global__ __launch_bounds__(kTileThreads) void k_dense_tile(cx<R> *amps, uint64
     for (; t < ntiles; t += gridDim.x) {
         __syncthreads();                       // the previous tile's products have read sX
 #pragma unroll
-        for (int i = 0; i < S::LOADS; ++i) sX[(j0 + i * JSTEP) * B + b_ld] = to_d(pre[i]);
+        for (int i = 0; i < S::LOADS; ++i) {
+            if constexpr (ROWS_FAST) sX[j_ld * XS + b_ld + BSTEP * i] = to_d(pre[i]);
+            else sX[(j_ld + i * JSTEP) * XS + b_ld] = to_d(pre[i]);
+        }
         __syncthreads();
         if (t + gridDim.x < ntiles) load_tile(t + gridDim.x);   // in flight during the products
         cx<double> acc[S::RB][S::BB];
@@ -122,9 +141,9 @@ __global__ __launch_bounds__(kTileThreads) void k_dense_tile(cx<R> *amps, uint64
         for (int j = 0; j < D; ++j) {
             cx<double> u[S::RB], x[S::BB];
 #pragma unroll
-            for (int r = 0; r < S::RB; ++r) u[r] = sU[(rg + S::RG * r) * D + j];
+            for (int r = 0; r < S::RB; ++r) u[r] = sU[(rg + S::RG * r) * US + j];
 #pragma unroll
-            for (int q = 0; q < S::BB; ++q) x[q] = sX[j * B + bg + S::BG * q];
+            for (int q = 0; q < S::BB; ++q) x[q] = sX[j * XS + bg + S::BG * q];
 #pragma unroll
             for (int r = 0; r < S::RB; ++r)
 #pragma unroll
@@ -345,7 +364,7 @@ PosList poslist(std::vector<int> pos) {
     std::sort(pos.begin(), pos.end());
     PosList pl;
     pl.count = (int32_t)pos.size();
-    for (size_t i = 0; i < pos.size(); ++i) pl.p[i] = (uint8_t)pos[i];
+    for (size_t i = 0; i < pos.size(); ++i) pl.set((int)i, (uint32_t)pos[i]);
     return pl;
 }
 
@@ -374,11 +393,16 @@ int launch_tile(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_ma
     if (rc) return rc;
     const uint64_t nbase = s->len >> allpos.size();
     const uint64_t ntiles = (nbase + TileShape<K>::B - 1) / TileShape<K>::B;
-    const int per_cu = K == 6 ? 1 : 2;       // LDS: 128 KB at K = 6, 80 KB and 68 KB below
+    const int per_cu = K == 6 ? 1 : 2;       // LDS: up to 134 KB at K = 6, 82 KB and 69 KB below
     const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)device_cus() * per_cu);
-    hipLaunchKernelGGL((k_dense_tile<R, K>), dim3((uint32_t)grid), dim3(kTileThreads), 0, s->stream,
-                       (cx<R> *)s->amps, nbase, ctrl_mask, poslist(allpos), (const uint64_t *)d,
-                       (const cx<double> *)((char *)d + sizeof(uint64_t) * D), ntiles);
+    const bool rows_fast = std::find(tbits.begin(), tbits.end(), 0) != tbits.end();
+    const cx<double> *dU = (const cx<double> *)((char *)d + sizeof(uint64_t) * D);
+    if (rows_fast)
+        hipLaunchKernelGGL((k_dense_tile<R, K, true>), dim3((uint32_t)grid), dim3(kTileThreads), 0, s->stream,
+                           (cx<R> *)s->amps, nbase, ctrl_mask, poslist(allpos), (const uint64_t *)d, dU, ntiles);
+    else
+        hipLaunchKernelGGL((k_dense_tile<R, K, false>), dim3((uint32_t)grid), dim3(kTileThreads), 0, s->stream,
+                           (cx<R> *)s->amps, nbase, ctrl_mask, poslist(allpos), (const uint64_t *)d, dU, ntiles);
     QIP_LAUNCHED();
     return QIPGPU_OK;
 }

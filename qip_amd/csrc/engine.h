@@ -22,17 +22,23 @@ struct __attribute__((aligned(2 * sizeof(R)))) cx {
 };
 
 // Ascending list of state-bit positions at which zero bits are inserted into a compressed
-// index (the positions of target and control qubits). Lives in kernel arguments.
+// index (the positions of target and control qubits). Lives in kernel arguments, four 8-bit
+// positions per 32-bit word: a loop over them then reads the kernel arguments with scalar loads
+// (byte elements would need vector loads, whose waits also drain every load and store the thread
+// has in flight), and the arguments stay small (launch cost matters for small states).
 struct PosList {
     int32_t count;
-    uint8_t p[44];
+    uint32_t w[(QIPGPU_MAX_QUBITS + 3) / 4];
+
+    __host__ __device__ __forceinline__ uint32_t at(int i) const { return (w[i >> 2] >> (8 * (i & 3))) & 0xffu; }
+    void set(int i, uint32_t p) { w[i >> 2] = (w[i >> 2] & ~(0xffu << (8 * (i & 3)))) | (p << (8 * (i & 3))); }
 };
 
 // Insert a zero bit at each listed position (ascending), turning an enumeration index over
 // the free bits into a state index with those bits clear.
 __device__ __forceinline__ uint64_t insert_zeros(uint64_t g, const PosList &pl) {
     for (int i = 0; i < pl.count; ++i) {
-        const uint32_t p = pl.p[i];
+        const uint32_t p = pl.at(i);
         const uint64_t lo = g & ((1ull << p) - 1ull);
         g = ((g ^ lo) << 1) | lo;
     }

@@ -621,11 +621,13 @@ __global__ __launch_bounds__(256) void k_diag(const DiagArgs<R, K> a) {
     const uint64_t g0 = ordered_block(blk, a.order) * (256 * ITEMS) + threadIdx.x;
     cx<R> v[ITEMS];
     uint64_t idx[ITEMS];
+    // items past the end load a valid amplitude and store nothing: unconditional loads let the
+    // compiler keep all of them in flight (a load under a branch makes it drain every earlier one)
 #pragma unroll
     for (int it = 0; it < ITEMS; ++it) {
         const uint64_t g = g0 + (uint64_t)it * 256;
-        idx[it] = insert_zeros(g, a.pl) | a.ctrl_mask;
-        if (g < a.count) v[it] = a.amps[idx[it]];
+        idx[it] = insert_zeros(g < a.count ? g : a.count - 1, a.pl) | a.ctrl_mask;
+        v[it] = a.amps[idx[it]];
     }
 #pragma unroll
     for (int it = 0; it < ITEMS; ++it) {
@@ -717,7 +719,7 @@ static Analysed analyse(int k, const int32_t *targets, const double *mat) {
 static void fill_poslist(PosList &pl, std::vector<int> pos) {
     std::sort(pos.begin(), pos.end());
     pl.count = (int32_t)pos.size();
-    for (size_t i = 0; i < pos.size(); ++i) pl.p[i] = (uint8_t)pos[i];
+    for (size_t i = 0; i < pos.size(); ++i) pl.set((int)i, (uint32_t)pos[i]);
 }
 
 template <typename R, int K>

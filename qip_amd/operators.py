@@ -35,9 +35,9 @@ class MatrixOp(Qubit):
         self._ms_groups = None
 
     def descriptor(self, index_groups: Sequence[Sequence[int]]) -> Mapping[IndexType, MatrixType]:
-        groups = tuple(tuple(g) for g in index_groups)
-        if self.ms is None or groups != self._ms_groups:
-            mats = self.makemats(index_groups)
+        if self.ms is None or self._ms_groups != index_groups:
+            groups = [list(g) for g in index_groups]
+            mats = self.makemats(groups)
             if self.dagger:
                 mats = {k: v.conj().T for k, v in mats.items()}
             self.ms, self._ms_groups = mats, groups

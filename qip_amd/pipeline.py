@@ -169,7 +169,8 @@ def _walk(sources: Sequence[PipelineObject], nodes: AbstractSet[PipelineObject],
 
 class Schedule:
     """A graph compiled for one set of outputs and feed keys: the sources (qid order) and, when
-    every node uses the base `feed` and `remap_index`, the steps as (node, index groups)."""
+    every node uses the base `feed` and `remap_index`, the steps as (node, index groups), the
+    groups None for nodes that keep the base `feed_indices` (the identity, annotation 0)."""
 
     __slots__ = ("sources", "nodes", "n", "steps", "epoch")
 
@@ -183,8 +184,11 @@ class Schedule:
         if all(type(v).feed is PipelineObject.feed and type(v).remap_index is PipelineObject.remap_index
                for v in nodes):
             steps = []
-            _walk(self.sources, self.nodes,
-                  lambda imap, v: steps.append((v, tuple(tuple(imap[i]) for i in v.inputs))))
+
+            def keep(imap, v):
+                identity = type(v).feed_indices is PipelineObject.feed_indices
+                steps.append((v, None if identity else tuple(tuple(imap[i]) for i in v.inputs)))
+            _walk(self.sources, self.nodes, keep)
             self.steps = steps
 
     def index_ranges(self) -> Dict[PipelineObject, List[int]]:
@@ -199,10 +203,17 @@ class Schedule:
         acc = NodeFeeder(state, self.n)
         if self.steps is None:
             _walk(self.sources, self.nodes, acc.feed)
-        else:
-            for node, groups in self.steps:
-                acc.feed_groups(node, [list(g) for g in groups])
-        return acc.state.get_state(), acc.classic_map
+            return acc.state.get_state(), acc.classic_map
+        classic = acc.classic_map
+        for node, groups in self.steps:
+            if groups is None:
+                classic[node] = 0            # the base node: identity, annotation 0, no qubit removed
+            else:
+                removed, note = node.feed_indices(state, [list(g) for g in groups], acc.n)
+                if removed > 0 or note is not None:
+                    classic[node] = note
+                    acc.n -= removed
+        return state.get_state(), classic
 
 
 def _schedule_for(outputs: Tuple[PipelineObject, ...], feed_keys: Tuple) -> Schedule:
