@@ -219,11 +219,13 @@ def _wrap_controls(controls: Sequence[int], groups):
 
 
 class _CMatLevel:
-    """A C_MAT level around m (same _kron_struct and fields as operators.CMat)."""
+    """A C_MAT level around m (same _kron_struct, m and shape as operators.CMat)."""
     _kron_struct = KRON_CMAT
 
     def __init__(self, m):
         self.m = m
+        rows, cols = m.shape
+        self.shape = (2 * rows, 2 * cols)
 
 
 def _plan_component(groups, controls: List[int]) -> list:
