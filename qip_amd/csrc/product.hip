@@ -17,6 +17,9 @@
 // per amplitude for up to 12 gates, and for a complex64 state one rounding instead of one per gate.
 // LDS rows of 16 amplitudes are padded by one slot, so that the 16-amplitude-stride reads of layout
 // C spread over the banks.
+//
+// A pass whose factors all sit on u bits 8..11 needs no LDS and runs a variant without the tile,
+// three workgroups per CU for a complex64 state.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -31,15 +34,39 @@ namespace {
 constexpr int kTileBits = 12;
 constexpr int kTile = 1 << kTileBits;
 constexpr int kPadded = kTile + kTile / 16;
-constexpr int kBlocksPerCU = 2;   // 68 KB of LDS per workgroup
 
+// The factor on a tile bit is applied in one of three ways, by what its 2x2 matrix is:
+//   X, exactly [[0, 1], [1, 0]]: no arithmetic. It flips the bit of every amplitude's index and
+//     commutes with the other factors, so the tile is stored with that bit of its addresses flipped
+//     (on u bits 0..5 and 8..11, which stay within the wave that loaded them; elsewhere X is real).
+//     A pass of X factors on low bits therefore needs no LDS transposes;
+//   real: two multiplies per output component;  complex: the full complex 2x2.
 struct ProductArgs {
     uint64_t ntiles;
     uint8_t pbits[12];                 // u bit i <-> state bit pbits[i], ascending
-    uint32_t fmask;                    // u bits that carry a factor
-    uint32_t rmask;                    // u bits whose factor is a real matrix (H, X, ...)
+    uint32_t fmask;                    // u bits whose factor needs arithmetic (all but X)
+    uint32_t rmask;                    // ... of which real matrices
+    uint32_t xmask;                    // u bits whose factor is X (flipped at the store)
     cx<double> g[12][4];               // the factor on u bit i: row-major 2x2
 };
+
+// The tile between layouts, in LDS: FP64 for both state precisions (68 KB, 2 workgroups per CU).
+struct TileStore {
+    cx<double> v[kPadded];
+    // A view of the slots base, base + 1, ...: put/get take a constant offset from base, which the
+    // LDS instructions carry as an immediate (one address register per layout, not one per slot)
+    struct View {
+        cx<double> *p;
+        __device__ __forceinline__ void put(int k, cx<double> x) const { p[k] = x; }
+        __device__ __forceinline__ cx<double> get(int k) const { return p[k]; }
+    };
+    __device__ __forceinline__ View at(int base) { return View{v + base}; }
+};
+
+// Workgroups per CU: the LDS variant is bounded by its tile, the variant without LDS by registers
+// (16 FP64 amplitudes and one raw prefetch buffer per thread: 3 fit with complex64 raw data).
+template <typename R, bool kLds>
+constexpr int blocks_per_cu() { return (kLds || sizeof(R) == 8) ? 2 : 3; }
 
 __device__ __forceinline__ int ppos(uint32_t u) { return (int)(u + (u >> 4)); }
 
@@ -109,79 +136,97 @@ __device__ __forceinline__ void load_raw(cx<R> (&v)[16], const cx<R> *amps, uint
     for (int i = 0; i < 16; ++i) v[i] = amps[base + u_offset<8, 12>(256u * i, a)];
 }
 
-// One tile: its 16 raw amplitudes per thread (layout A) in, widened, the factors applied, rounded
-// and stored back.
-template <typename R>
-__device__ __forceinline__ void product_tile(const cx<R> (&v)[16], cx<R> *amps, uint64_t base, cx<double> *t,
+// One tile: its 16 amplitudes per thread (layout A, widened to FP64) in, the factors applied, rounded
+// and stored back at sbase (the tile's base plus this thread's offset with the X bits flipped).
+template <typename R, bool kLds>
+__device__ __forceinline__ void product_tile(cx<double> (&x)[16], cx<R> *amps, uint64_t sbase, TileStore *lds,
                                              const ProductArgs &a, bool use_b, bool use_c, uint32_t tid) {
-    cx<double> x[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        x[i].re = (double)v[i].re;
-        x[i].im = (double)v[i].im;
-    }
     apply4<8>(x, a);
-    if (use_b || use_c) {
+    if (kLds && (use_b || use_c)) {
+        TileStore &t = *lds;
+        // Padded LDS slots of the three layouts, a per-thread base plus a constant per register:
+        // A: ppos(tid + 256 i) = base + 272 i;  B: ppos(u_b(tid, e)) = base + 17 e;  C: ppos(u_c(tid, e)) = base + e
+        const auto ta = t.at((int)(tid + (tid >> 4)));
+        const auto tb = t.at((int)((tid & 15u) + 272u * (tid >> 4)));
+        const auto tc = t.at((int)(17u * tid));
         // Every phase reads and writes only this thread's own positions of that phase's layout, and
         // a barrier separates phases: so the previous tile's last (A) reads need no barrier before
         // these A writes, and each phase's writes go back where it read
 #pragma unroll
-        for (int i = 0; i < 16; ++i) t[ppos(tid + 256u * i)] = x[i];
+        for (int i = 0; i < 16; ++i) ta.put(272 * i, x[i]);
         __syncthreads();
 #pragma unroll
-        for (int e = 0; e < 16; ++e) x[e] = t[ppos(u_b(tid, e))];
+        for (int e = 0; e < 16; ++e) x[e] = tb.get(17 * e);
         apply4<4>(x, a);
 #pragma unroll
-        for (int e = 0; e < 16; ++e) t[ppos(u_b(tid, e))] = x[e];
+        for (int e = 0; e < 16; ++e) tb.put(17 * e, x[e]);
         if (use_c) {
             __syncthreads();
 #pragma unroll
-            for (int e = 0; e < 16; ++e) x[e] = t[ppos(u_c(tid, e))];
+            for (int e = 0; e < 16; ++e) x[e] = tc.get(e);
             apply4<0>(x, a);
 #pragma unroll
-            for (int e = 0; e < 16; ++e) t[ppos(u_c(tid, e))] = x[e];
+            for (int e = 0; e < 16; ++e) tc.put(e, x[e]);
         }
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = t[ppos(tid + 256u * i)];
+        for (int i = 0; i < 16; ++i) x[i] = ta.get(272 * i);
     }
+    // register i holds u = tid + 256 i; it goes to u xor xmask (sbase has the thread bits flipped)
+    const uint32_t xhi = (a.xmask >> 8) & 15u;
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         cx<R> o;
         o.re = (R)x[i].re;
         o.im = (R)x[i].im;
-        amps[base + u_offset<8, 12>(256u * i, a)] = o;
+        amps[sbase + u_offset<8, 12>(256u * ((uint32_t)i ^ xhi), a)] = o;
     }
 }
 
-// Persistent grid: each workgroup takes a contiguous range of tiles, with the next tile's loads in
-// flight in a second register buffer while one is processed (the last prefetch re-reads an
-// in-range tile and is not used).
-template <typename R>
-__global__ __launch_bounds__(256, 2) void k_product(cx<R> *amps, ProductArgs a) {
-    __shared__ cx<double> t[kPadded];
+// Persistent grid: each workgroup takes a contiguous range of tiles. A tile's raw amplitudes are
+// widened to FP64 registers, and the raw registers at once take the next tile's loads, which stay
+// in flight while this tile is processed (the last prefetch re-reads an in-range tile and is not
+// used).
+template <typename R, bool kLds>
+__device__ __forceinline__ void product_loop(cx<R> *amps, const ProductArgs &a, TileStore *t) {
     const uint32_t tid = threadIdx.x;
     const uint64_t off_t = u_offset<0, 8>(tid, a);       // layout A: u bits 0..7 are the thread index
+    const uint64_t off_s = u_offset<0, 8>(tid ^ (a.xmask & 0xffu), a);
     const bool use_b = (a.fmask & 0xf0u) != 0, use_c = (a.fmask & 0x0fu) != 0;
     const uint64_t per = (a.ntiles + gridDim.x - 1) / gridDim.x;
     uint64_t tile = (uint64_t)blockIdx.x * per;
     const uint64_t end = min(tile + per, a.ntiles);
     if (tile >= end) return;
     const uint64_t last = end - 1;
-    cx<R> va[16], vb[16];
-    load_raw(va, amps, tile_base(tile, a) + off_t, a);
+    cx<R> v[16];
+    load_raw(v, amps, tile_base(tile, a) + off_t, a);
     for (;;) {
-        load_raw(vb, amps, tile_base(min(tile + 1, last), a) + off_t, a);
-        product_tile(va, amps, tile_base(tile, a) + off_t, t, a, use_b, use_c, tid);
-        if (++tile >= end) break;
-        load_raw(va, amps, tile_base(min(tile + 1, last), a) + off_t, a);
-        product_tile(vb, amps, tile_base(tile, a) + off_t, t, a, use_b, use_c, tid);
-        if (++tile >= end) break;
+        cx<double> x[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            x[i].re = (double)v[i].re;
+            x[i].im = (double)v[i].im;
+        }
+        load_raw(v, amps, tile_base(min(tile + 1, last), a) + off_t, a);
+        product_tile<R, kLds>(x, amps, tile_base(tile, a) + off_s, t, a, use_b, use_c, tid);
+        if (++tile >= end) return;
     }
 }
 
-uint32_t product_grid(uint64_t ntiles) {
-    return (uint32_t)std::min<uint64_t>(ntiles, (uint64_t)device_cus() * kBlocksPerCU);
+template <typename R, bool kLds>
+__global__ __launch_bounds__(256, (blocks_per_cu<R, kLds>())) void k_product(cx<R> *amps, ProductArgs a) {
+    if constexpr (kLds) {
+        __shared__ TileStore t;
+        product_loop<R, true>(amps, a, &t);
+    } else {
+        product_loop<R, false>(amps, a, nullptr);
+    }
+}
+
+template <typename R, bool kLds>
+void launch_product(cx<R> *amps, const ProductArgs &a, hipStream_t stream) {
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(a.ntiles, (uint64_t)device_cus() * blocks_per_cu<R, kLds>());
+    hipLaunchKernelGGL((k_product<R, kLds>), dim3(grid), dim3(256), 0, stream, amps, a);
 }
 
 constexpr int kLowBits = kProductLowBits;           // state bits every pass keeps in its tile
@@ -209,22 +254,33 @@ int product_pass(qipgpu_state *s, int32_t k, const int32_t *targets, const doubl
     std::memset(&a, 0, sizeof(a));
     a.ntiles = 1ull << (n - kTileBits);
     for (int i = 0; i < kTileBits; ++i) a.pbits[i] = (uint8_t)pbits[i];
-    // each factor sits on the u bit of its target
+    // each factor sits on the u bit of its target; its kind by its matrix (the comment at ProductArgs)
     for (int j = 0; j < k; ++j) {
         const int ub = (int)(std::find(pbits.begin(), pbits.end(), tbits[j]) - pbits.begin());
-        a.fmask |= 1u << ub;
         const double *m = mats + 8 * j;
-        if (m[1] == 0.0 && m[3] == 0.0 && m[5] == 0.0 && m[7] == 0.0) a.rmask |= 1u << ub;
+        const bool real = m[1] == 0.0 && m[3] == 0.0 && m[5] == 0.0 && m[7] == 0.0;
+        // An X is flipped at the store only on the bits a wave both loads and stores (u bits 0..5,
+        // its lanes, and 8..11, its registers): a flipped bit 6 or 7 (the wave index in layout A)
+        // would store into another wave's amplitudes before that wave has loaded them
+        if (real && m[0] == 0.0 && m[2] == 1.0 && m[4] == 1.0 && m[6] == 0.0 && ub != 6 && ub != 7) {
+            a.xmask |= 1u << ub;
+            continue;
+        }
+        a.fmask |= 1u << ub;
+        if (real) a.rmask |= 1u << ub;
         for (int e = 0; e < 4; ++e) {
             a.g[ub][e].re = m[2 * e];
             a.g[ub][e].im = m[2 * e + 1];
         }
     }
-    const uint32_t grid = product_grid(a.ntiles);
-    if (s->dtype == QIPGPU_C128)
-        hipLaunchKernelGGL(k_product<double>, dim3(grid), dim3(256), 0, s->stream, (cx<double> *)s->amps, a);
-    else
-        hipLaunchKernelGGL(k_product<float>, dim3(grid), dim3(256), 0, s->stream, (cx<float> *)s->amps, a);
+    const bool lds = (a.fmask & 0xffu) != 0;   // a factor below u bit 8 needs a transpose
+    if (s->dtype == QIPGPU_C128) {
+        if (lds) launch_product<double, true>((cx<double> *)s->amps, a, s->stream);
+        else launch_product<double, false>((cx<double> *)s->amps, a, s->stream);
+    } else {
+        if (lds) launch_product<float, true>((cx<float> *)s->amps, a, s->stream);
+        else launch_product<float, false>((cx<float> *)s->amps, a, s->stream);
+    }
     QIP_LAUNCHED();
     return QIPGPU_OK;
 }

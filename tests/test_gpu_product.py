@@ -75,3 +75,44 @@ def test_product_real_and_complex_factors(n, k, dtype, tol):
     ref = numpy.zeros(2 ** n, dtype=numpy.complex128)
     oracle.kronselect_dot(mats, psi, n, ref)
     assert numpy.max(numpy.abs(be.get_state() - ref)) < tol * 2 ** (-n / 2) * 50
+
+
+@pytest.mark.parametrize("n,low", [(16, [0, 1, 2, 3]), (20, [1, 2, 3]), (18, [])])
+def test_complex64_product_rounds_once(n, low):
+    """On a complex64 state the factors are applied in FP64 and the tile crosses LDS between register
+    layouts as an FP32 high and a BF16 low part: every component must come out within one FP32 ulp of
+    the FP64 product of the same complex64 input (H, X and complex factors on the tile bits of all
+    three layouts, i.e. on the four lowest state bits and eight high ones)."""
+    psi = (RNG.standard_normal(2 ** n) + 1j * RNG.standard_normal(2 ** n)).astype(numpy.complex64)
+    psi /= numpy.float32(numpy.linalg.norm(psi))
+    bits = low + list(range(n - 8, n))
+    qs = [n - 1 - b for b in bits]
+    H = numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2)
+    X = numpy.array([[0.0, 1.0], [1.0, 0.0]])
+    mats = {q: (H, X, rand_u2())[i % 3] for i, q in enumerate(qs)}
+    be = GpuBackend.make_state(n, [list(range(n))], [psi], statetype=numpy.complex64)
+    be.kronselect_dot(mats)
+    got = be.get_state()
+    ref = psi.astype(numpy.complex128).reshape((2,) * n)
+    for q, m in mats.items():            # qubit q is axis q
+        ref = numpy.moveaxis(numpy.tensordot(m, ref, axes=([1], [q])), 0, q)
+    ref = ref.reshape(-1)
+    for part in (numpy.real, numpy.imag):
+        r = part(ref)
+        ulp = numpy.spacing(numpy.abs(r).astype(numpy.float32)).astype(numpy.float64)
+        assert numpy.all(numpy.abs(part(got).astype(numpy.float64) - r) <= ulp)
+
+
+@pytest.mark.parametrize("dtype", [numpy.complex128, numpy.complex64])
+@pytest.mark.parametrize("n", [16, 22])
+def test_x_on_every_qubit_is_the_index_reversal(n, dtype):
+    """X on all n qubits maps amplitude i to i xor (2^n - 1): the state reversed, bit for bit. X
+    factors cost no arithmetic on most tile bits (the store addresses are flipped) and are real
+    factors on the others; both must give exact values, across many workgroups at once."""
+    psi = (RNG.standard_normal(2 ** n) + 1j * RNG.standard_normal(2 ** n)).astype(dtype)
+    X = numpy.array([[0.0, 1.0], [1.0, 0.0]])
+    be = GpuBackend.make_state(n, [list(range(n))], [psi], statetype=dtype)
+    be.kronselect_dot({q: X for q in range(n)})
+    assert numpy.array_equal(be.get_state(), psi[::-1])
+    be.kronselect_dot({q: X for q in range(n)})
+    assert numpy.array_equal(be.get_state(), psi)
