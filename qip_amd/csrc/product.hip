@@ -3,23 +3,27 @@
 //
 // A descriptor such as H on a whole register (qip/operators.py:97-99) is one operator in the
 // reference, applied in one cdot_loop sweep. Here its single-qubit factors commute and are applied
-// tile by tile: the state is cut into tiles of 4096 amplitudes spanning the target bits plus the
-// lowest free bits (12 tile bits P, u = tile bits ascending). Every pass keeps the four lowest
-// state bits in P, so a tile is read in runs of at least 16 amplitudes: a pass takes at most 8
-// targets above bit 3, and a product with more is applied in several passes.
+// tile by tile: the state is cut into tiles of 4096 amplitudes spanning 12 tile bits, numbered u.
+// u bits 0..3 are always state bits 0..3, so a tile is read in runs of at least 16 amplitudes: a
+// pass takes at most 8 targets above bit 3, and a product with more is applied in several passes.
+// The other targets and the lowest free state bits fill u bits 4..11, placed by what each factor
+// needs (product_pass).
 //
 // Each thread holds 16 amplitudes of the tile in FP64 registers, and a factor on u bit b is applied
 // where b is one of the four bits that vary across those 16:
 //   layout A (as loaded and stored, address order): u = tid + 256 e, u bits 8..11 = e;
 //   layout B: u bits 4..7 = e;  layout C: u bits 0..3 = e.
-// The tile moves A -> B -> C -> A through LDS only as far as the factors need: factors on u bits
-// 8..11 only cost no LDS traffic at all, factors on 4..11 two transposes. One HBM read and one write
-// per amplitude for up to 12 gates, and for a complex64 state one rounding instead of one per gate.
-// LDS rows of 16 amplitudes are padded by one slot, so that the 16-amplitude-stride reads of layout
-// C spread over the banks.
+// The tile moves through LDS only as far as the factors need: factors on u bits 8..11 cost no LDS
+// traffic at all; A -> B -> A, A -> C -> A (two transposes) or A -> B -> C -> A (three) for the
+// others. One HBM read and one write per amplitude for up to 12 gates, and for a complex64 state one
+// rounding instead of one per gate. LDS rows of 16 amplitudes are padded by one slot, so that the
+// 16-amplitude-stride reads of layout C spread over the banks.
 //
 // A pass whose factors all sit on u bits 8..11 needs no LDS and runs a variant without the tile,
-// three workgroups per CU for a complex64 state.
+// three workgroups per CU for a complex64 state. Such passes are latency-bound rather than
+// bandwidth-bound: the 68 KB FP64 tile and 176 VGPRs hold the LDS variant to 8 waves per CU, which
+// do not cover the barrier-separated phases (on a 24-qubit complex64 pass over all three layouts,
+// LDS is busy about a fifth of the time and the FP64 units about a third).
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -43,7 +47,8 @@ constexpr int kPadded = kTile + kTile / 16;
 //   real: two multiplies per output component;  complex: the full complex 2x2.
 struct ProductArgs {
     uint64_t ntiles;
-    uint8_t pbits[12];                 // u bit i <-> state bit pbits[i], ascending
+    uint8_t pbits[12];                 // u bit i <-> state bit pbits[i]
+    uint8_t zbits[12];                 // the same state bits, ascending (tile bases)
     uint32_t fmask;                    // u bits whose factor needs arithmetic (all but X)
     uint32_t rmask;                    // ... of which real matrices
     uint32_t xmask;                    // u bits whose factor is X (flipped at the store)
@@ -73,7 +78,7 @@ __device__ __forceinline__ int ppos(uint32_t u) { return (int)(u + (u >> 4)); }
 __device__ __forceinline__ uint64_t tile_base(uint64_t t, const ProductArgs &a) {
 #pragma unroll
     for (int i = 0; i < kTileBits; ++i) {
-        const uint32_t p = a.pbits[i];
+        const uint32_t p = a.zbits[i];
         const uint64_t lo = t & ((1ull << p) - 1ull);
         t = ((t ^ lo) << 1) | lo;
     }
@@ -155,13 +160,15 @@ __device__ __forceinline__ void product_tile(cx<double> (&x)[16], cx<R> *amps, u
 #pragma unroll
         for (int i = 0; i < 16; ++i) ta.put(272 * i, x[i]);
         __syncthreads();
+        if (use_b) {
 #pragma unroll
-        for (int e = 0; e < 16; ++e) x[e] = tb.get(17 * e);
-        apply4<4>(x, a);
+            for (int e = 0; e < 16; ++e) x[e] = tb.get(17 * e);
+            apply4<4>(x, a);
 #pragma unroll
-        for (int e = 0; e < 16; ++e) tb.put(17 * e, x[e]);
-        if (use_c) {
-            __syncthreads();
+            for (int e = 0; e < 16; ++e) tb.put(17 * e, x[e]);
+            if (use_c) __syncthreads();
+        }
+        if (use_c) {        // from B, or straight from A when no factor sits on u bits 4..7
 #pragma unroll
             for (int e = 0; e < 16; ++e) x[e] = tc.get(e);
             apply4<0>(x, a);
@@ -243,26 +250,59 @@ int product_pass(qipgpu_state *s, int32_t k, const int32_t *targets, const doubl
         tbits[j] = n - 1 - targets[j];
         tmask |= 1ull << tbits[j];
     }
-    std::vector<int> pbits;
-    for (int b = 0; b < n && (int)pbits.size() < kTileBits; ++b)
-        if (!(tmask >> b & 1)) pbits.push_back(b);
-    // keep the k targets, fill with the lowest free bits, sort
-    pbits.resize(kTileBits - k);
-    for (int j = 0; j < k; ++j) pbits.push_back(tbits[j]);
-    std::sort(pbits.begin(), pbits.end());
+    auto is_x = [&](int j) {
+        const double *m = mats + 8 * j;
+        return m[0] == 0.0 && m[1] == 0.0 && m[2] == 1.0 && m[3] == 0.0 && m[4] == 1.0 && m[5] == 0.0 &&
+               m[6] == 0.0 && m[7] == 0.0;
+    };
+    // u bits 0..3 are state bits 0..3, which every pass keeps. The targets above them go on u bits
+    // 4..11 by kind: a factor that needs arithmetic takes the register bits 8..11 of layout A first
+    // (applied without a transpose), then 6, 7, 4, 5; an X takes the lane bits 4, 5 first, then
+    // 8..11, where it is flipped at the store, and 6, 7 last (where it is arithmetic). The free u bits
+    // take the lowest free state bits.
+    int ubit[kTileBits];
+    std::fill(ubit, ubit + kTileBits, -1);
+    for (int b = 0; b < kLowBits; ++b) ubit[b] = b;
+    std::vector<int> place(k, -1);
+    for (int j = 0; j < k; ++j)
+        if (tbits[j] < kLowBits) place[j] = tbits[j];
+    auto take = [&](int j, std::initializer_list<int> order) {
+        for (int u : order)
+            if (ubit[u] < 0) {
+                ubit[u] = tbits[j];
+                place[j] = u;
+                return;
+            }
+    };
+    for (int j = 0; j < k; ++j)
+        if (place[j] < 0 && !is_x(j)) take(j, {8, 9, 10, 11});
+    for (int j = 0; j < k; ++j)
+        if (place[j] < 0 && is_x(j)) take(j, {4, 5, 8, 9, 10, 11});
+    for (int j = 0; j < k; ++j)
+        if (place[j] < 0) take(j, {6, 7, 4, 5});
+    for (int b = kLowBits, u = kLowBits; u < kTileBits; ++u) {
+        if (ubit[u] >= 0) continue;
+        while (tmask >> b & 1) ++b;
+        ubit[u] = b++;
+    }
     ProductArgs a;
     std::memset(&a, 0, sizeof(a));
     a.ntiles = 1ull << (n - kTileBits);
-    for (int i = 0; i < kTileBits; ++i) a.pbits[i] = (uint8_t)pbits[i];
-    // each factor sits on the u bit of its target; its kind by its matrix (the comment at ProductArgs)
+    std::vector<int> sorted(ubit, ubit + kTileBits);
+    std::sort(sorted.begin(), sorted.end());
+    for (int i = 0; i < kTileBits; ++i) {
+        a.pbits[i] = (uint8_t)ubit[i];
+        a.zbits[i] = (uint8_t)sorted[i];
+    }
+    // each factor's kind by its matrix (the comment at ProductArgs)
     for (int j = 0; j < k; ++j) {
-        const int ub = (int)(std::find(pbits.begin(), pbits.end(), tbits[j]) - pbits.begin());
+        const int ub = place[j];
         const double *m = mats + 8 * j;
         const bool real = m[1] == 0.0 && m[3] == 0.0 && m[5] == 0.0 && m[7] == 0.0;
         // An X is flipped at the store only on the bits a wave both loads and stores (u bits 0..5,
         // its lanes, and 8..11, its registers): a flipped bit 6 or 7 (the wave index in layout A)
         // would store into another wave's amplitudes before that wave has loaded them
-        if (real && m[0] == 0.0 && m[2] == 1.0 && m[4] == 1.0 && m[6] == 0.0 && ub != 6 && ub != 7) {
+        if (is_x(j) && ub != 6 && ub != 7) {
             a.xmask |= 1u << ub;
             continue;
         }

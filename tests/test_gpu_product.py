@@ -116,3 +116,32 @@ def test_x_on_every_qubit_is_the_index_reversal(n, dtype):
     assert numpy.array_equal(be.get_state(), psi[::-1])
     be.kronselect_dot({q: X for q in range(n)})
     assert numpy.array_equal(be.get_state(), psi)
+
+
+@pytest.mark.parametrize("dtype,tol", [(numpy.complex128, 1e-13), (numpy.complex64, 1e-6)])
+@pytest.mark.parametrize("low,high", [
+    ("xxxx", "xxxxxxxx"),     # X on every tile bit: two of them on the wave bits, as arithmetic
+    ("ccc", "hhhhhhxx"),      # more factors that need arithmetic than register bits
+    ("", "hhxxxxxx"),         # more X than lane bits
+    ("", "cccc"),             # the register bits only: no LDS
+    ("chh", ""),              # the low bits only: straight to layout C and back
+    ("hc", "cccc"),           # low bits and register bits: A -> C -> A
+    ("h", "hhhhx"),           # every layout
+])
+def test_product_tile_bit_placement(low, high, dtype, tol):
+    """Factors on the four lowest state bits and on higher bits, of each kind (H real, X, complex),
+    in numbers that exercise every placement of targets on tile bits and every LDS path."""
+    n = 16
+    psi = RNG.standard_normal(2 ** n) + 1j * RNG.standard_normal(2 ** n)
+    psi /= numpy.linalg.norm(psi)
+    H = numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2)
+    X = numpy.array([[0.0, 1.0], [1.0, 0.0]])
+    kinds = {"h": lambda: H, "x": lambda: X, "c": rand_u2}
+    low_bits = [int(b) for b in RNG.permutation(4)[:len(low)]]
+    high_bits = [int(b) for b in 4 + RNG.permutation(n - 4)[:len(high)]]
+    mats = {n - 1 - b: kinds[c]() for b, c in zip(low_bits + high_bits, low + high)}
+    be = GpuBackend.make_state(n, [list(range(n))], [psi], statetype=dtype)
+    be.kronselect_dot(mats)
+    ref = numpy.zeros(2 ** n, dtype=numpy.complex128)
+    oracle.kronselect_dot(mats, psi, n, ref)
+    assert numpy.max(numpy.abs(be.get_state() - ref)) < tol * 2 ** (-n / 2) * 50
