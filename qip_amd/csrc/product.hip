@@ -41,9 +41,10 @@ constexpr int kPadded = kTile + kTile / 16;
 
 // The factor on a tile bit is applied in one of three ways, by what its 2x2 matrix is:
 //   X, exactly [[0, 1], [1, 0]]: no arithmetic. It flips the bit of every amplitude's index and
-//     commutes with the other factors, so the tile is stored with that bit of its addresses flipped
-//     (on u bits 0..5 and 8..11, which stay within the wave that loaded them; elsewhere X is real).
-//     A pass of X factors on low bits therefore needs no LDS transposes;
+//     commutes with the other factors, so the tile is stored with that bit of its addresses flipped.
+//     On u bits 6, 7 (the wave index in layout A) a wave then stores amplitudes another wave loaded:
+//     a barrier after every wave's loads of the tile orders that (the LDS variant's first barrier,
+//     or one per tile in the variant without LDS). A pass of X factors needs no LDS transposes;
 //   real: two multiplies per output component;  complex: the full complex 2x2.
 struct ProductArgs {
     uint64_t ntiles;
@@ -52,6 +53,7 @@ struct ProductArgs {
     uint32_t fmask;                    // u bits whose factor needs arithmetic (all but X)
     uint32_t rmask;                    // ... of which real matrices
     uint32_t xmask;                    // u bits whose factor is X (flipped at the store)
+    uint32_t xsync;                    // 1: a barrier per tile before its stores (X on u bit 6 or 7, no LDS)
     cx<double> g[12][4];               // the factor on u bit i: row-major 2x2
 };
 
@@ -214,6 +216,7 @@ __device__ __forceinline__ void product_loop(cx<R> *amps, const ProductArgs &a, 
             x[i].re = (double)v[i].re;
             x[i].im = (double)v[i].im;
         }
+        if (!kLds && a.xsync) __syncthreads();   // every wave holds the tile before any wave stores
         load_raw(v, amps, tile_base(min(tile + 1, last), a) + off_t, a);
         product_tile<R, kLds>(x, amps, tile_base(tile, a) + off_s, t, a, use_b, use_c, tid);
         if (++tile >= end) return;
@@ -258,8 +261,8 @@ int product_pass(qipgpu_state *s, int32_t k, const int32_t *targets, const doubl
     // u bits 0..3 are state bits 0..3, which every pass keeps. The targets above them go on u bits
     // 4..11 by kind: a factor that needs arithmetic takes the register bits 8..11 of layout A first
     // (applied without a transpose), then 6, 7, 4, 5; an X takes the lane bits 4, 5 first, then
-    // 8..11, where it is flipped at the store, and 6, 7 last (where it is arithmetic). The free u bits
-    // take the lowest free state bits.
+    // 8..11, and 6, 7 last (flipped at the store everywhere; on 6, 7 at the cost of a barrier). The
+    // free u bits take the lowest free state bits.
     int ubit[kTileBits];
     std::fill(ubit, ubit + kTileBits, -1);
     for (int b = 0; b < kLowBits; ++b) ubit[b] = b;
@@ -299,10 +302,7 @@ int product_pass(qipgpu_state *s, int32_t k, const int32_t *targets, const doubl
         const int ub = place[j];
         const double *m = mats + 8 * j;
         const bool real = m[1] == 0.0 && m[3] == 0.0 && m[5] == 0.0 && m[7] == 0.0;
-        // An X is flipped at the store only on the bits a wave both loads and stores (u bits 0..5,
-        // its lanes, and 8..11, its registers): a flipped bit 6 or 7 (the wave index in layout A)
-        // would store into another wave's amplitudes before that wave has loaded them
-        if (is_x(j) && ub != 6 && ub != 7) {
+        if (is_x(j)) {
             a.xmask |= 1u << ub;
             continue;
         }
@@ -314,6 +314,7 @@ int product_pass(qipgpu_state *s, int32_t k, const int32_t *targets, const doubl
         }
     }
     const bool lds = (a.fmask & 0xffu) != 0;   // a factor below u bit 8 needs a transpose
+    a.xsync = !lds && (a.xmask & 0xc0u) != 0;
     if (s->dtype == QIPGPU_C128) {
         if (lds) launch_product<double, true>((cx<double> *)s->amps, a, s->stream);
         else launch_product<double, false>((cx<double> *)s->amps, a, s->stream);
