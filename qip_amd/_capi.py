@@ -4,6 +4,7 @@ The library is built in-tree by ``__graft_entry__.build()`` into ``qip_amd/lib/`
 CPU fallback: if the library or a GPU is missing, every backend call raises.
 """
 import ctypes
+import struct
 import os
 
 import numpy
@@ -140,6 +141,27 @@ def device_count() -> int:
 
 def i32(seq) -> numpy.ndarray:
     return numpy.ascontiguousarray(numpy.asarray(seq, dtype=numpy.int64).astype(numpy.int32))
+
+
+def i32_bytes(seq) -> bytes:
+    """A short int sequence packed as int32 for a `void*` argument (ctypes passes a bytes object's
+    own buffer, read-only, for the duration of the call)."""
+    return struct.pack("%di" % len(seq), *seq)
+
+
+# matrices up to this size are handed over as a bytes copy, which ctypes passes faster than a
+# numpy array's address; larger ones by address
+SMALL_MATRIX_BYTES = 1 << 16
+
+
+def c128_arg(m: numpy.ndarray):
+    """A complex128 matrix for a `const double*` argument: a bytes copy when small, else the address
+    of a C-contiguous complex128 array. The caller keeps the returned object until the call returns."""
+    if m.dtype != numpy.complex128 or not m.flags.c_contiguous:
+        m = numpy.ascontiguousarray(m, dtype=numpy.complex128)
+    if m.nbytes <= SMALL_MATRIX_BYTES:
+        return m.tobytes(), m
+    return m.ctypes.data, m
 
 
 def ptr(a: numpy.ndarray) -> int:

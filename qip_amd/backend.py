@@ -369,21 +369,21 @@ class GpuBackend(StateType):
                 self.apply_matrix(call.targets, call.mat)
             return
         for chunk in product_passes(self.n, targets):
-            t = _capi.i32([targets[j] for j in chunk])
-            m = numpy.ascontiguousarray(numpy.stack([numpy.asarray(mats[j], dtype=numpy.complex128).reshape(2, 2)
-                                                     for j in chunk]))
-            _capi.check(self._lib.qipgpu_apply_product(self._h, len(t), _capi.ptr(t), _capi.ptr(m.view(numpy.float64))),
-                        "apply_product")
+            m = b"".join(numpy.asarray(mats[j], dtype=numpy.complex128).reshape(2, 2).tobytes() for j in chunk)
+            rc = self._lib.qipgpu_apply_product(self._h, len(chunk), _capi.i32_bytes([targets[j] for j in chunk]), m)
+            if rc:
+                _capi.check(rc, "apply_product")
 
     def apply_matrix(self, targets: Sequence[int], mat: numpy.ndarray, controls: Sequence[int] = ()) -> None:
         """One engine call: `mat` on `targets` (first = MSB) where all `controls` are 1."""
-        t = _capi.i32(targets)
-        c = _capi.i32(controls)
-        m = numpy.ascontiguousarray(mat, dtype=numpy.complex128)
-        if m.shape != (2 ** len(t), 2 ** len(t)):
-            raise ValueError("matrix shape {} does not match {} targets".format(m.shape, len(t)))
-        _capi.check(self._lib.qipgpu_apply_matrix(self._h, len(t), _capi.ptr(t), _capi.ptr(m.view(numpy.float64)),
-                                                  len(c), _capi.ptr(c) if len(c) else None), "apply_matrix")
+        k = len(targets)
+        if mat.shape != (2 ** k, 2 ** k):
+            raise ValueError("matrix shape {} does not match {} targets".format(mat.shape, k))
+        m, keep = _capi.c128_arg(mat)
+        rc = self._lib.qipgpu_apply_matrix(self._h, k, _capi.i32_bytes(targets), m, len(controls),
+                                           _capi.i32_bytes(controls) if len(controls) else None)
+        if rc:
+            _capi.check(rc, "apply_matrix")
 
     def apply_fused(self, targets: Sequence[int], mat: numpy.ndarray) -> None:
         """Dense operator on <= 6 qubits through the split-BF16 MFMA kernel (complex64 states,

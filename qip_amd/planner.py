@@ -264,9 +264,12 @@ def plan(mats: Mapping) -> list:
     items = []
     for key, m in newmats.items():
         if type(m) == numpy.ndarray:
-            m = m.astype(numpy.complex128)      # qip/util.py:63
+            m = numpy.asarray(m, dtype=numpy.complex128)      # qip/util.py:63 (read-only here)
         _kind(m)
         items.append((key, m))
+    if len(items) == 1 and items[0][0]:
+        # one group is its own component, and its calls have nothing to merge with
+        return _plan_component(items, [])
     calls: list = []
     for comp in _components(items):
         calls.extend(_plan_component(comp, []))
