@@ -27,8 +27,9 @@ CPU baseline (--config-cpu-seconds of work) and its error figures: qft16 (16-qub
 circuit28 (28-qubit depth-20 circuit, plain complex128 vs unfused complex64), grover24 (24-qubit
 Grover search in both forms), batch512 (512 uneven circuits in the batching form), measure28
 (marginal probabilities, norm and measurement on a 28-qubit complex128 state), func28 (func_apply
-on a 28-qubit complex128 state) and dense28 (dense operators on 4..8 targets). --workload NAME runs one of them alone as its own JSON line
-(circuit30: the headline alone).
+on a 28-qubit complex128 state) and dense28 (dense operators on 4..8 targets). --workload NAME runs
+one of them alone as its own JSON line (circuit30: the headline alone); --workload product24 times
+the product sweep's pass shapes alone (not part of the default run).
 
 --no-cpu-baseline leaves out the CPU leg (for profiler runs); it does not change the timed work.
 """
@@ -717,6 +718,61 @@ def bench_dense28(args):
     return out
 
 
+PRODUCT_CASES = {
+    # name: (low state bits with a factor, number of high targets from the top down, factor)
+    "h4": ((), 4, "H"),            # tile bits 8..11 only: the variant without LDS
+    "h8": ((), 8, "H"),            # layouts A and B
+    "h3l8": ((1, 2, 3), 8, "H"),   # layouts A, B and C (the first pass of a 23-qubit H layer)
+    "u4l8": ((0, 1, 2, 3), 8, "U"),  # complex factors on every layout
+    "x3l8": ((1, 2, 3), 8, "X"),   # X only: flipped store addresses, no LDS
+}
+
+
+def bench_product24(args):
+    """The product sweep (k_product) alone: one pass of single-qubit factors per case (PRODUCT_CASES)
+    on 24- and 28-qubit states in both precisions, HIP events around repeated calls on the state's
+    stream, algorithmic bytes 2 S 2^n per pass. --case NAME runs one case at 24 qubits in complex64
+    only (for counter runs). Not part of the default run."""
+    import numpy
+    from qip_amd import GpuBackend
+    ev = HipEvents()
+    mats = {"H": numpy.array([[1, 1], [1, -1]]) / numpy.sqrt(2), "X": numpy.array([[0.0, 1.0], [1.0, 0.0]]),
+            "U": numpy.array([[0.6, 0.8j], [0.8j, 0.6]])}
+    reps = max(10, 4 * args.steps)
+    names = [args.case] if args.case else list(PRODUCT_CASES)
+    sizes = (24,) if args.case else (24, 28)
+    dtypes = (("c64", numpy.complex64, 8),) if args.case else (("c64", numpy.complex64, 8),
+                                                                ("c128", numpy.complex128, 16))
+    cases = {}
+    for n in sizes:
+        for label, st, amp in dtypes:
+            be = GpuBackend.make_state(n, [], [], statetype=st)
+            for name in names:
+                low, nhigh, f = PRODUCT_CASES[name]
+                bits = list(low) + list(range(n - nhigh, n))
+                targets, ms_ = [n - 1 - b for b in bits], [mats[f]] * len(bits)
+                be.apply_product(targets, ms_)
+                be.synchronize()
+                a, b = ev.create(), ev.create()
+                ev.record(a, be.stream)
+                for _ in range(reps):
+                    be.apply_product(targets, ms_)
+                ev.record(b, be.stream)
+                ms = ev.elapsed_ms(a, b) / reps
+                gbs = 2 * amp * 2 ** n / ms / 1e6
+                cases["{}_{}_n{}".format(name, label, n)] = {"ms": ms, "algorithmic_gbs": gbs,
+                                                             "hbm_frac": gbs / HBM_PEAK_GBS}
+            be.close()
+    c64 = [c for k, c in cases.items() if "_c64_n24" in k]
+    value = sum(c["algorithmic_gbs"] for c in c64) / len(c64)
+    cfg = {"workload": "product sweep passes on 24- and 28-qubit states (value: mean over the complex64 "
+                       "24-qubit cases)", "cases": {k: list(v[:2]) + [v[2]] for k, v in PRODUCT_CASES.items()}}
+    return _line("24-qubit c64 product sweep: algorithmic GB/s", value, "GB/s", reps, 1,
+                 sum(c["ms"] for c in c64) / len(c64), "c64", cfg, cases=cases,
+                 roofline={"bound": "hbm", "kernel": "k_product<float, lds>", "achieved": value,
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": value / HBM_PEAK_GBS, "traffic": None})
+
+
 def bench_batch512(args):
     """512 independent layered random circuits, n ~ U{10..22}, depth ~ U{8..32} (seeded), complex64:
     the batching form (one captured graph per run, autotuned) against the plain complex64 path with
@@ -984,16 +1040,18 @@ def main():
                          "traffic for calibrating hardware byte counters)")
     ap.add_argument("--workload", default="all",
                     choices=["all", "circuit30", "circuit28", "grover24", "qft16", "batch512", "measure28", "func28",
-                             "dense28"],
+                             "dense28", "product24"],
                     help="all (default): the headline, then every other configuration nested under 'configs' in the "
                          "same JSON line; or one workload alone, as its own JSON line")
+    ap.add_argument("--case", choices=sorted(PRODUCT_CASES), default=None,
+                    help="product24 only: one case, complex64 at 24 qubits (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the engine holds the whole state on a single MI355X; "
               "a multi-GPU run is not part of this build (--gpus {} requested)".format(args.gpus), file=sys.stderr)
         sys.exit(2)
     log = lambda msg: print("[bench] " + msg, file=sys.stderr, flush=True)  # noqa: E731
-    funcs = dict(SECONDARY, circuit30="bench_circuit30")
+    funcs = dict(SECONDARY, circuit30="bench_circuit30", product24="bench_product24")
     if args.workload != "all":
         result = globals()[funcs[args.workload]](args)
     else:
