@@ -283,6 +283,11 @@ int product_pass(qipgpu_state *s, int32_t k, const int32_t *targets, const doubl
         if (place[j] < 0 && is_x(j)) take(j, {4, 5, 8, 9, 10, 11});
     for (int j = 0; j < k; ++j)
         if (place[j] < 0) take(j, {6, 7, 4, 5});
+    // at most kHighPerPass = 8 targets above the low bits share the 8 u bits 4..11: after the
+    // rounds above every one has a place (arithmetic overflow finds 8..11 full and 4..7 free enough,
+    // X overflow finds only 6, 7 free)
+    for (int j = 0; j < k; ++j)
+        if (place[j] < 0) return set_error(QIPGPU_EINVAL, "product pass: target %d has no tile bit", targets[j]);
     for (int b = kLowBits, u = kLowBits; u < kTileBits; ++u) {
         if (ubit[u] >= 0) continue;
         while (tmask >> b & 1) ++b;
