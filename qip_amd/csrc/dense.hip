@@ -7,11 +7,14 @@
 // sum_j U[i][j] * in[base + off[j]]. That is the matrix product Y = U X with X[j][b] the amplitudes
 // of base b. FP64 arithmetic for both state precisions, rounded once on store.
 //
-//   k_dense_tile<R, K>   K = 4..6, in place. A persistent workgroup keeps U in LDS; per tile of B
+//   k_dense_tile<R, K>   K = 4, in place. A persistent workgroup keeps U in LDS; per tile of B
 //                        bases it stages X (D x B = 4096 amplitudes, 64 KB in FP64) in LDS, each
 //                        thread forms 4 rows x 4 bases of Y in registers, and the next tile's
 //                        amplitudes are loaded into registers while this one is multiplied. A tile
 //                        holds every amplitude of its bases before any is written, so in place is safe.
+//   k_dense_mfma<R, K>   K = 5, 6, in place, the same tiles on FP64 MFMA (v_mfma_f64_16x16x4f64): each
+//                        wave keeps its share of the real form of U in registers for the launch, X
+//                        and Y pass through LDS as Re / Im planes.
 //   k_dense_gemm<R>      K = 7..12, out of place: a 64 x 64 output tile per workgroup over 16-column
 //                        steps of j through LDS, into a compact [D][nbase] buffer; k_dense_scatter
 //                        writes it back to the bases' positions.
@@ -158,6 +161,170 @@ __global__ __launch_bounds__(kTileThreads) void k_dense_tile(cx<R> *amps, uint64
                 for (int r = 0; r < S::RB; ++r) amps[base + sOff[rg + S::RG * r]] = from_d<R>(acc[r][q]);
             }
         }
+    }
+}
+
+// ---- K = 5, 6: the same tiles on FP64 MFMA --------------------------------------------------------
+//
+// Y = U X as a real product: with real row 2i + a (a = 0 re, 1 im) of Y and real row 2j + a' of X,
+// the real matrix is M[2i + a][2j + a'] = Re U[i][j] if a = a', -Im U[i][j] for (0, 1), Im U[i][j] for
+// (1, 0) (2D x 2D). v_mfma_f64_16x16x4f64 computes D[16 x 16] += A[16 x 4] B[4 x 16]; lane l holds
+// A[l % 16][l / 16] and B[l / 16][l % 16], and D[4 i + l / 16][l % 16] for its items i = 0..3. So the
+// 16 real rows rho of a block are ordered rho = 4 i + g (g = rho % 4): items i = 0, 1 are Re, Im of
+// complex row 2 g and items 2, 3 those of row 2 g + 1, and every lane ends with whole complex outputs.
+// The 4 real columns of k-step s are Re, Im of complex rows 2 s and 2 s + 1 of X.
+//
+// Wave w owns real row blocks w * RBW .. (w + 1) * RBW - 1 (RBW = D / 32) for every column block of
+// the tile; its A fragments (RBW x KS doubles per lane, KS = D / 2 k-steps) are gathered from U once
+// per launch and stay in registers. X goes through LDS as two FP64 planes (Re, Im) with one slot of
+// padding per row (or per base, when lanes run along the rows), and so does Y on its way out, so
+// loads and stores keep the tile kernel's coalesced mapping. Per tile: 2 D x 2 D x B FMAs in D / 2 x (B / 16) x RBW MFMAs per wave.
+template <int K>
+struct MfmaShape {
+    static constexpr int D = 1 << K;
+    static constexpr int B = kTileElems / D;     // bases per tile
+    static constexpr int RBW = D / 32;           // 16-row real blocks per wave (4 waves)
+    static constexpr int CB = B / 16;            // 16-column blocks
+    static constexpr int KS = D / 2;             // 4-column k-steps over the 2 D real columns
+    static constexpr int XS = B + 1;             // padded plane row, row-major planes (doubles)
+    static constexpr int XSJ = D + 1;            // padded plane column, base-major planes
+    static constexpr int PLANE = D * B + (D > B ? D : B);
+    static_assert(K == 5 || K == 6, "MFMA tiles for 5 and 6 targets");
+};
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename R, int K, bool ROWS_FAST>
+__global__ __launch_bounds__(kTileThreads, K == 5 ? 2 : 1) void k_dense_mfma(cx<R> *amps, uint64_t nbase, uint64_t ctrl_mask,
+                                                                PosList pl, const uint64_t *__restrict__ off_g,
+                                                                const cx<double> *__restrict__ U_g, uint64_t ntiles) {
+    using S = TileShape<K>;
+    using M = MfmaShape<K>;
+    constexpr int D = M::D, B = M::B, XS = M::XS;
+    static_assert(S::B == M::B, "tile shape");
+    // Re and Im planes of the tile: row-major (row * XS + base) when lanes run along the bases,
+    // base-major (base * XSJ + row) when they run along the rows, so that consecutive lanes touch
+    // consecutive slots in the loads and stores and the MFMA operand reads spread over the banks
+    constexpr int XSJ = M::XSJ;
+    __shared__ double sX[2][M::PLANE];
+    auto idx = [](int j, int b) { return ROWS_FAST ? b * XSJ + j : j * XS + b; };
+    __shared__ uint64_t sOff[D];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int e = tid; e < D; e += kTileThreads) sOff[e] = off_g[e];
+
+    // this wave's A fragments: lane -> real row rho = lane % 16 of each block, k = lane / 16 of each step
+    const int rho = lane & 15, kap = lane >> 4;
+    const int a_out = (rho >> 2) & 1, m_out = rho >> 3, g_out = rho & 3;
+    const int a_in = kap & 1;
+    double afr[M::RBW][M::KS];
+#pragma unroll
+    for (int r = 0; r < M::RBW; ++r) {
+        const int i = 8 * (wave * M::RBW + r) + 2 * g_out + m_out;       // complex output row
+#pragma unroll
+        for (int s = 0; s < M::KS; ++s) {
+            const cx<double> u = U_g[i * D + 2 * s + (kap >> 1)];
+            afr[r][s] = a_out == a_in ? u.re : (a_out ? u.im : -u.im);
+        }
+    }
+    __syncthreads();
+
+    // loads and stores: element i of this thread is (row j, base b), as in k_dense_tile
+    constexpr int BSTEP = kTileThreads / D;
+    constexpr int JSTEP = kTileThreads / B;
+    const int b_ld = ROWS_FAST ? tid / D : tid % B;
+    const int j_ld = ROWS_FAST ? tid % D : tid / B;
+    cx<R> pre[S::LOADS];
+    // position of base t B + v (v < B; a base past the end reads the last one, whose products are
+    // not stored)
+    auto base_at = [&](uint64_t t, int v) {
+        const uint64_t g = t * B + v;
+        return base_of(g < nbase ? g : nbase - 1, pl, ctrl_mask);
+    };
+    auto load_tile = [&](uint64_t t) {
+        if constexpr (ROWS_FAST) {
+#pragma unroll
+            for (int i = 0; i < S::LOADS; ++i) pre[i] = amps[base_at(t, b_ld + BSTEP * i) + sOff[j_ld]];
+        } else {
+            const uint64_t base = base_at(t, b_ld);
+#pragma unroll
+            for (int i = 0; i < S::LOADS; ++i) pre[i] = amps[base + sOff[j_ld + i * JSTEP]];
+        }
+    };
+    // plane slot of element i of this thread
+    auto slot = [&](int i) { return ROWS_FAST ? idx(j_ld, b_ld + BSTEP * i) : idx(j_ld + i * JSTEP, b_ld); };
+
+    // the tile's amplitudes (pre) into LDS as Re and Im planes
+    auto write_x = [&]() {
+#pragma unroll
+        for (int i = 0; i < S::LOADS; ++i) {
+            const cx<double> v = to_d(pre[i]);
+            sX[0][slot(i)] = v.re;
+            sX[1][slot(i)] = v.im;
+        }
+    };
+    f64x4 acc[M::RBW][M::CB];
+    // Y = M X for the tile in LDS: lanes read the B operand from the Re or Im plane
+    auto products = [&]() {
+#pragma unroll
+        for (int r = 0; r < M::RBW; ++r)
+#pragma unroll
+            for (int c = 0; c < M::CB; ++c) acc[r][c] = f64x4{0.0, 0.0, 0.0, 0.0};
+        const double *bplane = sX[kap & 1];
+#pragma unroll
+        for (int s = 0; s < M::KS; ++s) {
+            const double *row = bplane + idx(2 * s + (kap >> 1), rho);
+            double bfr[M::CB];
+#pragma unroll
+            for (int c = 0; c < M::CB; ++c) bfr[c] = row[idx(0, 16 * c)];
+#pragma unroll
+            for (int r = 0; r < M::RBW; ++r)
+#pragma unroll
+                for (int c = 0; c < M::CB; ++c)
+                    acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(afr[r][s], bfr[c], acc[r][c], 0, 0, 0);
+        }
+    };
+    // item q of a block holds real row 4 q + g: Re (q even) or Im (q odd) of complex row 2 g + q / 2
+    auto write_y = [&]() {
+#pragma unroll
+        for (int r = 0; r < M::RBW; ++r)
+#pragma unroll
+            for (int c = 0; c < M::CB; ++c)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int i = 8 * (wave * M::RBW + r) + 2 * kap + (q >> 1);
+                    sX[q & 1][idx(i, 16 * c + rho)] = acc[r][c][q];
+                }
+    };
+    // the tile's outputs from LDS to the state, with the load mapping
+    auto store_y = [&](uint64_t t) {
+#pragma unroll
+        for (int i = 0; i < S::LOADS; ++i) {
+            const int v = ROWS_FAST ? b_ld + BSTEP * i : b_ld;
+            if (t * B + v < nbase) {
+                cx<double> y;
+                y.re = sX[0][slot(i)];
+                y.im = sX[1][slot(i)];
+                const int j = ROWS_FAST ? j_ld : j_ld + i * JSTEP;
+                amps[base_at(t, v) + sOff[j]] = from_d<R>(y);
+            }
+        }
+    };
+
+    const uint64_t G = gridDim.x;
+    uint64_t t = blockIdx.x;
+    if (t >= ntiles) return;                   // the whole workgroup: no barrier is skipped by part of it
+    load_tile(t);
+    for (; t < ntiles; t += G) {
+        __syncthreads();                       // the previous tile's outputs have been read back
+        write_x();
+        __syncthreads();
+        if (t + G < ntiles) load_tile(t + G);  // in flight during the products
+        products();
+        __syncthreads();                       // every wave has read X
+        write_y();
+        __syncthreads();
+        store_y(t);
     }
 }
 
@@ -393,16 +560,28 @@ int launch_tile(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_ma
     if (rc) return rc;
     const uint64_t nbase = s->len >> allpos.size();
     const uint64_t ntiles = (nbase + TileShape<K>::B - 1) / TileShape<K>::B;
-    const int per_cu = K == 6 ? 1 : 2;       // LDS: up to 134 KB at K = 6, 82 KB and 69 KB below
-    const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)device_cus() * per_cu);
     const bool rows_fast = std::find(tbits.begin(), tbits.end(), 0) != tbits.end();
     const cx<double> *dU = (const cx<double> *)((char *)d + sizeof(uint64_t) * D);
-    if (rows_fast)
-        hipLaunchKernelGGL((k_dense_tile<R, K, true>), dim3((uint32_t)grid), dim3(kTileThreads), 0, s->stream,
-                           (cx<R> *)s->amps, nbase, ctrl_mask, poslist(allpos), (const uint64_t *)d, dU, ntiles);
-    else
-        hipLaunchKernelGGL((k_dense_tile<R, K, false>), dim3((uint32_t)grid), dim3(kTileThreads), 0, s->stream,
-                           (cx<R> *)s->amps, nbase, ctrl_mask, poslist(allpos), (const uint64_t *)d, dU, ntiles);
+    if constexpr (K >= 5) {
+        // FP64 MFMA: the waves hold their operator fragments in registers, at K = 6 one wave per SIMD
+        // (one workgroup per CU), at K = 5 two
+        const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)device_cus() * (K == 5 ? 2 : 1));
+        if (rows_fast)
+            hipLaunchKernelGGL((k_dense_mfma<R, K, true>), dim3((uint32_t)grid), dim3(kTileThreads), 0, s->stream,
+                               (cx<R> *)s->amps, nbase, ctrl_mask, poslist(allpos), (const uint64_t *)d, dU, ntiles);
+        else
+            hipLaunchKernelGGL((k_dense_mfma<R, K, false>), dim3((uint32_t)grid), dim3(kTileThreads), 0, s->stream,
+                               (cx<R> *)s->amps, nbase, ctrl_mask, poslist(allpos), (const uint64_t *)d, dU, ntiles);
+    } else {
+        // vector FP64 with U in LDS: 69 KB, two workgroups per CU
+        const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)device_cus() * 2);
+        if (rows_fast)
+            hipLaunchKernelGGL((k_dense_tile<R, K, true>), dim3((uint32_t)grid), dim3(kTileThreads), 0, s->stream,
+                               (cx<R> *)s->amps, nbase, ctrl_mask, poslist(allpos), (const uint64_t *)d, dU, ntiles);
+        else
+            hipLaunchKernelGGL((k_dense_tile<R, K, false>), dim3((uint32_t)grid), dim3(kTileThreads), 0, s->stream,
+                               (cx<R> *)s->amps, nbase, ctrl_mask, poslist(allpos), (const uint64_t *)d, dU, ntiles);
+    }
     QIP_LAUNCHED();
     return QIPGPU_OK;
 }

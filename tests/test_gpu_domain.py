@@ -105,6 +105,28 @@ def test_dense_tiles(k, n, controls, dtype):
     check({tuple(qs): m}, n, dtype)
 
 
+@pytest.mark.parametrize("k", [5, 6])
+@pytest.mark.parametrize("where", ["low", "high", "low_controlled"])
+@pytest.mark.parametrize("dtype", [numpy.complex128, numpy.complex64])
+def test_dense_tiles_both_layouts(k, where, dtype):
+    """5 and 6 targets (the FP64 MFMA tiles) with state bit 0 among the targets (lanes along the
+    operator's rows) and without it (lanes along the bases), over enough tiles that every workgroup
+    takes several; once with a control on the highest qubit."""
+    n = 15
+    if where == "high":
+        qs = list(range(k))                                   # qubits 0.. = the highest state bits
+    else:
+        qs = list(range(n - k, n))[::-1]                      # state bits 0..k-1, in scrambled order
+        qs = qs[1::2] + qs[0::2]
+    if where == "low_controlled":
+        qs = [0] + qs
+    print("dense tile layout", where, "k", k, "qubits", qs, flush=True)
+    m = rand_u(2 ** k)
+    if where == "low_controlled":
+        m = CMat(m)
+    check({tuple(qs): m}, n, dtype)
+
+
 @pytest.mark.parametrize("k,n,controls", [(7, 7, 0), (7, 12, 1), (8, 12, 0), (8, 14, 2), (10, 13, 0), (12, 14, 0)])
 @pytest.mark.parametrize("dtype", [numpy.complex128, numpy.complex64])
 def test_dense_gemm(k, n, controls, dtype):
