@@ -636,7 +636,7 @@ def bench_func28(args):
 
 
 def bench_dense28(args):
-    """Dense operators on 4..6 targets (k_dense_tile) on a 28-qubit complex128 state (4 GiB): the
+    """Dense operators on 4..6 targets (k_dense_tile, k_dense_mfma) on a 28-qubit complex128 state (4 GiB): the
     targets on the lowest, the highest and scattered state bits, and one 6-target operator under a
     control; then 8 targets (the FP64 GEMM, k_dense_gemm + scatter) on 26 qubits. Per call: HIP-event
     time on the state's stream, algorithmic bytes 2 S 2^(n-c), FP64 flops 8 2^K 2^(n-c)."""
@@ -694,7 +694,7 @@ def bench_dense28(args):
     k6_gbs = sum(c["algorithmic_gbs"] for c in k6) / 3
     out = _line("28-qubit c128 dense 4-6 target operators: algorithmic GB/s", value, "GB/s", reps, 1,
                 sum(c["ms"] for c in tiles) / len(tiles), "c128", cfg, cases=cases,
-                roofline={"bound": "hbm", "kernel": "k_dense_tile<double,K> (K = 4..6, LDS tiles)", "achieved": value,
+                roofline={"bound": "hbm", "kernel": "k_dense_tile<double,4> (vector FP64, U in LDS) and k_dense_mfma<double,K> (K = 5, 6: FP64 MFMA)", "achieved": value,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": value / HBM_PEAK_GBS, "traffic": None,
                           "k6_fp64": {"achieved_gbs": k6_gbs,
                                       "fp64_tflops": sum(c["fp64_tflops"] for c in k6) / 3,
