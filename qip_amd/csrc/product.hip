@@ -75,8 +75,6 @@ struct TileStore {
 template <typename R, bool kLds>
 constexpr int blocks_per_cu() { return (kLds || sizeof(R) == 8) ? 2 : 3; }
 
-__device__ __forceinline__ int ppos(uint32_t u) { return (int)(u + (u >> 4)); }
-
 __device__ __forceinline__ uint64_t tile_base(uint64_t t, const ProductArgs &a) {
 #pragma unroll
     for (int i = 0; i < kTileBits; ++i) {
@@ -133,10 +131,6 @@ __device__ __forceinline__ void apply4(cx<double> (&x)[16], const ProductArgs &a
     }
 }
 
-// Layouts B and C: the u of register e of thread tid.
-__device__ __forceinline__ uint32_t u_b(uint32_t tid, int e) { return (tid & 15u) | ((uint32_t)e << 4) | ((tid >> 4) << 8); }
-__device__ __forceinline__ uint32_t u_c(uint32_t tid, int e) { return (uint32_t)e | (tid << 4); }
-
 template <typename R>
 __device__ __forceinline__ void load_raw(cx<R> (&v)[16], const cx<R> *amps, uint64_t base, const ProductArgs &a) {
 #pragma unroll
@@ -151,8 +145,10 @@ __device__ __forceinline__ void product_tile(cx<double> (&x)[16], cx<R> *amps, u
     apply4<8>(x, a);
     if (kLds && (use_b || use_c)) {
         TileStore &t = *lds;
-        // Padded LDS slots of the three layouts, a per-thread base plus a constant per register:
-        // A: ppos(tid + 256 i) = base + 272 i;  B: ppos(u_b(tid, e)) = base + 17 e;  C: ppos(u_c(tid, e)) = base + e
+        // Tile position u sits in LDS slot u + u / 16 (rows of 16 padded by one). Register e of thread
+        // tid holds u = tid + 256 e in layout A, (tid & 15) | e << 4 | (tid >> 4) << 8 in B and
+        // e | tid << 4 in C, so each layout's slots are a per-thread base plus a constant per register:
+        // A: tid + tid / 16 + 272 e;  B: (tid & 15) + 272 (tid >> 4) + 17 e;  C: 17 tid + e
         const auto ta = t.at((int)(tid + (tid >> 4)));
         const auto tb = t.at((int)((tid & 15u) + 272u * (tid >> 4)));
         const auto tc = t.at((int)(17u * tid));
