@@ -20,10 +20,10 @@
 // 16-amplitude-stride reads of layout C spread over the banks.
 //
 // A pass whose factors all sit on u bits 8..11 needs no LDS and runs a variant without the tile,
-// three workgroups per CU for a complex64 state. Such passes are latency-bound rather than
-// bandwidth-bound: the 68 KB FP64 tile and 176 VGPRs hold the LDS variant to 8 waves per CU, which
-// do not cover the barrier-separated phases (on a 24-qubit complex64 pass over all three layouts,
-// LDS is busy about a fifth of the time and the FP64 units about a third).
+// three workgroups per CU for a complex64 state. Passes that need transposes are latency-bound
+// rather than bandwidth-bound: the 68 KB FP64 tile and 176 VGPRs hold the LDS variant to 8 waves
+// per CU, which do not cover the barrier-separated phases (on a 24-qubit complex64 pass over all
+// three layouts, LDS is busy about a fifth of the time and the FP64 units about a third).
 #include <algorithm>
 #include <cstring>
 #include <vector>
