@@ -12,6 +12,8 @@ renormalisation, synchronise. `value` = 1780 * steps / timed seconds.
 Beside it, in the same run:
   * the plain complex128 GpuBackend (the drop-in) on the same circuit, once, with its gates/s,
     algorithmic GB/s and the HBM roofline of its dominant kernel (single-qubit dense sweep);
+  * the complex128 fused form (Fused128Backend: the same blocks applied in FP64) on the same circuit,
+    once, with its gates/s and its difference from the plain complex128 result;
   * 1 - F and max |amplitude difference| of the fused result against the complex128 result;
   * the HBM roofline of the fused kernel from HIP events on the state's stream, over one more run
     of the same circuit's blocks, composed up front and launched back to back;
@@ -432,6 +434,15 @@ def bench_qft16(args):
     walk_ms = (time.perf_counter() - t) * 1e3
     expect = numpy.fft.ifft(psi, norm="ortho")
     err = float(numpy.max(numpy.abs(state - expect)))
+    # the same runs on the complex128 fused form (blocks of up to 6 qubits applied in FP64)
+    from qip_amd.fused import Fused128Backend
+    run(sm, feed={q: psi}, backend_constructor=Fused128Backend.make_state)
+    t = time.perf_counter()
+    for _ in range(reps):
+        f_state, _ = run(sm, feed={q: psi}, backend_constructor=Fused128Backend.make_state)
+    f_dt = (time.perf_counter() - t) / reps
+    fused128 = {"circuits_per_s": 1.0 / f_dt, "ms_per_run": f_dt * 1e3,
+                "max_abs_err_vs_ifft": float(numpy.max(numpy.abs(f_state - expect)))}
     # the circuit's most frequent kernel: 120 controlled phases (k_diag on the quarter of the state with
     # both qubits 1), launched back to back between HIP events on a state of the same size
     be = GpuBackend.make_state(n, [list(range(n))], [psi])
@@ -459,7 +470,7 @@ def bench_qft16(args):
     diag_gbs = diag_bytes / (diag_ms * 1e-3) / 1e9
     out = _line("16-qubit QFT circuits/s (graph run incl. state upload and read-back)", 1.0 / dt, "circuits/s", reps,
                 1, dt * 1e3, "c128", cfg, max_abs_err_vs_ifft=err, max_abs_err_probabilities=perr,
-                ms_first_run_walking_graph=walk_ms,
+                ms_first_run_walking_graph=walk_ms, fused_c128=fused128,
                 roofline={"bound": "launch", "kernel": "k_diag<double,0> (controlled phase, 120 of the 144 gates)",
                           "achieved": diag_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": diag_gbs / HBM_PEAK_GBS,
                           "traffic": None, "bytes_per_launch": diag_bytes, "avg_launch_ms": diag_ms,
@@ -994,7 +1005,23 @@ def bench_circuit30(args):
         fz.flush()
         inf = infidelity(pl, fz)
         err = max_abs_diff(pl, fz)
-        dom = plain_dominant_roofline(pl, n, ev)
+        # the complex128 fused form: the same blocks applied in FP64 (dense sweeps, FP64 MFMA tiles)
+        from qip_amd.fused import Fused128Backend
+        f128 = Fused128Backend.make_state(n, [], [])
+        f128.synchronize()
+        start = time.perf_counter()
+        for d in ops:
+            f128.kronselect_dot(d)
+        f128.synchronize()
+        f128_s = time.perf_counter() - start
+        log("fused c128: {:.3f} s ({} blocks)".format(f128_s, f128.blocks_applied))
+        result["fused_c128"] = {
+            "gates_per_s": ngates / f128_s, "s_per_circuit": f128_s, "blocks": f128.blocks_applied,
+            "speedup_vs_plain_c128": plain_s / f128_s, "max_abs_err_vs_plain_c128": max_abs_diff(pl, f128),
+            "infidelity_vs_plain_c128": infidelity(pl, f128),
+            "note": "Fused128Backend: the precision form's blocks applied in FP64 on a complex128 state"}
+        f128.close()
+        dom = plain_dominant_roofline(pl, n, ev)     # applies further gates to pl: after every comparison
         result["plain_c128"] = {
             "gates_per_s": ngates / plain_s, "s_per_circuit": plain_s,
             "algorithmic_gbs": plain_bytes / plain_s / 1e9, "hbm_frac": plain_bytes / plain_s / 1e9 / HBM_PEAK_GBS,

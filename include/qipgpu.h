@@ -191,15 +191,17 @@ int qipgpu_fused_pattern_sweep(qipgpu_state *s, int32_t k, const int32_t *target
  * (kronprod.pyx:41-200) it replaces. */
 int qipgpu_compose(int32_t k, const int32_t *qubits, int32_t n_gates, const int32_t *gate_k,
                    const int32_t *gate_qubits, const double *mats, double *out);
-/* Record gates on a complex64 state for online fusion: the precision form's replacement of the
- * gate-by-gate kronselect_dot -> cdot_loop calls (kronprod.pyx:300-345, 41-200). Gate g has gate_k[g]
+/* Record gates for online fusion: the fused forms' replacement of the gate-by-gate
+ * kronselect_dot -> cdot_loop calls (kronprod.pyx:300-345, 41-200), on a complex64 state (the
+ * precision form) or a complex128 one (FP64 blocks). Gate g has gate_k[g]
  * (1..width) distinct qubits gate_qubits[6 g ..] (first = MSB of its matrix) and the next
  * 4^gate_k[g] interleaved complex entries of `mats`. Gates are planned into blocks of at most
  * `width` qubits by the light-cone rule (a block grows from the earliest gate whose predecessors are
  * all placed, absorbing every such gate within its qubits and taking the qubits of the one, preferably
  * sharing a qubit with it, that a two-step look-ahead scores best, while the width allows). A block
  * that no later gate can change is composed (qipgpu_compose) and launched when streaming
- * (qipgpu_fused_apply for n >= 12, the plain dense sweep below); the record is planned again every
+ * (qipgpu_fused_apply on a complex64 state with n >= 12; the FP64 dense sweep of qipgpu_apply_matrix
+ * on a complex128 state, and below 12 qubits); the record is planned again every
  * max(16, pending / 8) gates. For n >= 12 a run
  * of consecutive blocks holding single-qubit gates only is one product of per-qubit 2x2s on the
  * product sweep (qipgpu_apply_product, FP64 arithmetic); a trailing run waits until a final block

@@ -1,7 +1,10 @@
-// recorder.hip -- precision form, host side: gates recorded on a complex64 state are planned into
-// blocks of at most `width` qubits by the light-cone rule (plan.h), and each block is composed
-// (qipgpu_compose) and launched (qipgpu_fused_apply, or the plain dense sweep below 12 qubits) as soon
-// as it is final, i.e. no gate recorded later can change it.
+// recorder.hip -- fused forms, host side: gates recorded on a state are planned into blocks of at
+// most `width` qubits by the light-cone rule (plan.h), and each block is composed in FP64
+// (qipgpu_compose) and launched as soon as it is final, i.e. no gate recorded later can change it.
+// On a complex64 state from 12 qubits on a block goes to the split-BF16 MFMA kernel
+// (qipgpu_fused_apply, the precision form); on a complex128 state, or below 12 qubits, to the FP64
+// dense sweep (qipgpu_apply_matrix: registers, LDS tiles or FP64 MFMA tiles by width), so a
+// complex128 state keeps FP64 arithmetic with one dense sweep per block instead of one per gate.
 //
 // Streaming. The recorded gates not launched yet are planned again every max(16, pending / 8)
 // recorded gates (so planning costs a bounded number of steps per gate), and the final leading
@@ -200,7 +203,8 @@ int compose_product(const Recorder &r, const std::vector<PlannedBlock> &blocks, 
 int launch(qipgpu_state *s, const Launch &c) {
     const int k = (int)c.qubits.size();
     if (!c.product) {
-        if (s->n >= kFusedMinQubits) return qipgpu_fused_apply(s, k, c.qubits.data(), c.mat.data());
+        if (s->dtype == QIPGPU_C64 && s->n >= kFusedMinQubits)
+            return qipgpu_fused_apply(s, k, c.qubits.data(), c.mat.data());
         return qipgpu_apply_matrix(s, k, c.qubits.data(), c.mat.data(), 0, nullptr);
     }
     // one product call per pass: the factors on the kProductLowBits lowest state bits with up to
@@ -288,7 +292,6 @@ int launch_final(qipgpu_state *s) {
 }
 
 int ensure_recorder(qipgpu_state *s) {
-    if (s->dtype != QIPGPU_C64) return set_error(QIPGPU_EINVAL, "gates are recorded on complex64 states");
     if (!s->recorder) {
         s->recorder = new Recorder();
         s->recorder->reset(s->n, 6);

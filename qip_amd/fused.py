@@ -1,4 +1,6 @@
-"""Precision form: complex64 state, gates fused into <= 6-qubit dense blocks on MFMA.
+"""Fused forms: gates fused into <= 6-qubit dense blocks. The precision form (FusedBackend) keeps a
+complex64 state and applies the blocks on split-BF16 MFMA; Fused128Backend keeps a complex128 state and
+applies the same blocks in FP64 (dense sweeps, FP64 MFMA tiles at 5 and 6 qubits).
 
 ``FusedBackend`` is a StateType (qip/backend.py:14-65) like GpuBackend, but kronselect_dot only
 records: each engine call of the plan (targets, controls, matrix) becomes a dense gate on its
@@ -390,3 +392,19 @@ class FusedBackend(GpuBackend):
         before = self.blocks_applied
         _capi.check(self._lib.qipgpu_fused_flush(self._h), "flush")
         return self.blocks_applied - before
+
+
+class Fused128Backend(FusedBackend):
+    """complex128 state with the same recording and fusion as FusedBackend, each block applied in FP64
+    by the dense sweeps (registers up to 3 qubits, LDS tiles at 4, FP64 MFMA tiles at 5 and 6) instead
+    of the split-BF16 kernel: the plain path's arithmetic with one dense sweep per block instead of one
+    per gate (the results differ from the gate-by-gate ones only by FP64 rounding). Runs of
+    single-qubit-only blocks take the product sweep, as in the precision form. There is no
+    renormalisation: the blocks are applied in FP64 as composed."""
+
+    @classmethod
+    def make_state(cls, n, index_groups, feed_list, inputstartindex=None, inputendindex=None,
+                   outputstartindex=None, outputendindex=None, statetype=numpy.complex128) -> "Fused128Backend":
+        # complex128 whatever is asked: FusedBackend is the complex64 form
+        return super(FusedBackend, cls).make_state(n, index_groups, feed_list, inputstartindex, inputendindex,
+                                                   outputstartindex, outputendindex, statetype=numpy.complex128)
