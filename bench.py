@@ -281,8 +281,9 @@ def bench_circuit28(args):
 
 def bench_grover24(args):
     """24-qubit iterative Grover (23 search qubits + ancilla, 2274 iterations): plain complex128
-    with H/X/multi-controlled-Z diffusion, and the fused complex64 backend with F diffusion; then
-    a seeded measurement of the register. Amplitude error against the closed form."""
+    with H/X/multi-controlled-Z diffusion, the fused complex64 backend with F diffusion, and the
+    complex128 fused form with H/X/multi-controlled-Z diffusion; then a seeded measurement of the
+    register. Amplitude error against the closed form."""
     import random
     import numpy
     from qip_amd import GpuBackend
@@ -295,8 +296,10 @@ def bench_grover24(args):
     expect = grover_closed_form(ns, x_star, t_iter)
     groups, feeds = grover_initial(ns)
     res = {}
+    from qip_amd.fused import Fused128Backend
     for name, make, diff in (("plain_c128", GpuBackend.make_state, "mcz"),
-                             ("fused_c64", FusedBackend.make_state, "f")):
+                             ("fused_c64", FusedBackend.make_state, "f"),
+                             ("fused_c128", Fused128Backend.make_state, "mcz")):
         be = make(ns + 1, groups, feeds)
         ops = grover_iteration_ops(ns, x_star, diff)
         apply_ops(be, ops)                      # warm-up iteration (tables, kernels)
@@ -310,13 +313,18 @@ def bench_grover24(args):
         err = float(numpy.max(numpy.abs(be.get_state() - expect)))
         random.seed(args.seed)
         m, p = be.measure(list(range(ns)))
-        alg, touched = grover_iteration_bytes(ns, 16 if name == "plain_c128" else 8, diff)
+        amp = 8 if name == "fused_c64" else 16
+        alg, touched = grover_iteration_bytes(ns, amp, diff)
         res[name] = {"iterations_per_s": t_iter / dt, "s_total": dt, "diffusion": diff,
                      "max_abs_err_vs_closed_form": err, "measured": m, "measured_prob": p,
                      "found_marked": m == x_star,
                      "algorithmic_gbs": alg * t_iter / dt / 1e9, "hbm_frac": alg * t_iter / dt / 1e9 / HBM_PEAK_GBS,
-                     "touched_gbs": touched * t_iter / dt / 1e9,
-                     "product_pass": product_pass_roofline(be, ns, 16 if name == "plain_c128" else 8)}
+                     "touched_gbs": touched * t_iter / dt / 1e9}
+        if name != "fused_c128":
+            res[name]["product_pass"] = product_pass_roofline(be, ns, amp)
+        else:
+            res[name]["note"] = ("the same H/X/MCZ iteration on Fused128Backend: each run of H and X layers "
+                                 "is one product of per-qubit 2x2s (3 product passes instead of 6)")
         be.close()
     cfg = {"workload": "24-qubit iterative Grover: 23 search qubits + |-> ancilla, F(x==x*) oracle, {} iterations, "
                        "seeded measurement".format(t_iter), "n_qubits": ns + 1, "iterations": t_iter,
