@@ -363,8 +363,10 @@ def test_recorder_edges():
     assert lib.qipgpu_fused_record(fz.handle, 2, _capi.ptr(k), _capi.ptr(q), _capi.ptr(m)) == _capi.EINVAL
     assert fz.pending_gates() == 0
     fz.close()
+    # a complex128 state records too (its blocks are applied in FP64); recording nothing is a no-op
     c128 = GpuBackend.make_state(n, [], [])
-    assert lib.qipgpu_fused_record(c128.handle, 0, None, None, None) == _capi.EINVAL
+    assert lib.qipgpu_fused_record(c128.handle, 0, None, None, None) == 0
+    assert numpy.array_equal(c128.get_state(), GpuBackend.make_state(n, [], []).get_state())
     c128.close()
 
 
