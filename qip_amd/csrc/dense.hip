@@ -186,6 +186,9 @@ struct MfmaShape {
     static constexpr int RBW = D / 32;           // 16-row real blocks per wave (4 waves)
     static constexpr int CB = B / 16;            // 16-column blocks
     static constexpr int KS = D / 2;             // 4-column k-steps over the 2 D real columns
+    // at K = 6 the operator fragments take 128 VGPRs: without prefetch registers two workgroups (two
+    // waves per SIMD) fit; at K = 5 they fit with them
+    static constexpr bool PREFETCH = K == 5;
     static constexpr int XS = B + 1;             // padded plane row, row-major planes (doubles)
     static constexpr int XSJ = D + 1;            // padded plane column, base-major planes
     static constexpr int PLANE = D * B + (D > B ? D : B);
@@ -195,7 +198,7 @@ struct MfmaShape {
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 template <typename R, int K, bool ROWS_FAST>
-__global__ __launch_bounds__(kTileThreads, K == 5 ? 2 : 1) void k_dense_mfma(cx<R> *amps, uint64_t nbase, uint64_t ctrl_mask,
+__global__ __launch_bounds__(kTileThreads, 2) void k_dense_mfma(cx<R> *amps, uint64_t nbase, uint64_t ctrl_mask,
                                                                 PosList pl, const uint64_t *__restrict__ off_g,
                                                                 const cx<double> *__restrict__ U_g, uint64_t ntiles) {
     using S = TileShape<K>;
@@ -314,17 +317,32 @@ __global__ __launch_bounds__(kTileThreads, K == 5 ? 2 : 1) void k_dense_mfma(cx<
     const uint64_t G = gridDim.x;
     uint64_t t = blockIdx.x;
     if (t >= ntiles) return;                   // the whole workgroup: no barrier is skipped by part of it
-    load_tile(t);
-    for (; t < ntiles; t += G) {
-        __syncthreads();                       // the previous tile's outputs have been read back
-        write_x();
-        __syncthreads();
-        if (t + G < ntiles) load_tile(t + G);  // in flight during the products
-        products();
-        __syncthreads();                       // every wave has read X
-        write_y();
-        __syncthreads();
-        store_y(t);
+    if constexpr (M::PREFETCH) {
+        load_tile(t);
+        for (; t < ntiles; t += G) {
+            __syncthreads();                   // the previous tile's outputs have been read back
+            write_x();
+            __syncthreads();
+            if (t + G < ntiles) load_tile(t + G);   // in flight during the products
+            products();
+            __syncthreads();                   // every wave has read X
+            write_y();
+            __syncthreads();
+            store_y(t);
+        }
+    } else {
+        // no prefetch registers: the other workgroup on the CU covers this one's loads
+        for (; t < ntiles; t += G) {
+            __syncthreads();
+            load_tile(t);
+            write_x();
+            __syncthreads();
+            products();
+            __syncthreads();
+            write_y();
+            __syncthreads();
+            store_y(t);
+        }
     }
 }
 
@@ -563,9 +581,8 @@ int launch_tile(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_ma
     const bool rows_fast = std::find(tbits.begin(), tbits.end(), 0) != tbits.end();
     const cx<double> *dU = (const cx<double> *)((char *)d + sizeof(uint64_t) * D);
     if constexpr (K >= 5) {
-        // FP64 MFMA: the waves hold their operator fragments in registers, at K = 6 one wave per SIMD
-        // (one workgroup per CU), at K = 5 two
-        const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)device_cus() * (K == 5 ? 2 : 1));
+        // FP64 MFMA: the waves hold their operator fragments in registers; two workgroups per CU
+        const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)device_cus() * 2);
         if (rows_fast)
             hipLaunchKernelGGL((k_dense_mfma<R, K, true>), dim3((uint32_t)grid), dim3(kTileThreads), 0, s->stream,
                                (cx<R> *)s->amps, nbase, ctrl_mask, poslist(allpos), (const uint64_t *)d, dU, ntiles);
