@@ -116,6 +116,15 @@ int qipgpu_table_create(int32_t n_reg1, const int32_t *table, qipgpu_table **out
 int qipgpu_table_destroy(qipgpu_table *t);
 int qipgpu_func_apply_table(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, int32_t n_reg2,
                             const int32_t *reg2, const qipgpu_table *t);
+/* qipgpu_func_apply_table with flags. QIPGPU_FUNC_REFERENCE_BOUND reproduces the reference's loop
+ * over the rest qubits (func_apply.pyx:107-111), which sets only the first n_reg2 of them (in qubit
+ * order): when more than n_reg2 qubits lie outside both registers, every amplitude with one of the
+ * other rest qubits at 1 ends 0, as the reference's zeroed output leaves it (func_apply.pyx:42-44).
+ * With n_rest <= n_reg2 the flag changes nothing. Without it the permutation is applied to the
+ * whole state. */
+#define QIPGPU_FUNC_REFERENCE_BOUND 1
+int qipgpu_func_apply_table_ex(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, int32_t n_reg2,
+                               const int32_t *reg2, const qipgpu_table *t, int32_t flags);
 
 /* ---- reductions and measurement (qip/ext/kronprod.pyx:238-490) ------------------------------- */
 

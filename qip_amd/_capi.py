@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libqipgpu.so")
 
 C128, C64 = 0, 1
+FUNC_REFERENCE_BOUND = 1   # QIPGPU_FUNC_REFERENCE_BOUND
 OK, EINVAL, ENOMEM, EHIP, EZERODIV, EUNSUP = 0, -1, -2, -3, -4, -5
 
 _P = ctypes.c_void_p
@@ -44,6 +45,7 @@ SIGNATURES = {
     "qipgpu_table_create": (ctypes.c_int, [_I32, _P, _PP]),
     "qipgpu_table_destroy": (ctypes.c_int, [_P]),
     "qipgpu_func_apply_table": (ctypes.c_int, [_P, _I32, _P, _I32, _P, _P]),
+    "qipgpu_func_apply_table_ex": (ctypes.c_int, [_P, _I32, _P, _I32, _P, _P, _I32]),
     "qipgpu_total_prob": (ctypes.c_int, [_P, _PD]),
     "qipgpu_probabilities": (ctypes.c_int, [_P, _I32, _P, _P]),
     "qipgpu_probabilities_noalloc": (ctypes.c_int, [_P, _I32, _P, _P]),

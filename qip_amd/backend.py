@@ -7,7 +7,7 @@ lives in HBM behind a libqipgpu handle and every method is one or a few HIP kern
 
   make_state            -> qipgpu_init_product / qipgpu_set_range        (backend.py:73-104)
   kronselect_dot        -> planner.plan + qipgpu_apply_matrix             (backend.py:112-116)
-  func_apply            -> qipgpu_func_apply_table                        (backend.py:118-121)
+  func_apply            -> qipgpu_func_apply_table_ex                     (backend.py:118-121)
   total_prob            -> qipgpu_total_prob                              (backend.py:123-124)
   measure               -> qipgpu_measure                                 (backend.py:126-134)
   reduce_measure        -> qipgpu_reduce_measure                          (backend.py:136-151)
@@ -25,7 +25,8 @@ Differences from CythonBackend, all deliberate:
     end, kronprod.pyx:298-311), ties ordered by ascending outcome;
   * func_apply applies the intended permutation on every rest register bit (the reference loops
     j < n_reg2 over the rest qubits, func_apply.pyx:109, and loses amplitudes when there are more
-    rest qubits than reg2 qubits); f tables are cached per function object.
+    rest qubits than reg2 qubits); set ``reference_func_bound = True`` for the reference's output
+    there.
 """
 import ctypes
 import random
@@ -241,6 +242,11 @@ class GpuBackend(StateType):
     # to keep one device table per function object instead (only for functions whose values never
     # change).
     cache_func_tables = False
+    # True: func_apply keeps the reference's rest-qubit loop bound (func_apply.pyx:107-111). With
+    # more rest qubits than reg2 qubits, the reference sets only the first n_reg2 rest qubits (in
+    # qubit order) of its output index, so every amplitude with one of the others at 1 ends 0. The
+    # default applies the permutation to the whole state.
+    reference_func_bound = False
 
     def __init__(self, n: int, handle: ctypes.c_void_p, dtype: int):
         super().__init__(n, None)
@@ -419,8 +425,9 @@ class GpuBackend(StateType):
         reg1 = _capi.i32(reg1_indices)
         reg2 = _capi.i32(reg2_indices)
         table = func_table(func, len(reg1), cache=self.cache_func_tables)
-        _capi.check(self._lib.qipgpu_func_apply_table(self._h, len(reg1), _capi.ptr(reg1), len(reg2),
-                                                      _capi.ptr(reg2), table.handle), "func_apply")
+        flags = _capi.FUNC_REFERENCE_BOUND if self.reference_func_bound else 0
+        _capi.check(self._lib.qipgpu_func_apply_table_ex(self._h, len(reg1), _capi.ptr(reg1), len(reg2),
+                                                         _capi.ptr(reg2), table.handle, flags), "func_apply")
 
     # -- reductions and measurement -------------------------------------------------------------
     def total_prob(self) -> float:

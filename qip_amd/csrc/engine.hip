@@ -1014,15 +1014,58 @@ __global__ __launch_bounds__(256) void k_func_tile(cx<R> *amps, const int32_t *_
     }
 }
 
+// Collapse (kronprod.pyx:439-444), in place: amplitudes whose measured bits equal `target` are
+// scaled, the rest are zeroed without being read.
+template <typename R>
+__global__ __launch_bounds__(256) void k_collapse(cx<R> *amps, uint64_t len, uint64_t mask, uint64_t target, double mult) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
+        cx<R> o;
+        if ((i & mask) == target) {
+            const cx<R> v = amps[i];
+            o.re = (R)((double)v.re * mult);
+            o.im = (R)((double)v.im * mult);
+        } else {
+            o.re = 0;
+            o.im = 0;
+        }
+        amps[i] = o;
+    }
+}
+
 namespace {
+
+// The reference's rest-qubit loop (func_apply.pyx:107-111) sets only the first n_reg2 rest qubits,
+// in qubit order, into its output index; the others stay 0 there, so every output amplitude with
+// one of them at 1 keeps the 0 it was cleared to (func_apply.pyx:42-44). The permutation leaves the
+// rest bits where they are, so that output is the whole permutation with those amplitudes zeroed:
+// k_collapse with the dropped rest bits as the mask, target 0 and factor 1 (x * 1.0 == x).
+static int zero_dropped_rest(qipgpu_state *s, const std::vector<int> &other, int32_t n_reg2) {
+    uint64_t mask = 0;
+    for (size_t j = (size_t)n_reg2; j < other.size(); ++j) mask |= 1ull << (s->n - 1 - other[j]);
+    if (!mask) return QIPGPU_OK;
+    const uint32_t nb = std::min<uint32_t>(blocks_for(s->len, 256 * 4), 1u << 20);
+    if (s->dtype == QIPGPU_C128)
+        hipLaunchKernelGGL(k_collapse<double>, dim3(nb), dim3(256), 0, s->stream, (cx<double> *)s->amps, s->len, mask,
+                           (uint64_t)0, 1.0);
+    else
+        hipLaunchKernelGGL(k_collapse<float>, dim3(nb), dim3(256), 0, s->stream, (cx<float> *)s->amps, s->len, mask,
+                           (uint64_t)0, 1.0);
+    QIP_LAUNCHED();
+    return QIPGPU_OK;
+}
+
+static int permute_reg2(qipgpu_state *s, int32_t n_reg1, int32_t n_reg2, const int32_t *d_table, uint64_t nonzero,
+                        const FuncArgs &a);
 
 // `nonzero` = entries of the table that are not 0: where fewer than one x in 64 moves anything, the
 // per-pair kernel (which touches only the moved pairs) runs instead of the tiles.
 static int func_apply_dev(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, int32_t n_reg2, const int32_t *reg2,
-                          const int32_t *d_table, uint64_t nonzero) {
+                          const int32_t *d_table, uint64_t nonzero, int32_t flags) {
     const int n = s->n;
     if (n_reg1 < 1 || n_reg2 < 1 || n_reg1 + n_reg2 > n)
         return set_error(QIPGPU_EINVAL, "func_apply: register sizes %d + %d for n=%d", n_reg1, n_reg2, n);
+    if (flags & ~QIPGPU_FUNC_REFERENCE_BOUND) return set_error(QIPGPU_EINVAL, "func_apply: unknown flags %d", flags);
     std::vector<int32_t> all(reg1, reg1 + n_reg1);
     all.insert(all.end(), reg2, reg2 + n_reg2);
     int rc = check_qubits(s, (int32_t)all.size(), all.data(), "func_apply registers");
@@ -1049,6 +1092,16 @@ static int func_apply_dev(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, 
     for (int q = 0; q < n; ++q)
         if (!(used & (1ull << q))) other.push_back(q);
     for (int b = 0; b < a.nother; ++b) a.rpos[b] = (uint8_t)(n - 1 - other[a.nother - 1 - b]);
+    rc = permute_reg2(s, n_reg1, n_reg2, d_table, nonzero, a);
+    if (rc) return rc;
+    return (flags & QIPGPU_FUNC_REFERENCE_BOUND) ? zero_dropped_rest(s, other, n_reg2) : QIPGPU_OK;
+}
+
+// The permutation itself: |x>|q>|r> -> |x>|f(x) xor q>|r> over the whole state, on the tile kernel or
+// the per-pair kernel.
+static int permute_reg2(qipgpu_state *s, int32_t n_reg1, int32_t n_reg2, const int32_t *d_table, uint64_t nonzero,
+                        const FuncArgs &a) {
+    const int n = s->n;
     if (n >= kFuncTileBits && n_reg2 <= 8 && n_reg1 <= 31 && nonzero * 64 >= (1ull << n_reg1)) {
         FuncTileArgs f;
         std::memset(&f, 0, sizeof(f));
@@ -1126,7 +1179,14 @@ int qipgpu_func_apply_table(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1
                             const qipgpu_table *t) {
     if (!t || t->n_reg1 != n_reg1)
         return set_error(QIPGPU_EINVAL, "function table is for a %d-qubit register, not %d", t ? t->n_reg1 : -1, n_reg1);
-    return func_apply_dev(s, n_reg1, reg1, n_reg2, reg2, t->d, t->nonzero);
+    return func_apply_dev(s, n_reg1, reg1, n_reg2, reg2, t->d, t->nonzero, 0);
+}
+
+int qipgpu_func_apply_table_ex(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, int32_t n_reg2,
+                               const int32_t *reg2, const qipgpu_table *t, int32_t flags) {
+    if (!t || t->n_reg1 != n_reg1)
+        return set_error(QIPGPU_EINVAL, "function table is for a %d-qubit register, not %d", t ? t->n_reg1 : -1, n_reg1);
+    return func_apply_dev(s, n_reg1, reg1, n_reg2, reg2, t->d, t->nonzero, flags);
 }
 
 int qipgpu_func_apply(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, int32_t n_reg2, const int32_t *reg2,
@@ -1138,7 +1198,7 @@ int qipgpu_func_apply(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, int3
     QIP_HIP(hipStreamSynchronize(s->stream));
     QIP_HIP(hipMemcpy(s->scratch, table, bytes, hipMemcpyHostToDevice));
     rc = func_apply_dev(s, n_reg1, reg1, n_reg2, reg2, (const int32_t *)s->scratch,
-                        count_nonzero(table, 1ull << n_reg1));
+                        count_nonzero(table, 1ull << n_reg1), 0);
     if (rc) return rc;
     QIP_HIP(hipStreamSynchronize(s->stream));  // scratch may be reused by the next call
     return QIPGPU_OK;
@@ -1531,25 +1591,6 @@ static int marginals_host(qipgpu_state *s, int32_t k, const int32_t *indices, Or
 }
 
 }  // namespace
-
-// Collapse (kronprod.pyx:439-444), in place: amplitudes whose measured bits equal `target` are
-// scaled, the rest are zeroed without being read.
-template <typename R>
-__global__ __launch_bounds__(256) void k_collapse(cx<R> *amps, uint64_t len, uint64_t mask, uint64_t target, double mult) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) {
-        cx<R> o;
-        if ((i & mask) == target) {
-            const cx<R> v = amps[i];
-            o.re = (R)((double)v.re * mult);
-            o.im = (R)((double)v.im * mult);
-        } else {
-            o.re = 0;
-            o.im = 0;
-        }
-        amps[i] = o;
-    }
-}
 
 // reduce_measure (kronprod.pyx:482-489): out[i] = in[measured bits m, unmeasured bits = low
 // n-k bits of i] * mult, out of place.

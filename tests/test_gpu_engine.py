@@ -211,6 +211,32 @@ def test_func_apply_tiles(n, reg1, reg2, kind, dtype):
     assert numpy.array_equal(be.get_state().astype(numpy.complex128), ref)
 
 
+@pytest.mark.parametrize("dtype", [numpy.complex128, numpy.complex64])
+@pytest.mark.parametrize("n,reg1,reg2", [
+    (9, [1, 4], [6]),                              # per-pair kernel (n < 12); rest 0, 2, 3, 5, 7, 8
+    (14, [3, 9, 0, 12], [7, 1, 13]),               # tile kernel; 7 rest qubits, 4 of them dropped
+    (16, list(range(12)), [15]),                   # one reg2 qubit, rest 12, 13, 14: two dropped
+    (14, list(range(6, 14)), list(range(6))),      # no rest qubits: the flag changes nothing
+    (13, [0, 1, 2], [5, 6, 7, 8, 9, 10, 11, 12, 3]),  # reg2 of 9 qubits, rest 4 only
+])
+def test_func_apply_reference_bound(n, reg1, reg2, dtype):
+    """reference_func_bound = True: the reference's rest-qubit loop bound (func_apply.pyx:107-111),
+    amplitudes with a dropped rest qubit at 1 end 0; bit-exact against the oracle's faithful mode
+    on both kernels and both precisions."""
+    psi = rand_state(n)
+    be = gpu_from(psi, n, dtype)
+    be.reference_func_bound = True
+    v = be.get_state().astype(numpy.complex128)
+    table = RNG.integers(0, 2 ** 31 - 1, size=2 ** len(reg1)).astype(numpy.int32)
+    be.func_apply(numpy.array(reg1, dtype=numpy.int32), numpy.array(reg2, dtype=numpy.int32), table)
+    ref = numpy.zeros_like(v)
+    oracle.func_apply(reg1, reg2, None, v, n, ref, faithful=True, table=table)
+    assert numpy.array_equal(be.get_state().astype(numpy.complex128), ref)
+    rest = n - len(reg1) - len(reg2)
+    if rest > len(reg2):
+        assert numpy.count_nonzero(ref) < 2 ** n // 2
+
+
 def test_func_apply_callable_and_cache():
     n = 6
     psi = rand_state(n)

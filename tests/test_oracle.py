@@ -250,12 +250,39 @@ def test_func_apply_permutation(trial):
     n_other = n - n1 - n2
     out_f = numpy.zeros_like(psi)
     oracle.func_apply(reg1, reg2, None, psi, n, out_f, faithful=True, table=table)
-    if n_other <= n2:
-        assert numpy.array_equal(out_f, expect)     # the reference's loop bound is harmless here
-    else:
-        # func_apply.pyx:109-111 drops rest bits beyond n_reg2: amplitude is lost
-        assert numpy.sum(numpy.abs(out_f) ** 2) < numpy.sum(numpy.abs(psi) ** 2) - 1e-12 or \
-            numpy.count_nonzero(out_f) < numpy.count_nonzero(expect)
+    assert numpy.array_equal(out_f, reference_bound_expected(reg1, reg2, expect, n))
+
+
+def reference_bound_expected(reg1, reg2, expect, n):
+    """The reference's output under its rest-qubit loop bound (func_apply.pyx:107-111): only the
+    first len(reg2) rest qubits in qubit order are iterated, so the full permutation's amplitudes
+    with any later rest qubit at 1 stay at the 0 the output was cleared to (func_apply.pyx:42-44).
+    With no more rest qubits than reg2 qubits this is the full permutation."""
+    rest = [q for q in range(n) if q not in reg1 and q not in reg2]
+    dropped = sum(1 << (n - 1 - q) for q in rest[len(reg2):])
+    keep = (numpy.arange(2 ** n) & dropped) == 0
+    return numpy.where(keep, expect, 0)
+
+
+@pytest.mark.parametrize("n,reg1,reg2", [
+    (7, [1, 4], [6], ),               # rest 0, 2, 3, 5: 2, 3, 5 dropped
+    (8, [7, 0], [3, 5], ),            # rest 1, 2, 4, 6: 4, 6 dropped
+    (6, [2], [0], ),                  # rest 1, 3, 4, 5: 3, 4, 5 dropped
+    (6, [5, 4, 3], [0, 1, 2], ),      # no rest qubits
+])
+def test_func_apply_reference_bound(n, reg1, reg2):
+    """faithful=True against the full permutation with the dropped rest qubits' amplitudes cleared,
+    on cases with more rest qubits than reg2 qubits (amplitude is lost) and with none."""
+    table = RNG.integers(-9, 9, size=2 ** len(reg1)).astype(numpy.int32)
+    psi = rand_state(n)
+    expect = func_apply_expected(reg1, reg2, table, psi, n)
+    out_f = numpy.zeros_like(psi)
+    oracle.func_apply(reg1, reg2, None, psi, n, out_f, faithful=True, table=table)
+    bound = reference_bound_expected(reg1, reg2, expect, n)
+    assert numpy.array_equal(out_f, bound)
+    rest = n - len(reg1) - len(reg2)
+    if rest > len(reg2):
+        assert numpy.sum(numpy.abs(out_f) ** 2) < numpy.sum(numpy.abs(psi) ** 2) - 1e-3
 
 
 def test_func_apply_overflow():
