@@ -76,8 +76,8 @@ int qipgpu_add_range(qipgpu_state *s, int64_t start, int64_t end, const void *ho
  * `controls` qubit being 1 (the C_MAT structure diag(I, M), kronprod.pyx:215-225). The engine
  * finds further control qubits and diagonal structure inside `mat` itself, so a caller may pass
  * a whole controlled operator as one dense matrix. k <= QIPGPU_MAX_DENSE. Up to 3 targets are
- * applied from registers, 4 through LDS tiles, 5 and 6 through LDS tiles on FP64 MFMA, 7..12 as
- * an FP64 GEMM out of place (a temporary of 2^(n - controls) amplitudes). */
+ * applied from registers, 4..6 through LDS tiles on FP64 MFMA, 7..12 as an FP64 GEMM out of place
+ * (a temporary of 2^(n - controls) amplitudes). */
 int qipgpu_apply_matrix(qipgpu_state *s, int32_t k, const int32_t *targets, const double *mat,
                         int32_t n_ctrl, const int32_t *controls);
 

@@ -7,14 +7,12 @@
 // sum_j U[i][j] * in[base + off[j]]. That is the matrix product Y = U X with X[j][b] the amplitudes
 // of base b. FP64 arithmetic for both state precisions, rounded once on store.
 //
-//   k_dense_tile<R, K>   K = 4, in place. A persistent workgroup keeps U in LDS; per tile of B
-//                        bases it stages X (D x B = 4096 amplitudes, 64 KB in FP64) in LDS, each
-//                        thread forms 4 rows x 4 bases of Y in registers, and the next tile's
-//                        amplitudes are loaded into registers while this one is multiplied. A tile
-//                        holds every amplitude of its bases before any is written, so in place is safe.
-//   k_dense_mfma<R, K>   K = 5, 6, in place, the same tiles on FP64 MFMA (v_mfma_f64_16x16x4f64): each
-//                        wave keeps its share of the real form of U in registers for the launch, X
-//                        and Y pass through LDS as Re / Im planes.
+//   k_dense_mfma<R, K>   K = 4..6, in place. A persistent workgroup takes tiles of B bases: it stages
+//                        X (D x B = 4096 amplitudes, 64 KB in FP64) in LDS as Re / Im planes, forms
+//                        Y = U X with three real products on FP64 MFMA (v_mfma_f64_16x16x4f64), each
+//                        wave holding its share of Re U and Im U in registers for the launch, and
+//                        writes Y back through the same planes. A tile holds every amplitude of its
+//                        bases before any is written, so in place is safe.
 //   k_dense_gemm<R>      K = 7..12, out of place: a 64 x 64 output tile per workgroup over 16-column
 //                        steps of j through LDS, into a compact [D][nbase] buffer; k_dense_scatter
 //                        writes it back to the bases' positions.
@@ -61,138 +59,42 @@ __device__ __forceinline__ uint64_t base_of(uint64_t g, const PosList &pl, uint6
     return insert_zeros(g, pl) | ctrl_mask;
 }
 
-// ---- K = 4..6: LDS tiles, in place ---------------------------------------------------------------
-
 constexpr int kTileThreads = 256;
 constexpr int kTileElems = 4096;        // D x B amplitudes per tile
 
-template <int K>
-struct TileShape {
-    static constexpr int D = 1 << K;
-    static constexpr int RB = 4;                       // rows per thread
-    static constexpr int BB = 4;                       // bases per thread
-    static constexpr int RG = D / RB;                  // row groups
-    static constexpr int BG = kTileThreads / RG;       // base groups
-    static constexpr int B = BG * BB;                  // bases per tile
-    static constexpr int LOADS = kTileElems / kTileThreads;
-    static_assert(D * B == kTileElems, "tile shape");
-};
-
-template <typename R, int K, bool ROWS_FAST>
-__global__ __launch_bounds__(kTileThreads) void k_dense_tile(cx<R> *amps, uint64_t nbase, uint64_t ctrl_mask,
-                                                             PosList pl, const uint64_t *__restrict__ off_g,
-                                                             const cx<double> *__restrict__ U_g, uint64_t ntiles) {
-    using S = TileShape<K>;
-    constexpr int D = S::D, B = S::B;
-    // Row strides of X and U: with rows across lanes, consecutive lanes write consecutive rows of X
-    // and read consecutive rows of U, which one slot of padding spreads over the banks
-    constexpr int XS = ROWS_FAST ? B + 1 : B;
-    constexpr int US = ROWS_FAST ? D + 1 : D;
-    __shared__ cx<double> sU[D * US];
-    __shared__ cx<double> sX[D * XS];
-    __shared__ uint64_t sOff[D];
-    const int tid = threadIdx.x;
-    for (int e = tid; e < D * D; e += kTileThreads) sU[(e / D) * US + e % D] = U_g[e];
-    for (int e = tid; e < D; e += kTileThreads) sOff[e] = off_g[e];
-    __syncthreads();
-
-    // Element i of this thread is (row j, base b) of the tile, e = tid + 256 i. Bases across lanes
-    // (b = e mod B): a thread keeps one base. Rows across lanes (ROWS_FAST, j = e mod D; used when
-    // state bit 0 is a target, so a base's amplitudes are contiguous): a wave reads whole bases.
-    constexpr int BSTEP = kTileThreads / D;
-    constexpr int JSTEP = kTileThreads / B;
-    const int b_ld = ROWS_FAST ? tid / D : tid % B;
-    const int j_ld = ROWS_FAST ? tid % D : tid / B;
-    // compute role: rows rg + RG * r, bases bg + BG * q; with ROWS_FAST the row group varies fastest
-    // across lanes, so the stores of one instruction cover whole runs of rows
-    const int rg = ROWS_FAST ? tid % S::RG : tid / S::BG;
-    const int bg = ROWS_FAST ? tid / S::RG : tid % S::BG;
-
-    cx<R> pre[S::LOADS];
-    auto load_tile = [&](uint64_t t) {
-        // bases past the end load a valid amplitude (their products are not stored): unconditional
-        // loads stay in flight together
-        if constexpr (ROWS_FAST) {
-#pragma unroll
-            for (int i = 0; i < S::LOADS; ++i) {
-                const uint64_t g = t * B + b_ld + BSTEP * i;
-                pre[i] = amps[base_of(g < nbase ? g : nbase - 1, pl, ctrl_mask) + sOff[j_ld]];
-            }
-        } else {
-            const uint64_t g = t * B + b_ld;
-            const uint64_t base = base_of(g < nbase ? g : nbase - 1, pl, ctrl_mask);
-#pragma unroll
-            for (int i = 0; i < S::LOADS; ++i) pre[i] = amps[base + sOff[j_ld + i * JSTEP]];
-        }
-    };
-    uint64_t t = blockIdx.x;
-    if (t < ntiles) load_tile(t);
-    for (; t < ntiles; t += gridDim.x) {
-        __syncthreads();                       // the previous tile's products have read sX
-#pragma unroll
-        for (int i = 0; i < S::LOADS; ++i) {
-            if constexpr (ROWS_FAST) sX[j_ld * XS + b_ld + BSTEP * i] = to_d(pre[i]);
-            else sX[(j_ld + i * JSTEP) * XS + b_ld] = to_d(pre[i]);
-        }
-        __syncthreads();
-        if (t + gridDim.x < ntiles) load_tile(t + gridDim.x);   // in flight during the products
-        cx<double> acc[S::RB][S::BB];
-#pragma unroll
-        for (int r = 0; r < S::RB; ++r)
-#pragma unroll
-            for (int q = 0; q < S::BB; ++q) acc[r][q].re = acc[r][q].im = 0.0;
-        for (int j = 0; j < D; ++j) {
-            cx<double> u[S::RB], x[S::BB];
-#pragma unroll
-            for (int r = 0; r < S::RB; ++r) u[r] = sU[(rg + S::RG * r) * US + j];
-#pragma unroll
-            for (int q = 0; q < S::BB; ++q) x[q] = sX[j * XS + bg + S::BG * q];
-#pragma unroll
-            for (int r = 0; r < S::RB; ++r)
-#pragma unroll
-                for (int q = 0; q < S::BB; ++q) mac(acc[r][q], u[r], x[q]);
-        }
-#pragma unroll
-        for (int q = 0; q < S::BB; ++q) {
-            const uint64_t g = t * B + bg + S::BG * q;
-            if (g < nbase) {
-                const uint64_t base = base_of(g, pl, ctrl_mask);
-#pragma unroll
-                for (int r = 0; r < S::RB; ++r) amps[base + sOff[rg + S::RG * r]] = from_d<R>(acc[r][q]);
-            }
-        }
-    }
-}
-
-// ---- K = 5, 6: the same tiles on FP64 MFMA --------------------------------------------------------
+// ---- K = 4..6: tiles in place on FP64 MFMA -------------------------------------------------------
 //
-// Y = U X as a real product: with real row 2i + a (a = 0 re, 1 im) of Y and real row 2j + a' of X,
-// the real matrix is M[2i + a][2j + a'] = Re U[i][j] if a = a', -Im U[i][j] for (0, 1), Im U[i][j] for
-// (1, 0) (2D x 2D). v_mfma_f64_16x16x4f64 computes D[16 x 16] += A[16 x 4] B[4 x 16]; lane l holds
-// A[l % 16][l / 16] and B[l / 16][l % 16], and D[4 i + l / 16][l % 16] for its items i = 0..3. So the
-// 16 real rows rho of a block are ordered rho = 4 i + g (g = rho % 4): items i = 0, 1 are Re, Im of
-// complex row 2 g and items 2, 3 those of row 2 g + 1, and every lane ends with whole complex outputs.
-// The 4 real columns of k-step s are Re, Im of complex rows 2 s and 2 s + 1 of X.
+// Y = U X with U = A + i Bm and X = P + i Q takes three real D x D products (the 3M method, Higham,
+// "Stability of a method for multiplying complex matrices with three real matrix multiplications",
+// SIAM J. Matrix Anal. Appl. 13 (1992)): T1 = A P, T2 = Bm Q, T3 = (A + Bm)(P + Q), then
+// Re Y = T1 - T2 and Im Y = T3 - T1 - T2. That is 6 D^2 real FMAs per base instead of the 8 of the
+// real 2D x 2D form. Re Y rounds as the usual form does; the rounding error of Im Y is bounded in
+// terms of (|A| + |Bm|)(|P| + |Q|) instead of |A||Q| + |Bm||P|, at most a small constant factor more
+// in FP64 (the paper's bound).
+// v_mfma_f64_16x16x4f64 computes D[16 x 16] += A[16 x 4] B[4 x 16]; lane l holds A[l % 16][l / 16]
+// and B[l / 16][l % 16], and D[4 i + l / 16][l % 16] for its items i = 0..3, so the three products of
+// one output block land in the same lanes and are combined in registers.
 //
-// Wave w owns real row blocks w * RBW .. (w + 1) * RBW - 1 (RBW = D / 32) for every column block of
-// the tile; its A fragments (RBW x KS doubles per lane, KS = D / 2 k-steps) are gathered from U once
-// per launch and stay in registers. X goes through LDS as two FP64 planes (Re, Im) with one slot of
-// padding per row (or per base, when lanes run along the rows), and so does Y on its way out, so
-// loads and stores keep the tile kernel's coalesced mapping. Per tile: 2 D x 2 D x B FMAs in D / 2 x (B / 16) x RBW MFMAs per wave.
+// Wave w owns output row block w % RB (RB = D / 16 blocks of 16 complex rows) and CW consecutive
+// 16-base column blocks starting at (w / RB) CW; its fragments of A and Bm (2 x D / 4 doubles per
+// lane) are gathered from U once per launch and stay in registers, and A + Bm is formed per k-step.
+// X goes through LDS as two FP64 planes (Re, Im) with one slot of padding per row (or per base, when
+// lanes run along the rows), and so does Y on its way out, so loads and stores stay coalesced. Per
+// tile: 3 D x D x B FMAs in 3 x (D / 4) x CW MFMAs per wave.
 template <int K>
 struct MfmaShape {
     static constexpr int D = 1 << K;
     static constexpr int B = kTileElems / D;     // bases per tile
-    static constexpr int RBW = D / 32;           // 16-row real blocks per wave (4 waves)
+    static constexpr int LOADS = kTileElems / kTileThreads;   // amplitudes per thread
+    static constexpr int RB = D / 16;            // 16-row output blocks
     static constexpr int CB = B / 16;            // 16-column blocks
-    static constexpr int KS = D / 2;             // 4-column k-steps over the 2 D real columns
-    // at K = 6 the operator fragments take 128 VGPRs: without prefetch registers two workgroups (two
-    // waves per SIMD) fit; at K = 5 they fit with them
-    static constexpr bool PREFETCH = K == 5;
+    static constexpr int CW = CB * RB / 4;       // column blocks per wave (4 waves)
+    static constexpr int KS = D / 4;             // 4-column k-steps over the D complex columns
     static constexpr int XS = B + 1;             // padded plane row, row-major planes (doubles)
     static constexpr int XSJ = D + 1;            // padded plane column, base-major planes
     static constexpr int PLANE = D * B + (D > B ? D : B);
-    static_assert(K == 5 || K == 6, "MFMA tiles for 5 and 6 targets");
+    static_assert(K >= 4 && K <= 6, "MFMA tiles for 4 to 6 targets");
+    static_assert(CW * 4 == CB * RB, "column blocks split evenly over the waves");
 };
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -201,10 +103,8 @@ template <typename R, int K, bool ROWS_FAST>
 __global__ __launch_bounds__(kTileThreads, 2) void k_dense_mfma(cx<R> *amps, uint64_t nbase, uint64_t ctrl_mask,
                                                                 PosList pl, const uint64_t *__restrict__ off_g,
                                                                 const cx<double> *__restrict__ U_g, uint64_t ntiles) {
-    using S = TileShape<K>;
     using M = MfmaShape<K>;
     constexpr int D = M::D, B = M::B, XS = M::XS;
-    static_assert(S::B == M::B, "tile shape");
     // Re and Im planes of the tile: row-major (row * XS + base) when lanes run along the bases,
     // base-major (base * XSJ + row) when they run along the rows, so that consecutive lanes touch
     // consecutive slots in the loads and stores and the MFMA operand reads spread over the banks
@@ -216,28 +116,27 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_dense_mfma(cx<R> *amps, uin
     const int lane = tid & 63, wave = tid >> 6;
     for (int e = tid; e < D; e += kTileThreads) sOff[e] = off_g[e];
 
-    // this wave's A fragments: lane -> real row rho = lane % 16 of each block, k = lane / 16 of each step
+    // this wave's fragments: lane -> output row rho = lane % 16 of its block, k = lane / 16 of each
+    // k-step; afr[0] = Re U, afr[1] = Im U (Re U + Im U is formed per k-step)
     const int rho = lane & 15, kap = lane >> 4;
-    const int a_out = (rho >> 2) & 1, m_out = rho >> 3, g_out = rho & 3;
-    const int a_in = kap & 1;
-    double afr[M::RBW][M::KS];
+    const int row0 = 16 * (wave % M::RB);        // first output row of the wave's block
+    const int col0 = 16 * M::CW * (wave / M::RB); // first base of the wave's column blocks
+    double afr[2][M::KS];
 #pragma unroll
-    for (int r = 0; r < M::RBW; ++r) {
-        const int i = 8 * (wave * M::RBW + r) + 2 * g_out + m_out;       // complex output row
-#pragma unroll
-        for (int s = 0; s < M::KS; ++s) {
-            const cx<double> u = U_g[i * D + 2 * s + (kap >> 1)];
-            afr[r][s] = a_out == a_in ? u.re : (a_out ? u.im : -u.im);
-        }
+    for (int s = 0; s < M::KS; ++s) {
+        const cx<double> u = U_g[(row0 + rho) * D + 4 * s + kap];
+        afr[0][s] = u.re;
+        afr[1][s] = u.im;
     }
     __syncthreads();
 
-    // loads and stores: element i of this thread is (row j, base b), as in k_dense_tile
+    // loads and stores: element i of this thread is (row j, base b); lanes run along the bases, or
+    // along the rows when ROWS_FAST
     constexpr int BSTEP = kTileThreads / D;
     constexpr int JSTEP = kTileThreads / B;
     const int b_ld = ROWS_FAST ? tid / D : tid % B;
     const int j_ld = ROWS_FAST ? tid % D : tid / B;
-    cx<R> pre[S::LOADS];
+    cx<R> pre[M::LOADS];
     // position of base t B + v (v < B; a base past the end reads the last one, whose products are
     // not stored)
     auto base_at = [&](uint64_t t, int v) {
@@ -247,11 +146,11 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_dense_mfma(cx<R> *amps, uin
     auto load_tile = [&](uint64_t t) {
         if constexpr (ROWS_FAST) {
 #pragma unroll
-            for (int i = 0; i < S::LOADS; ++i) pre[i] = amps[base_at(t, b_ld + BSTEP * i) + sOff[j_ld]];
+            for (int i = 0; i < M::LOADS; ++i) pre[i] = amps[base_at(t, b_ld + BSTEP * i) + sOff[j_ld]];
         } else {
             const uint64_t base = base_at(t, b_ld);
 #pragma unroll
-            for (int i = 0; i < S::LOADS; ++i) pre[i] = amps[base + sOff[j_ld + i * JSTEP]];
+            for (int i = 0; i < M::LOADS; ++i) pre[i] = amps[base + sOff[j_ld + i * JSTEP]];
         }
     };
     // plane slot of element i of this thread
@@ -260,49 +159,57 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_dense_mfma(cx<R> *amps, uin
     // the tile's amplitudes (pre) into LDS as Re and Im planes
     auto write_x = [&]() {
 #pragma unroll
-        for (int i = 0; i < S::LOADS; ++i) {
+        for (int i = 0; i < M::LOADS; ++i) {
             const cx<double> v = to_d(pre[i]);
             sX[0][slot(i)] = v.re;
             sX[1][slot(i)] = v.im;
         }
     };
-    f64x4 acc[M::RBW][M::CB];
-    // Y = M X for the tile in LDS: lanes read the B operand from the Re or Im plane
+    f64x4 acc[3][M::CW];
+    // the three products of the tile in LDS: lanes read P and Q of row 4 s + kap, base col0 + 16 c + rho
     auto products = [&]() {
 #pragma unroll
-        for (int r = 0; r < M::RBW; ++r)
+        for (int m = 0; m < 3; ++m)
 #pragma unroll
-            for (int c = 0; c < M::CB; ++c) acc[r][c] = f64x4{0.0, 0.0, 0.0, 0.0};
-        const double *bplane = sX[kap & 1];
+            for (int c = 0; c < M::CW; ++c) acc[m][c] = f64x4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int s = 0; s < M::KS; ++s) {
-            const double *row = bplane + idx(2 * s + (kap >> 1), rho);
-            double bfr[M::CB];
+            // the empty asm makes the copies opaque, so the sum below stays in the tile loop instead
+            // of being hoisted out of it as a third set of fragment registers
+            double a0 = afr[0][s], a1 = afr[1][s];
+            asm volatile("" : "+v"(a0), "+v"(a1));
+            const double a2 = a0 + a1;
+            const int at = idx(4 * s + kap, col0 + rho);
+            double p[M::CW], q[M::CW];
 #pragma unroll
-            for (int c = 0; c < M::CB; ++c) bfr[c] = row[idx(0, 16 * c)];
+            for (int c = 0; c < M::CW; ++c) {
+                p[c] = sX[0][at + idx(0, 16 * c)];
+                q[c] = sX[1][at + idx(0, 16 * c)];
+            }
 #pragma unroll
-            for (int r = 0; r < M::RBW; ++r)
-#pragma unroll
-                for (int c = 0; c < M::CB; ++c)
-                    acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(afr[r][s], bfr[c], acc[r][c], 0, 0, 0);
+            for (int c = 0; c < M::CW; ++c) {
+                acc[0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, p[c], acc[0][c], 0, 0, 0);
+                acc[1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, q[c], acc[1][c], 0, 0, 0);
+                acc[2][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, p[c] + q[c], acc[2][c], 0, 0, 0);
+            }
         }
     };
-    // item q of a block holds real row 4 q + g: Re (q even) or Im (q odd) of complex row 2 g + q / 2
+    // item i of a block is output row row0 + 4 i + kap, base col0 + 16 c + rho
     auto write_y = [&]() {
 #pragma unroll
-        for (int r = 0; r < M::RBW; ++r)
+        for (int c = 0; c < M::CW; ++c)
 #pragma unroll
-            for (int c = 0; c < M::CB; ++c)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int i = 8 * (wave * M::RBW + r) + 2 * kap + (q >> 1);
-                    sX[q & 1][idx(i, 16 * c + rho)] = acc[r][c][q];
-                }
+            for (int i = 0; i < 4; ++i) {
+                const int at = idx(row0 + 4 * i + kap, col0 + 16 * c + rho);
+                const double t1 = acc[0][c][i], t2 = acc[1][c][i];
+                sX[0][at] = t1 - t2;
+                sX[1][at] = acc[2][c][i] - t1 - t2;
+            }
     };
     // the tile's outputs from LDS to the state, with the load mapping
     auto store_y = [&](uint64_t t) {
 #pragma unroll
-        for (int i = 0; i < S::LOADS; ++i) {
+        for (int i = 0; i < M::LOADS; ++i) {
             const int v = ROWS_FAST ? b_ld + BSTEP * i : b_ld;
             if (t * B + v < nbase) {
                 cx<double> y;
@@ -317,7 +224,11 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_dense_mfma(cx<R> *amps, uin
     const uint64_t G = gridDim.x;
     uint64_t t = blockIdx.x;
     if (t >= ntiles) return;                   // the whole workgroup: no barrier is skipped by part of it
-    if constexpr (M::PREFETCH) {
+    // the next tile's loads in registers while the products run, where they fit beside the
+    // fragments and accumulators at two workgroups per CU (K = 5 with lanes along the bases);
+    // otherwise the other workgroup on the CU covers this one's loads
+    constexpr bool kPrefetch = K == 4 || (K == 5 && !ROWS_FAST);
+    if constexpr (kPrefetch) {
         load_tile(t);
         for (; t < ntiles; t += G) {
             __syncthreads();                   // the previous tile's outputs have been read back
@@ -577,28 +488,17 @@ int launch_tile(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_ma
     int rc = ring_upload(s, host.data(), host.size(), &d);
     if (rc) return rc;
     const uint64_t nbase = s->len >> allpos.size();
-    const uint64_t ntiles = (nbase + TileShape<K>::B - 1) / TileShape<K>::B;
+    const uint64_t ntiles = (nbase + MfmaShape<K>::B - 1) / MfmaShape<K>::B;
     const bool rows_fast = std::find(tbits.begin(), tbits.end(), 0) != tbits.end();
     const cx<double> *dU = (const cx<double> *)((char *)d + sizeof(uint64_t) * D);
-    if constexpr (K >= 5) {
-        // FP64 MFMA: the waves hold their operator fragments in registers; two workgroups per CU
-        const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)device_cus() * 2);
-        if (rows_fast)
-            hipLaunchKernelGGL((k_dense_mfma<R, K, true>), dim3((uint32_t)grid), dim3(kTileThreads), 0, s->stream,
-                               (cx<R> *)s->amps, nbase, ctrl_mask, poslist(allpos), (const uint64_t *)d, dU, ntiles);
-        else
-            hipLaunchKernelGGL((k_dense_mfma<R, K, false>), dim3((uint32_t)grid), dim3(kTileThreads), 0, s->stream,
-                               (cx<R> *)s->amps, nbase, ctrl_mask, poslist(allpos), (const uint64_t *)d, dU, ntiles);
-    } else {
-        // vector FP64 with U in LDS: 69 KB, two workgroups per CU
-        const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)device_cus() * 2);
-        if (rows_fast)
-            hipLaunchKernelGGL((k_dense_tile<R, K, true>), dim3((uint32_t)grid), dim3(kTileThreads), 0, s->stream,
-                               (cx<R> *)s->amps, nbase, ctrl_mask, poslist(allpos), (const uint64_t *)d, dU, ntiles);
-        else
-            hipLaunchKernelGGL((k_dense_tile<R, K, false>), dim3((uint32_t)grid), dim3(kTileThreads), 0, s->stream,
-                               (cx<R> *)s->amps, nbase, ctrl_mask, poslist(allpos), (const uint64_t *)d, dU, ntiles);
-    }
+    // FP64 MFMA: the waves hold their operator fragments in registers; two workgroups per CU
+    const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)device_cus() * 2);
+    if (rows_fast)
+        hipLaunchKernelGGL((k_dense_mfma<R, K, true>), dim3((uint32_t)grid), dim3(kTileThreads), 0, s->stream,
+                           (cx<R> *)s->amps, nbase, ctrl_mask, poslist(allpos), (const uint64_t *)d, dU, ntiles);
+    else
+        hipLaunchKernelGGL((k_dense_mfma<R, K, false>), dim3((uint32_t)grid), dim3(kTileThreads), 0, s->stream,
+                           (cx<R> *)s->amps, nbase, ctrl_mask, poslist(allpos), (const uint64_t *)d, dU, ntiles);
     QIP_LAUNCHED();
     return QIPGPU_OK;
 }

@@ -1,6 +1,6 @@
 """Fused forms: gates fused into <= 6-qubit dense blocks. The precision form (FusedBackend) keeps a
 complex64 state and applies the blocks on split-BF16 MFMA; Fused128Backend keeps a complex128 state and
-applies the same blocks in FP64 (dense sweeps, FP64 MFMA tiles at 5 and 6 qubits).
+applies the same blocks in FP64 (dense sweeps, FP64 MFMA tiles at 4 to 6 qubits).
 
 ``FusedBackend`` is a StateType (qip/backend.py:14-65) like GpuBackend, but kronselect_dot only
 records: each engine call of the plan (targets, controls, matrix) becomes a dense gate on its
@@ -396,7 +396,7 @@ class FusedBackend(GpuBackend):
 
 class Fused128Backend(FusedBackend):
     """complex128 state with the same recording and fusion as FusedBackend, each block applied in FP64
-    by the dense sweeps (registers up to 3 qubits, LDS tiles at 4, FP64 MFMA tiles at 5 and 6) instead
+    by the dense sweeps (registers up to 3 qubits, FP64 MFMA tiles at 4 to 6) instead
     of the split-BF16 kernel: the plain path's arithmetic with one dense sweep per block instead of one
     per gate (the results differ from the gate-by-gate ones only by FP64 rounding). Runs of
     single-qubit-only blocks take the product sweep, as in the precision form. There is no

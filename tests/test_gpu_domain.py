@@ -105,11 +105,11 @@ def test_dense_tiles(k, n, controls, dtype):
     check({tuple(qs): m}, n, dtype)
 
 
-@pytest.mark.parametrize("k", [5, 6])
+@pytest.mark.parametrize("k", [4, 5, 6])
 @pytest.mark.parametrize("where", ["low", "high", "low_controlled"])
 @pytest.mark.parametrize("dtype", [numpy.complex128, numpy.complex64])
 def test_dense_tiles_both_layouts(k, where, dtype):
-    """5 and 6 targets (the FP64 MFMA tiles) with state bit 0 among the targets (lanes along the
+    """4 to 6 targets (the FP64 MFMA tiles) with state bit 0 among the targets (lanes along the
     operator's rows) and without it (lanes along the bases), over enough tiles that every workgroup
     takes several; once with a control on the highest qubit."""
     n = 15
