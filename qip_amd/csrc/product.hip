@@ -13,11 +13,13 @@
 // where b is one of the four bits that vary across those 16:
 //   layout A (as loaded and stored, address order): u = tid + 256 e, u bits 8..11 = e;
 //   layout B: u bits 4..7 = e;  layout C: u bits 0..3 = e.
-// The tile moves through LDS only as far as the factors need: factors on u bits 8..11 cost no LDS
-// traffic at all; A -> B -> A, A -> C -> A (two transposes) or A -> B -> C -> A (three) for the
-// others. One HBM read and one write per amplitude for up to 12 gates, and for a complex64 state one
-// rounding instead of one per gate. LDS rows of 16 amplitudes are padded by one slot, so that the
-// 16-amplitude-stride reads of layout C spread over the banks.
+// The tile is loaded in layout A and moves through LDS only as far as the factors need: factors on
+// u bits 8..11 cost no LDS traffic at all; A -> B (one transpose, stored from B), A -> C -> A or
+// A -> C -> B (two) for the others. Layout B stores whole runs of 16 amplitudes per 16 lanes, as A
+// does; C does not (its lanes run 16 amplitudes apart), so a pass never ends there. One HBM read and
+// one write per amplitude for up to 12 gates, and for a complex64 state one rounding instead of one
+// per gate. LDS rows of 16 amplitudes are padded by one slot, so that the 16-amplitude-stride reads
+// of layout C spread over the banks.
 //
 // A pass whose factors all sit on u bits 8..11 needs no LDS and runs a variant without the tile,
 // three workgroups per CU for a complex64 state. Passes that need transposes are latency-bound
@@ -137,11 +139,24 @@ __device__ __forceinline__ void load_raw(cx<R> (&v)[16], const cx<R> *amps, uint
     for (int i = 0; i < 16; ++i) v[i] = amps[base + u_offset<8, 12>(256u * i, a)];
 }
 
+template <typename R>
+__device__ __forceinline__ cx<R> narrow(cx<double> x) {
+    cx<R> o;
+    o.re = (R)x.re;
+    o.im = (R)x.im;
+    return o;
+}
+
 // One tile: its 16 amplitudes per thread (layout A, widened to FP64) in, the factors applied, rounded
-// and stored back at sbase (the tile's base plus this thread's offset with the X bits flipped).
+// and stored back. The tile visits only the layouts that hold factors, loaded in A and stored from the
+// last one: A; A -> B; A -> C -> B; A -> C -> A (no factor on u bits 4..7: C cannot be stored from,
+// since its lanes run 16 amplitudes apart). Layouts A and B both store whole runs of 16 amplitudes
+// per 16 lanes. The store addresses have the X bits flipped: off_sa / off_sb are this thread's part
+// of the tile offset in A / B with them flipped.
 template <typename R, bool kLds>
-__device__ __forceinline__ void product_tile(cx<double> (&x)[16], cx<R> *amps, uint64_t sbase, TileStore *lds,
-                                             const ProductArgs &a, bool use_b, bool use_c, uint32_t tid) {
+__device__ __forceinline__ void product_tile(cx<double> (&x)[16], cx<R> *amps, uint64_t tbase, uint64_t off_sa,
+                                             uint64_t off_sb, TileStore *lds, const ProductArgs &a, bool use_b,
+                                             bool use_c, uint32_t tid) {
     apply4<8>(x, a);
     if (kLds && (use_b || use_c)) {
         TileStore &t = *lds;
@@ -152,40 +167,39 @@ __device__ __forceinline__ void product_tile(cx<double> (&x)[16], cx<R> *amps, u
         const auto ta = t.at((int)(tid + (tid >> 4)));
         const auto tb = t.at((int)((tid & 15u) + 272u * (tid >> 4)));
         const auto tc = t.at((int)(17u * tid));
-        // Every phase reads and writes only this thread's own positions of that phase's layout, and
-        // a barrier separates phases: so the previous tile's last (A) reads need no barrier before
-        // these A writes, and each phase's writes go back where it read
+        // A phase reads slots other threads wrote in the phase before it, behind a barrier, and writes
+        // back only where it read. A tile that ends in A (its own slots) leaves nothing another thread
+        // still reads; one that ends in B does, so the next tile's A writes wait for a barrier.
+        if (use_b) __syncthreads();
 #pragma unroll
         for (int i = 0; i < 16; ++i) ta.put(272 * i, x[i]);
         __syncthreads();
-        if (use_b) {
-#pragma unroll
-            for (int e = 0; e < 16; ++e) x[e] = tb.get(17 * e);
-            apply4<4>(x, a);
-#pragma unroll
-            for (int e = 0; e < 16; ++e) tb.put(17 * e, x[e]);
-            if (use_c) __syncthreads();
-        }
-        if (use_c) {        // from B, or straight from A when no factor sits on u bits 4..7
+        if (use_c) {
 #pragma unroll
             for (int e = 0; e < 16; ++e) x[e] = tc.get(e);
             apply4<0>(x, a);
 #pragma unroll
             for (int e = 0; e < 16; ++e) tc.put(e, x[e]);
+            __syncthreads();
         }
-        __syncthreads();
+        if (use_b) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) x[e] = tb.get(17 * e);
+            apply4<4>(x, a);
+            // register e holds u bits 4..7 = e; it goes to those bits xor the X bits
+            const uint32_t xmid = (a.xmask >> 4) & 15u;
+#pragma unroll
+            for (int e = 0; e < 16; ++e)
+                amps[tbase + off_sb + u_offset<4, 8>(((uint32_t)e ^ xmid) << 4, a)] = narrow<R>(x[e]);
+            return;
+        }
 #pragma unroll
         for (int i = 0; i < 16; ++i) x[i] = ta.get(272 * i);
     }
-    // register i holds u = tid + 256 i; it goes to u xor xmask (sbase has the thread bits flipped)
+    // register i holds u = tid + 256 i; it goes to u xor xmask (off_sa has the thread bits flipped)
     const uint32_t xhi = (a.xmask >> 8) & 15u;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        cx<R> o;
-        o.re = (R)x[i].re;
-        o.im = (R)x[i].im;
-        amps[sbase + u_offset<8, 12>(256u * ((uint32_t)i ^ xhi), a)] = o;
-    }
+    for (int i = 0; i < 16; ++i) amps[tbase + off_sa + u_offset<8, 12>(256u * ((uint32_t)i ^ xhi), a)] = narrow<R>(x[i]);
 }
 
 // Persistent grid: each workgroup takes a contiguous range of tiles. A tile's raw amplitudes are
@@ -196,7 +210,9 @@ template <typename R, bool kLds>
 __device__ __forceinline__ void product_loop(cx<R> *amps, const ProductArgs &a, TileStore *t) {
     const uint32_t tid = threadIdx.x;
     const uint64_t off_t = u_offset<0, 8>(tid, a);       // layout A: u bits 0..7 are the thread index
-    const uint64_t off_s = u_offset<0, 8>(tid ^ (a.xmask & 0xffu), a);
+    const uint64_t off_sa = u_offset<0, 8>(tid ^ (a.xmask & 0xffu), a);
+    // layout B: u bits 0..3 and 8..11 are the thread index
+    const uint64_t off_sb = u_offset<0, 12>(((tid & 15u) | (tid >> 4) << 8) ^ (a.xmask & 0xf0fu), a);
     const bool use_b = (a.fmask & 0xf0u) != 0, use_c = (a.fmask & 0x0fu) != 0;
     const uint64_t per = (a.ntiles + gridDim.x - 1) / gridDim.x;
     uint64_t tile = (uint64_t)blockIdx.x * per;
@@ -214,7 +230,7 @@ __device__ __forceinline__ void product_loop(cx<R> *amps, const ProductArgs &a, 
         }
         if (!kLds && a.xsync) __syncthreads();   // every wave holds the tile before any wave stores
         load_raw(v, amps, tile_base(min(tile + 1, last), a) + off_t, a);
-        product_tile<R, kLds>(x, amps, tile_base(tile, a) + off_s, t, a, use_b, use_c, tid);
+        product_tile<R, kLds>(x, amps, tile_base(tile, a), off_sa, off_sb, t, a, use_b, use_c, tid);
         if (++tile >= end) return;
     }
 }

@@ -80,9 +80,9 @@ def test_product_real_and_complex_factors(n, k, dtype, tol):
 @pytest.mark.parametrize("n,low", [(16, [0, 1, 2, 3]), (20, [1, 2, 3]), (18, [])])
 def test_complex64_product_rounds_once(n, low):
     """On a complex64 state the factors are applied in FP64 and the tile crosses LDS between register
-    layouts as an FP32 high and a BF16 low part: every component must come out within one FP32 ulp of
-    the FP64 product of the same complex64 input (H, X and complex factors on the tile bits of all
-    three layouts, i.e. on the four lowest state bits and eight high ones)."""
+    layouts in FP64: every component must come out within one FP32 ulp of the FP64 product of the
+    same complex64 input (H, X and complex factors on the tile bits of all three layouts, i.e. on the
+    four lowest state bits and eight high ones)."""
     psi = (RNG.standard_normal(2 ** n) + 1j * RNG.standard_normal(2 ** n)).astype(numpy.complex64)
     psi /= numpy.float32(numpy.linalg.norm(psi))
     bits = low + list(range(n - 8, n))
@@ -126,7 +126,10 @@ def test_x_on_every_qubit_is_the_index_reversal(n, dtype):
     ("", "cccc"),             # the register bits only: no LDS
     ("chh", ""),              # the low bits only: straight to layout C and back
     ("hc", "cccc"),           # low bits and register bits: A -> C -> A
-    ("h", "hhhhx"),           # every layout
+    ("h", "hhhhx"),           # low bits and register bits, X on a lane bit: A -> C -> A
+    ("", "hhhhhhhh"),         # register bits and u bits 4..7: A -> B, stored from B
+    ("x", "hhhhhhxx"),        # A -> B with X flipped on a low bit and on u bits 4, 5 at B's store
+    ("xc", "hhhhhx"),         # every layout: A -> C -> B, X on a low bit and on u bit 4
 ])
 def test_product_tile_bit_placement(low, high, dtype, tol):
     """Factors on the four lowest state bits and on higher bits, of each kind (H real, X, complex),
