@@ -232,6 +232,17 @@ def product_passes(n: int, targets: Sequence[int]):
     return [c for c in chunks if c]
 
 
+def pack_matrix(targets: Sequence[int], mat: numpy.ndarray, controls: Sequence[int]) -> tuple:
+    """qipgpu_apply_matrix's arguments after the state handle: (k, targets, matrix, number of
+    controls, controls), then the array the matrix argument points into (kept alive with them)."""
+    k = len(targets)
+    if mat.shape != (2 ** k, 2 ** k):
+        raise ValueError("matrix shape {} does not match {} targets".format(mat.shape, k))
+    m, keep = _capi.c128_arg(mat)
+    return (k, _capi.i32_bytes(targets), m, len(controls), _capi.i32_bytes(controls) if len(controls) else None,
+            keep)
+
+
 class GpuBackend(StateType):
     """Plain MI355X backend: one in-place HIP sweep per connected component of each operator,
     complex128 (the drop-in) or complex64 state, FP64 reductions."""
@@ -350,14 +361,22 @@ class GpuBackend(StateType):
     # -- operators ------------------------------------------------------------------------------
     def kronselect_dot(self, mats: Mapping[IndexType, MatrixType],
                        input_offset: int = 0, output_offset: int = 0) -> None:
-        check_windows(self.n, 2 ** self.n, input_offset, output_offset)
+        if input_offset or output_offset:
+            check_windows(self.n, 2 ** self.n, input_offset, output_offset)
         for call in planner.plan(mats):
-            if isinstance(call, planner.ProductCall):
-                self.apply_product(call.targets, call.mats)
-            elif isinstance(call, planner.GeneralCall):
-                self.apply_general(call)
+            if isinstance(call, planner.MatrixCall):
+                # the call's arguments are packed once and kept with it (a planned Descriptor's calls
+                # serve every run of its gate node)
+                p = call.packed
+                if p is None:
+                    p = call.packed = pack_matrix(call.targets, call.mat, call.controls)
+                rc = self._lib.qipgpu_apply_matrix(self._h, p[0], p[1], p[2], p[3], p[4])
+                if rc:
+                    _capi.check(rc, "apply_matrix")
+            elif isinstance(call, planner.ProductCall):
+                self._apply_product_call(call)
             else:
-                self.apply_matrix(call.targets, call.mat, call.controls)
+                self.apply_general(call)
 
     def apply_general(self, call: "planner.GeneralCall") -> None:
         """A descriptor component by the reference's column-sum formula (qipgpu_apply_kron)."""
@@ -370,24 +389,30 @@ class GpuBackend(StateType):
     def apply_product(self, targets: Sequence[int], mats: Sequence[numpy.ndarray]) -> None:
         """Single-qubit operators mats[j] on distinct qubits targets[j], up to 12 per sweep
         (n >= 12; smaller states take Kronecker blocks of 3 through apply_matrix)."""
+        self._apply_product_call(planner.ProductCall(targets, mats))
+
+    def _apply_product_call(self, call: "planner.ProductCall") -> None:
         if self.n < PRODUCT_MIN_QUBITS:
-            for call in planner.ProductCall(targets, mats).matrix_calls():
-                self.apply_matrix(call.targets, call.mat)
+            for c in call.matrix_calls():
+                self.apply_matrix(c.targets, c.mat)
             return
-        for chunk in product_passes(self.n, targets):
-            m = b"".join(numpy.asarray(mats[j], dtype=numpy.complex128).reshape(2, 2).tobytes() for j in chunk)
-            rc = self._lib.qipgpu_apply_product(self._h, len(chunk), _capi.i32_bytes([targets[j] for j in chunk]), m)
+        # the passes depend on n: packed per state size and kept with the call
+        if call.packed is None or call.packed[0] != self.n:
+            passes = []
+            for chunk in product_passes(self.n, call.targets):
+                m = b"".join(numpy.asarray(call.mats[j], dtype=numpy.complex128).reshape(2, 2).tobytes()
+                             for j in chunk)
+                passes.append((len(chunk), _capi.i32_bytes([call.targets[j] for j in chunk]), m))
+            call.packed = (self.n, passes)
+        for k, t, m in call.packed[1]:
+            rc = self._lib.qipgpu_apply_product(self._h, k, t, m)
             if rc:
                 _capi.check(rc, "apply_product")
 
     def apply_matrix(self, targets: Sequence[int], mat: numpy.ndarray, controls: Sequence[int] = ()) -> None:
         """One engine call: `mat` on `targets` (first = MSB) where all `controls` are 1."""
-        k = len(targets)
-        if mat.shape != (2 ** k, 2 ** k):
-            raise ValueError("matrix shape {} does not match {} targets".format(mat.shape, k))
-        m, keep = _capi.c128_arg(mat)
-        rc = self._lib.qipgpu_apply_matrix(self._h, k, _capi.i32_bytes(targets), m, len(controls),
-                                           _capi.i32_bytes(controls) if len(controls) else None)
+        p = pack_matrix(targets, mat, controls)
+        rc = self._lib.qipgpu_apply_matrix(self._h, p[0], p[1], p[2], p[3], p[4])
         if rc:
             _capi.check(rc, "apply_matrix")
 

@@ -14,6 +14,7 @@ from typing import Any, Callable, Dict, Mapping, Sequence, Tuple, Union
 
 import numpy
 
+from . import planner
 from .backend import StateType
 from .qip import OpConstructor, Qubit
 from .qubit_util import QubitOpWrapper
@@ -26,7 +27,8 @@ class MatrixOp(Qubit):
     """A node applying an operator descriptor made by `makemats(index_groups)`. The descriptor is
     made on first use and kept in `ms`, and made again only if the node's index groups change
     between runs (the reference keeps the first one, qip/operators.py:15-26). dagger=True applies
-    the conjugate transpose of every matrix."""
+    the conjugate transpose of every matrix. The descriptor is a planner.Descriptor with read-only
+    numpy matrices, so the engine calls planned on its first use serve the later runs."""
 
     def __init__(self, *inputs: Qubit, **kwargs):
         super().__init__(*inputs, **kwargs)
@@ -40,6 +42,9 @@ class MatrixOp(Qubit):
             mats = self.makemats(groups)
             if self.dagger:
                 mats = {k: v.conj().T for k, v in mats.items()}
+            mats = planner.Descriptor(mats)
+            for m in mats.values():
+                planner.freeze(m)
             self.ms, self._ms_groups = mats, groups
         return self.ms
 

@@ -41,13 +41,14 @@ _SWAP4 = numpy.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], d
 
 class MatrixCall:
     """One engine call: dense `mat` (2^k x 2^k, first target = MSB) on `targets`, applied where
-    every `controls` qubit is 1."""
-    __slots__ = ("targets", "controls", "mat")
+    every `controls` qubit is 1. `packed` holds a backend's ready-made call arguments."""
+    __slots__ = ("targets", "controls", "mat", "packed")
 
     def __init__(self, targets: Sequence[int], controls: Sequence[int], mat: numpy.ndarray):
         self.targets = list(targets)
         self.controls = list(controls)
         self.mat = mat
+        self.packed = None
 
     def __repr__(self):
         return "MatrixCall(t={}, c={})".format(self.targets, self.controls)
@@ -258,8 +259,81 @@ def _plan_component(groups, controls: List[int]) -> list:
     return [GeneralCall(_wrap_controls(controls, groups))]
 
 
+class Descriptor(dict):
+    """A descriptor that keeps its plan: the graph frontend's gate nodes hand these to the backend
+    and do not change them once made, and their numpy matrices are read-only, so the calls planned
+    for the first run serve every later one. A change through the dict's own methods drops the
+    kept plan; a descriptor holding a matrix that is not read-only (a list, a writeable array) is
+    planned on every use."""
+    __slots__ = ("_plan",)
+
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+        self._plan = None
+
+    def __setitem__(self, key, value):
+        self._plan = None
+        super().__setitem__(key, value)
+
+    def __delitem__(self, key):
+        self._plan = None
+        super().__delitem__(key)
+
+    def __ior__(self, other):
+        self._plan = None
+        return super().__ior__(other)
+
+    def clear(self):
+        self._plan = None
+        super().clear()
+
+    def pop(self, *args):
+        self._plan = None
+        return super().pop(*args)
+
+    def popitem(self):
+        self._plan = None
+        return super().popitem()
+
+    def setdefault(self, *args):
+        self._plan = None
+        return super().setdefault(*args)
+
+    def update(self, *args, **kwargs):
+        self._plan = None
+        super().update(*args, **kwargs)
+
+
+def freeze(m) -> None:
+    """Make the numpy matrix inside a descriptor entry (through C_MAT levels) read-only."""
+    while getattr(m, "_kron_struct", None) == KRON_CMAT:
+        m = m.m
+    if type(m) == numpy.ndarray:
+        m.flags.writeable = False
+
+
+def _frozen(m) -> bool:
+    while getattr(m, "_kron_struct", None) == KRON_CMAT:
+        m = m.m
+    if getattr(m, "_kron_struct", None) == KRON_SWAP:
+        return True
+    return type(m) == numpy.ndarray and not m.flags.writeable
+
+
 def plan(mats: Mapping) -> list:
-    """Engine calls equivalent to one kronselect_dot(mats): MatrixCall, ProductCall and GeneralCall."""
+    """Engine calls equivalent to one kronselect_dot(mats): MatrixCall, ProductCall and GeneralCall.
+    A Descriptor whose matrices are all read-only keeps its calls for the next use."""
+    if type(mats) is Descriptor:
+        if mats._plan is None:
+            calls = _plan(mats)
+            if all(_frozen(m) for m in mats.values()):
+                mats._plan = tuple(calls)
+            return calls
+        return list(mats._plan)
+    return _plan(mats)
+
+
+def _plan(mats: Mapping) -> list:
     newmats = normalise(mats)
     items = []
     for key, m in newmats.items():
@@ -278,13 +352,15 @@ def plan(mats: Mapping) -> list:
 
 class ProductCall:
     """The uncontrolled single-qubit factors of one descriptor (distinct qubits, so they commute),
-    applied together: mats[j] (2x2) on targets[j]."""
-    __slots__ = ("targets", "mats")
+    applied together: mats[j] (2x2) on targets[j]. `packed` holds a backend's ready-made call
+    arguments."""
+    __slots__ = ("targets", "mats", "packed")
     controls = ()
 
     def __init__(self, targets: Sequence[int], mats: Sequence[numpy.ndarray]):
         self.targets = list(targets)
         self.mats = list(mats)
+        self.packed = None
 
     def matrix_calls(self, width: int = KRON_MERGE_WIDTH) -> List[MatrixCall]:
         """The same product as Kronecker blocks on up to `width` qubits each."""

@@ -161,3 +161,47 @@ def test_random_descriptors(trial):
     if not mats:
         mats = {0: H}
     check(mats, n)
+
+
+def test_descriptor_keeps_its_plan_while_unchanged():
+    """A Descriptor with read-only matrices is planned once; the kept calls are the fresh plan's;
+    a change through the dict drops them; a writeable or list matrix is planned on every use."""
+    m1, m2 = rand_u(4), rand_u(2)
+    d = planner.Descriptor({(0, 1, 2): CMat(m1), 3: m2, (4, 5): SwapMat(1)})
+    assert planner.plan(d) is not planner.plan(d)           # writeable matrices: nothing kept
+    assert d._plan is None
+    for m in d.values():
+        planner.freeze(m)
+    assert not m1.flags.writeable and not m2.flags.writeable
+    first = planner.plan(d)
+    kept = d._plan
+    assert kept is not None
+    again = planner.plan(d)
+    assert again is not first and list(kept) == again        # a new list of the same calls
+    assert [repr(c) for c in again] == [repr(c) for c in planner._plan(dict(d))]
+    with pytest.raises(ValueError):
+        m2[0, 0] = 0.0                                        # a read-only matrix cannot change under it
+    for change in (lambda: d.__setitem__(6, H.copy()), lambda: d.pop(6), lambda: d.update({7: X}),
+                   lambda: d.setdefault(8, X), lambda: d.popitem(), lambda: d.__delitem__(3)):
+        planner.plan(d)
+        change()
+        assert d._plan is None
+    d2 = planner.Descriptor({0: [[0.0, 1.0], [1.0, 0.0]]})  # a list matrix
+    planner.plan(d2)
+    assert d2._plan is None
+
+
+def test_graph_gate_descriptors_are_frozen():
+    """The frontend's gate nodes hand out Descriptors with read-only matrices, remade when the
+    node's index groups change."""
+    from qip_amd import C, H as HOp, Qubit
+    from qip_amd.operators import MatrixOp
+    op = C(HOp)(Qubit(n=1), Qubit(n=2))[1]
+    node = op if isinstance(op, MatrixOp) else op.inputs[0]
+    assert isinstance(node, MatrixOp)
+    d = node.descriptor([[0], [1, 2]])
+    assert type(d) is planner.Descriptor
+    assert planner.plan(d) and d._plan is not None
+    assert node.descriptor([[0], [1, 2]]) is d
+    d3 = node.descriptor([[3], [4, 5]])
+    assert d3 is not d and d3._plan is None
