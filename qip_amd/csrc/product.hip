@@ -23,9 +23,9 @@
 //
 // A pass whose factors all sit on u bits 8..11 needs no LDS and runs a variant without the tile,
 // three workgroups per CU for a complex64 state. Passes that need transposes are latency-bound
-// rather than bandwidth-bound: the 68 KB FP64 tile and 176 VGPRs hold the LDS variant to 8 waves
+// rather than bandwidth-bound: the 68 KB FP64 tile and 178 VGPRs hold the LDS variant to 8 waves
 // per CU, which do not cover the barrier-separated phases (on a 24-qubit complex64 pass over all
-// three layouts, LDS is busy about a fifth of the time and the FP64 units about a third).
+// three layouts, LDS is busy about an eighth of the time and the FP64 units about a third).
 #include <algorithm>
 #include <cstring>
 #include <vector>
