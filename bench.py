@@ -657,10 +657,10 @@ def bench_func28(args):
 def bench_dense28(args):
     """Dense operators on 4..6 targets (k_dense_mfma) on a 28-qubit complex128 state (4 GiB): the
     targets on the lowest, the highest and scattered state bits, and one 6-target operator under a
-    control; then 8 targets (the FP64 GEMM, k_dense_gemm + scatter) on 26 qubits. Per call: HIP-event
+    control; then 8 targets (the FP64 MFMA GEMM, k_dense_gemm + scatter) on 26 qubits. Per call: HIP-event
     time on the state's stream, algorithmic bytes 2 S 2^(n-c), FP64 flops 8 2^K 2^(n-c) (a complex
-    matrix-vector product counted the usual way); the MFMA tiles execute 6 2^K 2^(n-c) of them, with
-    three real products per complex one."""
+    matrix-vector product counted the usual way); the MFMA tiles and the GEMM execute 6 2^K 2^(n-c)
+    of them, with three real products per complex one."""
     import numpy
     from qip_amd import CMat, GpuBackend
     ev = HipEvents()
@@ -689,7 +689,7 @@ def bench_dense28(args):
         ms = ev.elapsed_ms(a, b) / reps
         nb = 2 * 16 * 2 ** (n - controls)
         fl = 8 * 2 ** K * 2 ** (n - controls)
-        ex = (6 if K <= 6 else 8) * 2 ** K * 2 ** (n - controls)
+        ex = 6 * 2 ** K * 2 ** (n - controls)
         cases[name] = {"n": n, "qubits": list(qubits), "controls": controls, "targets": K, "ms": ms,
                        "algorithmic_gbs": nb / ms / 1e6, "hbm_frac": nb / ms / 1e6 / HBM_PEAK_GBS,
                        "fp64_tflops": fl / ms / 1e9, "fp64_frac": fl / ms / 1e9 / FP64_PEAK_TFLOPS,

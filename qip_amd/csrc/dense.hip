@@ -258,64 +258,83 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_dense_mfma(cx<R> *amps, uin
 }
 
 // ---- K = 7..12: tiled GEMM out of place ---------------------------------------------------------
+//
+// A workgroup computes a 64-row x 64-base block of Y = U X into a compact [D][nbase] buffer, over
+// the D input rows in steps of 16 through LDS (Re / Im planes of the U block and of the X block,
+// padded by one slot per row). On FP64 MFMA with the same three real products per complex one as
+// the tiles above: wave w owns output rows 16 w .. 16 w + 15 and all four 16-base column blocks,
+// 12 accumulators; per step it reads its A fragments (Re U, Im U; Re U + Im U formed in registers)
+// and the B fragments of X from LDS. Bases past nbase read the last base and are not stored.
 
 constexpr int kGemmTile = 64;
 constexpr int kGemmStep = 16;
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_dense_gemm(const cx<R> *__restrict__ amps, cx<R> *__restrict__ out,
-                                                    uint64_t nbase, int D, uint64_t ctrl_mask, PosList pl,
-                                                    const uint64_t *__restrict__ off,
-                                                    const cx<double> *__restrict__ U) {
-    __shared__ cx<double> sU[kGemmTile][kGemmStep + 1];
-    __shared__ cx<double> sX[kGemmStep][kGemmTile];
+__global__ __launch_bounds__(256, 2) void k_dense_gemm(const cx<R> *__restrict__ amps, cx<R> *__restrict__ out,
+                                                       uint64_t nbase, int D, uint64_t ctrl_mask, PosList pl,
+                                                       const uint64_t *__restrict__ off,
+                                                       const cx<double> *__restrict__ U) {
+    constexpr int US = kGemmStep + 1;          // padded U row (doubles)
+    constexpr int XS = kGemmTile + 1;          // padded X row
+    __shared__ double sU[2][kGemmTile * US];
+    __shared__ double sX[2][kGemmStep * XS];
     const int tid = threadIdx.x;
-    const uint64_t b0 = (uint64_t)blockIdx.x * kGemmTile;   // first base of the tile
-    const int i0 = blockIdx.y * kGemmTile;                    // first row of the tile
-    // loader roles: X column b_ld with steps jj = tid / 64 + 4 s; U row u_row, columns u_col + 4 s
+    const int lane = tid & 63, wave = tid >> 6;
+    const int rho = lane & 15, kap = lane >> 4;
+    const uint64_t b0 = (uint64_t)blockIdx.x * kGemmTile;   // first base of the block
+    const int i0 = blockIdx.y * kGemmTile;                    // first output row of the block
+    // loader roles: X row jj0 + 4 s of base b_ld; U row u_row, columns u_col + 4 s
     const int b_ld = tid % kGemmTile;
     const int jj0 = tid / kGemmTile;
-    const bool have_base = b0 + b_ld < nbase;
-    const uint64_t base = have_base ? base_of(b0 + b_ld, pl, ctrl_mask) : 0;
+    const uint64_t gb = b0 + b_ld;
+    const uint64_t base = base_of(gb < nbase ? gb : nbase - 1, pl, ctrl_mask);
     const int u_row = tid / 4, u_col = tid % 4;
-    // compute roles: rows ty * 4 + r, bases tx + 16 q
-    const int ty = tid / 16, tx = tid % 16;
-    cx<double> acc[4][4];
+    f64x4 acc[3][4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int m = 0; m < 3; ++m)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) acc[r][q].re = acc[r][q].im = 0.0;
+        for (int c = 0; c < 4; ++c) acc[m][c] = f64x4{0.0, 0.0, 0.0, 0.0};
     for (int jc = 0; jc < D; jc += kGemmStep) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int jj = jj0 + 4 * s;
-            cx<double> v;
-            v.re = v.im = 0.0;
-            if (have_base) v = to_d(amps[base + off[jc + jj]]);
-            sX[jj][b_ld] = v;
-            sU[u_row][u_col + 4 * s] = U[(size_t)(i0 + u_row) * D + jc + u_col + 4 * s];
+            const cx<double> v = to_d(amps[base + off[jc + jj]]);
+            sX[0][jj * XS + b_ld] = v.re;
+            sX[1][jj * XS + b_ld] = v.im;
+            const cx<double> u = U[(size_t)(i0 + u_row) * D + jc + u_col + 4 * s];
+            sU[0][u_row * US + u_col + 4 * s] = u.re;
+            sU[1][u_row * US + u_col + 4 * s] = u.im;
         }
         __syncthreads();
 #pragma unroll
-        for (int jj = 0; jj < kGemmStep; ++jj) {
-            cx<double> u[4], x[4];
+        for (int s = 0; s < kGemmStep / 4; ++s) {
+            const int ua = (16 * wave + rho) * US + 4 * s + kap;
+            const double a0 = sU[0][ua], a1 = sU[1][ua];
+            const double a2 = a0 + a1;
+            const int xa = (4 * s + kap) * XS + rho;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) u[r] = sU[ty * 4 + r][jj];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) x[q] = sX[jj][tx + 16 * q];
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) mac(acc[r][q], u[r], x[q]);
+            for (int c = 0; c < 4; ++c) {
+                const double p = sX[0][xa + 16 * c], q = sX[1][xa + 16 * c];
+                acc[0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, p, acc[0][c], 0, 0, 0);
+                acc[1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, q, acc[1][c], 0, 0, 0);
+                acc[2][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, p + q, acc[2][c], 0, 0, 0);
+            }
         }
         __syncthreads();
     }
+    // item i of column block c: output row i0 + 16 wave + 4 i + kap, base b0 + 16 c + rho
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const uint64_t b = b0 + tx + 16 * q;
+    for (int c = 0; c < 4; ++c) {
+        const uint64_t b = b0 + 16 * c + rho;
         if (b < nbase) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) out[(size_t)(i0 + ty * 4 + r) * nbase + b] = from_d<R>(acc[r][q]);
+            for (int i = 0; i < 4; ++i) {
+                const double t1 = acc[0][c][i], t2 = acc[1][c][i];
+                cx<double> y;
+                y.re = t1 - t2;
+                y.im = acc[2][c][i] - t1 - t2;
+                out[(size_t)(i0 + 16 * wave + 4 * i + kap) * nbase + b] = from_d<R>(y);
+            }
         }
     }
 }
@@ -464,18 +483,6 @@ PosList poslist(std::vector<int> pos) {
     return pl;
 }
 
-// Upload [off | U] (U as interleaved FP64) to a stream-ordered buffer; the host data is consumed
-// before return.
-int upload_operator(qipgpu_state *s, StreamBuffer &buf, const std::vector<uint64_t> &off,
-                    const std::complex<double> *U, size_t D) {
-    const size_t ob = off.size() * sizeof(uint64_t), ub = D * D * sizeof(cx<double>);
-    QIP_HIP(buf.alloc(ob + ub));
-    QIP_HIP(hipMemcpyAsync(buf.p, off.data(), ob, hipMemcpyHostToDevice, s->stream));
-    QIP_HIP(hipMemcpyAsync((char *)buf.p + ob, U, ub, hipMemcpyHostToDevice, s->stream));
-    QIP_HIP(hipStreamSynchronize(s->stream));   // the host copies may be pageable
-    return QIPGPU_OK;
-}
-
 template <typename R, int K>
 int launch_tile(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mask, const std::vector<int> &allpos,
                 const std::complex<double> *U) {
@@ -503,28 +510,80 @@ int launch_tile(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_ma
     return QIPGPU_OK;
 }
 
+// Temporaries of wide dense operators up to this size stay with the state for the next call.
+constexpr size_t kKeepGemmTmpBytes = size_t(1) << 30;
+
+// The offsets and U of a wide dense operator into the state's operator buffer, on its queue: through
+// pinned memory, so the copy is asynchronous and the queue is not drained first. The pinned buffer is
+// rewritten only once the previous copy out of it has run; the device buffer is rewritten in queue
+// order, after the kernels that read it.
+int stage_gemm_operator(qipgpu_state *s, const std::vector<uint64_t> &off, const std::complex<double> *U, size_t D,
+                        void **dev) {
+    const size_t ob = off.size() * sizeof(uint64_t), ub = D * D * sizeof(cx<double>), need = ob + ub;
+    if (!s->gemm_ev) QIP_HIP(hipEventCreateWithFlags(&s->gemm_ev, hipEventDisableTiming));
+    else QIP_HIP(hipEventSynchronize(s->gemm_ev));
+    if (s->h_gemm_op_bytes < need) {
+        if (s->h_gemm_op) QIP_HIP(hipHostFree(s->h_gemm_op));
+        s->h_gemm_op = nullptr;
+        s->h_gemm_op_bytes = 0;
+        QIP_HIP(hipHostMalloc(&s->h_gemm_op, need, hipHostMallocDefault));
+        s->h_gemm_op_bytes = need;
+    }
+    if (s->d_gemm_op_bytes < need) {
+        QIP_HIP(hipStreamSynchronize(s->stream));   // queued kernels may still read the old buffer
+        if (s->d_gemm_op) QIP_HIP(hipFree(s->d_gemm_op));
+        s->d_gemm_op = nullptr;
+        s->d_gemm_op_bytes = 0;
+        QIP_HIP(hipMalloc(&s->d_gemm_op, need));
+        s->d_gemm_op_bytes = need;
+    }
+    std::memcpy(s->h_gemm_op, off.data(), ob);
+    std::memcpy((char *)s->h_gemm_op + ob, U, ub);
+    QIP_HIP(hipMemcpyAsync(s->d_gemm_op, s->h_gemm_op, need, hipMemcpyHostToDevice, s->stream));
+    QIP_HIP(hipEventRecord(s->gemm_ev, s->stream));
+    *dev = s->d_gemm_op;
+    return QIPGPU_OK;
+}
+
 template <typename R>
 int launch_gemm(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mask, const std::vector<int> &allpos,
                 const std::complex<double> *U) {
     const int K = (int)tbits.size();
     const int D = 1 << K;
     const std::vector<uint64_t> off = offsets(tbits);
-    StreamBuffer op(s->stream), tmp(s->stream);
-    int rc = upload_operator(s, op, off, U, (size_t)D);
+    void *op = nullptr;
+    int rc = stage_gemm_operator(s, off, U, (size_t)D, &op);
     if (rc) return rc;
     const uint64_t nbase = s->len >> allpos.size();
-    QIP_HIP(tmp.alloc(nbase * (size_t)D * sizeof(cx<R>)));
+    // the compact temporary: the state's kept one when small, else one for this call
+    const size_t tb = nbase * (size_t)D * sizeof(cx<R>);
+    StreamBuffer big(s->stream);
+    void *tmp = nullptr;
+    if (tb <= kKeepGemmTmpBytes) {
+        if (s->gemm_tmp_bytes < tb) {
+            QIP_HIP(hipStreamSynchronize(s->stream));   // queued kernels may still use the old one
+            if (s->gemm_tmp) QIP_HIP(hipFree(s->gemm_tmp));
+            s->gemm_tmp = nullptr;
+            s->gemm_tmp_bytes = 0;
+            QIP_HIP(hipMalloc(&s->gemm_tmp, tb));
+            s->gemm_tmp_bytes = tb;
+        }
+        tmp = s->gemm_tmp;
+    } else {
+        QIP_HIP(big.alloc(tb));
+        tmp = big.p;
+    }
     const PosList pl = poslist(allpos);
-    const uint64_t *doff = (const uint64_t *)op.p;
-    const cx<double> *dU = (const cx<double> *)((char *)op.p + off.size() * sizeof(uint64_t));
+    const uint64_t *doff = (const uint64_t *)op;
+    const cx<double> *dU = (const cx<double> *)((char *)op + off.size() * sizeof(uint64_t));
     const dim3 grid((uint32_t)((nbase + kGemmTile - 1) / kGemmTile), (uint32_t)(D / kGemmTile));
-    hipLaunchKernelGGL((k_dense_gemm<R>), grid, dim3(256), 0, s->stream, (const cx<R> *)s->amps, (cx<R> *)tmp.p,
+    hipLaunchKernelGGL((k_dense_gemm<R>), grid, dim3(256), 0, s->stream, (const cx<R> *)s->amps, (cx<R> *)tmp,
                        nbase, D, ctrl_mask, pl, doff, dU);
     QIP_LAUNCHED();
     const uint64_t total = nbase * (uint64_t)D;
     const uint32_t sb = (uint32_t)std::min<uint64_t>((total + 255) / 256, (uint64_t)device_cus() * 64);
     hipLaunchKernelGGL((k_dense_scatter<R>), dim3(sb), dim3(256), 0, s->stream, (cx<R> *)s->amps,
-                       (const cx<R> *)tmp.p, nbase, D, ctrl_mask, pl, doff);
+                       (const cx<R> *)tmp, nbase, D, ctrl_mask, pl, doff);
     QIP_LAUNCHED();
     return QIPGPU_OK;
 }

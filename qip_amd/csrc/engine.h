@@ -87,6 +87,16 @@ struct qipgpu_state {
     // fused path operator staging (split BF16 planes), device
     void *d_fmat = nullptr;
     size_t d_fmat_bytes = 0;
+    // dense operators on 7..12 targets (dense.hip launch_gemm): the operator and its offsets pass
+    // through pinned host memory to a device buffer on the state's queue (gemm_ev marks the last
+    // copy out of the pinned buffer), and the compact temporary is kept while it is small
+    void *h_gemm_op = nullptr;   // pinned
+    size_t h_gemm_op_bytes = 0;
+    void *d_gemm_op = nullptr;
+    size_t d_gemm_op_bytes = 0;
+    hipEvent_t gemm_ev = nullptr;
+    void *gemm_tmp = nullptr;
+    size_t gemm_tmp_bytes = 0;
     // gates recorded for online fusion and not launched yet (complex64 states)
     qipgpu::Recorder *recorder = nullptr;
 };

@@ -379,6 +379,10 @@ int qipgpu_destroy(qipgpu_state *s) {
         if (s->ring_ev[i]) (void)hipEventDestroy(s->ring_ev[i]);
     if (s->d_fused) (void)hipFree(s->d_fused);
     if (s->d_fmat) (void)hipFree(s->d_fmat);
+    if (s->h_gemm_op) (void)hipHostFree(s->h_gemm_op);
+    if (s->d_gemm_op) (void)hipFree(s->d_gemm_op);
+    if (s->gemm_ev) (void)hipEventDestroy(s->gemm_ev);
+    if (s->gemm_tmp) (void)hipFree(s->gemm_tmp);
     recorder_free(s);
     (void)hipStreamDestroy(s->stream);
     delete s;

@@ -138,6 +138,30 @@ def test_dense_gemm(k, n, controls, dtype):
     check({tuple(qs): m}, n, dtype)
 
 
+@pytest.mark.parametrize("dtype", [numpy.complex128, numpy.complex64])
+def test_dense_gemm_calls_in_a_row(dtype):
+    """Several wide operators on one state: the operator staging and the kept temporary are reused,
+    grown (8 -> 10 targets) and reused smaller, each call against the oracle applied in turn."""
+    n = 13
+    psi = RNG.standard_normal(2 ** n) + 1j * RNG.standard_normal(2 ** n)
+    psi /= numpy.linalg.norm(psi)
+    be = GpuBackend.make_state(n, [list(range(n))], [psi], statetype=dtype)
+    ref = be.get_state().astype(numpy.complex128)
+    tol = 1e-12 if dtype == numpy.complex128 else 2e-6
+    for k, controls in ((8, 0), (7, 0), (10, 0), (8, 1), (9, 0)):
+        qs = [int(q) for q in RNG.permutation(n)[:k + controls]]
+        print("gemm in a row: k", k, "controls", controls, "qubits", qs, flush=True)
+        m = rand_u(2 ** k)
+        for _ in range(controls):
+            m = CMat(m)
+        be.kronselect_dot({tuple(qs): m})
+        out = numpy.zeros_like(ref)
+        oracle.kronselect_dot({tuple(qs): m}, ref, n, out)
+        ref = out
+        assert numpy.max(numpy.abs(be.get_state() - ref)) < tol
+    be.close()
+
+
 def test_dense_eight_qubit_matrix_op():
     """A user MatrixOp on 8 qubits (a random unitary), through the graph frontend."""
     U = rand_u(256)
