@@ -8,7 +8,7 @@
 // of base b. FP64 arithmetic for both state precisions, rounded once on store.
 //
 //   k_dense_mfma<R, K>   K = 4..6, in place. A persistent workgroup takes tiles of B bases: it stages
-//                        X (D x B = 4096 amplitudes, 64 KB in FP64) in LDS as Re / Im planes, forms
+//                        X (D x B = 2048 amplitudes, 32 KB in FP64) in LDS as Re / Im planes, forms
 //                        Y = U X with three real products on FP64 MFMA (v_mfma_f64_16x16x4f64), each
 //                        wave holding its share of Re U and Im U in registers for the launch, and
 //                        writes Y back through the same planes. A tile holds every amplitude of its
@@ -60,7 +60,6 @@ __device__ __forceinline__ uint64_t base_of(uint64_t g, const PosList &pl, uint6
 }
 
 constexpr int kTileThreads = 256;
-constexpr int kTileElems = 4096;        // D x B amplitudes per tile
 
 // ---- K = 4..6: tiles in place on FP64 MFMA -------------------------------------------------------
 //
@@ -84,8 +83,13 @@ constexpr int kTileElems = 4096;        // D x B amplitudes per tile
 template <int K>
 struct MfmaShape {
     static constexpr int D = 1 << K;
-    static constexpr int B = kTileElems / D;     // bases per tile
-    static constexpr int LOADS = kTileElems / kTileThreads;   // amplitudes per thread
+    // 2048 amplitudes per tile: 34 KB of LDS and at most 168 VGPRs, so three workgroups share a CU
+    // (three waves per SIMD); with 4096 (67 KB) only two fit, which left the MFMA units idle while
+    // a workgroup loads or stores
+    static constexpr int TE = 2048;
+    static constexpr int WGS = 3;                // workgroups per CU
+    static constexpr int B = TE / D;             // bases per tile
+    static constexpr int LOADS = TE / kTileThreads;   // amplitudes per thread
     static constexpr int RB = D / 16;            // 16-row output blocks
     static constexpr int CB = B / 16;            // 16-column blocks
     static constexpr int CW = CB * RB / 4;       // column blocks per wave (4 waves)
@@ -100,7 +104,7 @@ struct MfmaShape {
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 template <typename R, int K, bool ROWS_FAST>
-__global__ __launch_bounds__(kTileThreads, 2) void k_dense_mfma(cx<R> *amps, uint64_t nbase, uint64_t ctrl_mask,
+__global__ __launch_bounds__(kTileThreads, MfmaShape<K>::WGS) void k_dense_mfma(cx<R> *amps, uint64_t nbase, uint64_t ctrl_mask,
                                                                 PosList pl, const uint64_t *__restrict__ off_g,
                                                                 const cx<double> *__restrict__ U_g, uint64_t ntiles) {
     using M = MfmaShape<K>;
@@ -225,9 +229,10 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_dense_mfma(cx<R> *amps, uin
     uint64_t t = blockIdx.x;
     if (t >= ntiles) return;                   // the whole workgroup: no barrier is skipped by part of it
     // the next tile's loads in registers while the products run, where they fit beside the
-    // fragments and accumulators at two workgroups per CU (K = 5 with lanes along the bases);
-    // otherwise the other workgroup on the CU covers this one's loads
-    constexpr bool kPrefetch = K == 4 || (K == 5 && !ROWS_FAST);
+    // fragments and accumulators at three workgroups per CU (all but K = 6 with lanes along the rows,
+    // whose extra address registers would spill); otherwise the other workgroups on the CU cover
+    // this one's loads
+    constexpr bool kPrefetch = K <= 5 || !ROWS_FAST;
     if constexpr (kPrefetch) {
         load_tile(t);
         for (; t < ntiles; t += G) {
@@ -499,7 +504,7 @@ int launch_tile(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_ma
     const bool rows_fast = std::find(tbits.begin(), tbits.end(), 0) != tbits.end();
     const cx<double> *dU = (const cx<double> *)((char *)d + sizeof(uint64_t) * D);
     // FP64 MFMA: the waves hold their operator fragments in registers; two workgroups per CU
-    const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)device_cus() * 2);
+    const uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)device_cus() * MfmaShape<K>::WGS);
     if (rows_fast)
         hipLaunchKernelGGL((k_dense_mfma<R, K, true>), dim3((uint32_t)grid), dim3(kTileThreads), 0, s->stream,
                            (cx<R> *)s->amps, nbase, ctrl_mask, poslist(allpos), (const uint64_t *)d, dU, ntiles);
