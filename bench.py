@@ -428,8 +428,10 @@ def bench_qft16(args):
     psi /= numpy.linalg.norm(psi)
     q = Qubit(n=n)
     sm = StochasticMeasure(QFFT(q))
-    run(sm, feed={q: psi}, backend_constructor=GpuBackend.make_state)      # warm-up
-    reps = 20
+    # a run takes about a millisecond: 5 untimed runs (clocks, first uses), then 100 timed ones
+    warm, reps = 5, 100
+    for _ in range(warm):
+        run(sm, feed={q: psi}, backend_constructor=GpuBackend.make_state)
     t = time.perf_counter()
     for _ in range(reps):
         state, classic = run(sm, feed={q: psi}, backend_constructor=GpuBackend.make_state)
@@ -444,7 +446,8 @@ def bench_qft16(args):
     err = float(numpy.max(numpy.abs(state - expect)))
     # the same runs on the complex128 fused form (blocks of up to 6 qubits applied in FP64)
     from qip_amd.fused import Fused128Backend
-    run(sm, feed={q: psi}, backend_constructor=Fused128Backend.make_state)
+    for _ in range(warm):
+        run(sm, feed={q: psi}, backend_constructor=Fused128Backend.make_state)
     t = time.perf_counter()
     for _ in range(reps):
         f_state, _ = run(sm, feed={q: psi}, backend_constructor=Fused128Backend.make_state)
@@ -477,7 +480,7 @@ def bench_qft16(args):
                        "measurement and the read-back run each time"}
     diag_gbs = diag_bytes / (diag_ms * 1e-3) / 1e9
     out = _line("16-qubit QFT circuits/s (graph run incl. state upload and read-back)", 1.0 / dt, "circuits/s", reps,
-                1, dt * 1e3, "c128", cfg, max_abs_err_vs_ifft=err, max_abs_err_probabilities=perr,
+                warm, dt * 1e3, "c128", cfg, max_abs_err_vs_ifft=err, max_abs_err_probabilities=perr,
                 ms_first_run_walking_graph=walk_ms, fused_c128=fused128,
                 roofline={"bound": "launch", "kernel": "k_diag<double,0> (controlled phase, 120 of the 144 gates)",
                           "achieved": diag_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": diag_gbs / HBM_PEAK_GBS,
