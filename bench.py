@@ -476,8 +476,9 @@ def bench_qft16(args):
     perr = float(numpy.max(numpy.abs(classic[sm] - (numpy.abs(expect) ** 2)[perm])))
     cfg = {"workload": "16-qubit QFT (qip/qfft.py: 16 H, 120 controlled phases, 8 swaps) + StochasticMeasure, graph "
                        "frontend, seeded random complex128 state", "n_qubits": n, "gates": 144, "seed": args.seed,
-           "frontend": "visiting order and index groups stored by the warm-up run and replayed; every gate, the "
-                       "measurement and the read-back run each time"}
+           "frontend": "visiting order and index groups stored by the warm-up run and replayed; the 144 gates "
+                       "launched as one HIP graph recorded on an earlier run (the state buffer is reused); the "
+                       "state upload, every gate, the measurement and the read-back run each time"}
     diag_gbs = diag_bytes / (diag_ms * 1e-3) / 1e9
     out = _line("16-qubit QFT circuits/s (graph run incl. state upload and read-back)", 1.0 / dt, "circuits/s", reps,
                 warm, dt * 1e3, "c128", cfg, max_abs_err_vs_ifft=err, max_abs_err_probabilities=perr,

@@ -106,6 +106,31 @@ int qipgpu_apply_product(qipgpu_state *s, int32_t k, const int32_t *targets, con
 int qipgpu_apply_diagonal(qipgpu_state *s, int32_t k, const int32_t *targets, const double *diag,
                           int32_t n_ctrl, const int32_t *controls);
 
+/* ---- gate graphs: no counterpart in the reference ----------------------------------------------
+ * For callers that apply the same sequence of small gates again and again to states of one size
+ * (a circuit run many times on a few hundred thousand amplitudes is bound by kernel launches, not
+ * by HBM). Between qipgpu_capture_begin and qipgpu_capture_end the state's gate calls add their
+ * kernels to a graph, each after the one before, instead of launching them (the graph is built node
+ * by node; no queue is put in a capture mode, so other states and threads are not affected). Only
+ * qipgpu_apply_matrix on up to 3 targets, qipgpu_apply_diagonal on up to 6 and qipgpu_apply_product
+ * may be called, whose operands travel as kernel arguments; any other call on the state returns
+ * QIPGPU_EUNSUP (or QIPGPU_EINVAL) and makes qipgpu_capture_end drop the recording with
+ * QIPGPU_EUNSUP. Nothing recorded has run: launch the graph to apply it. A graph updates the
+ * state buffer it was captured on; it can be launched on any state that holds that buffer
+ * (qipgpu_graph_matches). A destroyed state of up to 256 MiB leaves its buffer and queue to the next
+ * state of the same size on the same device (at most 8 such buffers are kept, until the process
+ * ends), so a state made again after the first was destroyed can launch the first one's graphs. */
+typedef struct qipgpu_graph qipgpu_graph;
+int qipgpu_capture_begin(qipgpu_state *s);
+int qipgpu_capture_end(qipgpu_state *s, qipgpu_graph **out);
+/* 1 when g updates s's buffer (same buffer, size, dtype and device), else 0. */
+int qipgpu_graph_matches(const qipgpu_state *s, const qipgpu_graph *g);
+/* Enqueue the graph's kernels on s's queue (QIPGPU_EINVAL unless qipgpu_graph_matches). Launches of
+ * one graph from several threads at once are not supported. */
+int qipgpu_graph_launch(qipgpu_state *s, qipgpu_graph *g);
+/* Waits for the graph's last launch to finish, then frees it. */
+int qipgpu_graph_destroy(qipgpu_graph *g);
+
 /* ---- classical functions: func_apply (qip/ext/func_apply.pyx:11-112) --------------------------
  * |x>|q>|r> -> |x>|f(x) xor q>|r> with x read MSB-first from reg1 and only the low n_reg2 bits of
  * f(x) used (func_apply.pyx:82-97). `table[x]` = f(x) for x in [0, 2^n_reg1). In place. */

@@ -69,6 +69,11 @@ SIGNATURES = {
     "qipgpu_overlap": (ctypes.c_int, [_P, _P, _PD, _PD, _PD, _PD]),
     "qipgpu_max_abs_diff": (ctypes.c_int, [_P, _P, _PD]),
     "qipgpu_synchronize": (ctypes.c_int, [_P]),
+    "qipgpu_capture_begin": (ctypes.c_int, [_P]),
+    "qipgpu_capture_end": (ctypes.c_int, [_P, _PP]),
+    "qipgpu_graph_matches": (ctypes.c_int, [_P, _P]),
+    "qipgpu_graph_launch": (ctypes.c_int, [_P, _P]),
+    "qipgpu_graph_destroy": (ctypes.c_int, [_P]),
     "qipgpu_get_stream": (ctypes.c_int, [_P, _PP]),
     "qipgpu_batch_plan": (ctypes.c_int, [_I32, _P, _I64, _P, _P, _P, _I32, _I32, _P, _P]),
     "qipgpu_batch_create": (ctypes.c_int, [_I32, _P, _PP]),

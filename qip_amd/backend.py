@@ -232,6 +232,16 @@ def product_passes(n: int, targets: Sequence[int]):
     return [c for c in chunks if c]
 
 
+class GateGraph:
+    """A captured gate sequence (qipgpu_graph): the kernels of the gates applied between
+    GpuBackend.capture_begin and capture_end, launched again with launch_graph on any state that
+    holds the buffer it was captured on. Freed with the object."""
+
+    def __init__(self, lib, handle: ctypes.c_void_p):
+        self.handle = handle
+        self._finalizer = weakref.finalize(self, lib.qipgpu_graph_destroy, handle)
+
+
 def pack_matrix(targets: Sequence[int], mat: numpy.ndarray, controls: Sequence[int]) -> tuple:
     """qipgpu_apply_matrix's arguments after the state handle: (k, targets, matrix, number of
     controls, controls), then the array the matrix argument points into (kept alive with them)."""
@@ -265,6 +275,40 @@ class GpuBackend(StateType):
         self._dtype = dtype
         self._np_dtype = numpy.complex128 if dtype == _capi.C128 else numpy.complex64
         self._lib = _capi.load()
+
+    # A destroyed state of up to 256 MiB leaves its buffer to the next state of its size, so a gate
+    # graph captured on one run's state can be launched on the next run's (pipeline.Schedule.play)
+    GRAPH_STATE_BYTES = 256 << 20
+
+    @property
+    def graphs_reusable(self) -> bool:
+        """Whether gate graphs captured on this state can serve the states made after it."""
+        return type(self).supports_gate_graphs and (2 ** self.n) * numpy.dtype(self._np_dtype).itemsize <= \
+            self.GRAPH_STATE_BYTES
+
+    supports_gate_graphs = True
+
+    def capture_begin(self) -> None:
+        """Record the following gate calls into a graph instead of running them (only gates whose
+        operands are kernel arguments: dense on up to 3 targets, diagonal on up to 6, products of
+        single-qubit factors; anything else raises NotImplementedError and drops the capture)."""
+        _capi.check(self._lib.qipgpu_capture_begin(self._h), "capture_begin")
+
+    def capture_end(self) -> Optional[GateGraph]:
+        """The graph of the gates recorded since capture_begin (none of them has run yet), or None
+        when the capture was dropped."""
+        g = ctypes.c_void_p()
+        rc = self._lib.qipgpu_capture_end(self._h, ctypes.byref(g))
+        if rc == _capi.EUNSUP:
+            return None
+        _capi.check(rc, "capture_end")
+        return GateGraph(self._lib, g)
+
+    def graph_matches(self, graph: GateGraph) -> bool:
+        return bool(self._lib.qipgpu_graph_matches(self._h, graph.handle))
+
+    def launch_graph(self, graph: GateGraph) -> None:
+        _capi.check(self._lib.qipgpu_graph_launch(self._h, graph.handle), "graph_launch")
 
     # -- construction: qip/backend.py:73-104 --------------------------------------------------
     @classmethod

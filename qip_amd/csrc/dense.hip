@@ -679,6 +679,7 @@ namespace qipgpu {
 
 int diag_large(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mask, const std::vector<int> &cpos,
                const std::complex<double> *d) {
+    QIP_NOT_CAPTURING(s, "a diagonal on more than 6 targets");
     if ((int)tbits.size() > QIPGPU_MAX_DENSE)
         return set_error(QIPGPU_EUNSUP, "diagonal operator on %d qubits exceeds %d", (int)tbits.size(), QIPGPU_MAX_DENSE);
     return s->dtype == QIPGPU_C128 ? launch_diag_table<double>(s, tbits, ctrl_mask, cpos, d)
@@ -687,6 +688,7 @@ int diag_large(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mas
 
 int dense_large(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mask, const std::vector<int> &allpos,
                 const std::complex<double> *U) {
+    QIP_NOT_CAPTURING(s, "a dense operator on more than 3 targets");
     const int K = (int)tbits.size();
     const bool c128 = s->dtype == QIPGPU_C128;
     switch (K) {
@@ -707,6 +709,7 @@ extern "C" {
 int qipgpu_apply_kron(qipgpu_state *s, int32_t n_groups, const int32_t *group_len, const int32_t *group_idx,
                       const int32_t *levels, const int32_t *kinds, const int32_t *params, const double *mats) {
     if (!s) return set_error(QIPGPU_EINVAL, "null state");
+    QIP_NOT_CAPTURING(s, "a descriptor by the column sum");
     if (n_groups < 1) return set_error(QIPGPU_EINVAL, "a descriptor needs at least one group");
     std::vector<int32_t> all;
     int32_t at = 0;

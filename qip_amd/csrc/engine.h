@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <complex>
 #include <cstdio>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -97,6 +98,13 @@ struct qipgpu_state {
     hipEvent_t gemm_ev = nullptr;
     void *gemm_tmp = nullptr;
     size_t gemm_tmp_bytes = 0;
+    // a gate graph being recorded (qipgpu_capture_begin .. qipgpu_capture_end): gate kernels whose
+    // operands are all kernel arguments become graph nodes, each after the previous one, instead of
+    // launches (launch_gate); any other operation is refused and marks the recording refused
+    bool capturing = false;
+    bool capture_refused = false;
+    hipGraph_t rec_graph = nullptr;
+    hipGraphNode_t rec_last = nullptr;
     // gates recorded for online fusion and not launched yet (complex64 states)
     qipgpu::Recorder *recorder = nullptr;
 };
@@ -153,6 +161,44 @@ int check_qubits(const qipgpu_state *s, int32_t k, const int32_t *q, const char 
 int ring_upload(qipgpu_state *s, const void *src, size_t bytes, void **dev);
 // Launch every recorded gate (recorder.hip); a no-op while the recorder itself is launching.
 int flush_recorded(qipgpu_state *s);
+// While s is capturing: mark the capture refused and return QIPGPU_EUNSUP naming `what`.
+int refuse_in_capture(qipgpu_state *s, const char *what);
+#define QIP_NOT_CAPTURING(s, what)                                                        \
+    do {                                                                                   \
+        if ((s)->capturing) return ::qipgpu::refuse_in_capture((s), (what));               \
+    } while (0)
+
+// A gate kernel whose operands all travel as kernel arguments: launched on the state's queue, or,
+// while a gate graph is recorded, added to it after the previously recorded kernel (the node copies
+// the arguments).
+template <typename... P>
+int launch_gate(qipgpu_state *s, void (*kernel)(P...), dim3 grid, dim3 block, P... args) {
+    if (!s->capturing) {
+        hipLaunchKernelGGL(kernel, grid, block, 0, s->stream, args...);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return set_error(QIPGPU_EHIP, "kernel launch failed: %s", hipGetErrorString(e));
+        return QIPGPU_OK;
+    }
+    void *params[] = {static_cast<void *>(&args)...};
+    hipKernelNodeParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.func = reinterpret_cast<void *>(kernel);
+    p.gridDim = grid;
+    p.blockDim = block;
+    p.sharedMemBytes = 0;
+    p.kernelParams = params;
+    p.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    const hipGraphNode_t prev = s->rec_last;
+    const hipError_t e = hipGraphAddKernelNode(&node, s->rec_graph, prev ? &prev : nullptr, prev ? 1 : 0, &p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        s->capture_refused = true;
+        return set_error(QIPGPU_EHIP, "hipGraphAddKernelNode failed: %s", hipGetErrorString(e));
+    }
+    s->rec_last = node;
+    return QIPGPU_OK;
+}
 // Drop recorded gates (the state is about to be overwritten) / free the recorder.
 void recorder_discard(qipgpu_state *s);
 void recorder_free(qipgpu_state *s);

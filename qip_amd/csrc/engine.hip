@@ -51,7 +51,14 @@ int device_cus() {
     return cus;
 }
 
+int refuse_in_capture(qipgpu_state *s, const char *what) {
+    s->capture_refused = true;
+    return set_error(QIPGPU_EUNSUP, "%s cannot be recorded into a gate graph (only gates whose operands are "
+                                    "kernel arguments can)", what);
+}
+
 int ensure_scratch(qipgpu_state *s, size_t bytes) {
+    QIP_NOT_CAPTURING(s, "an operation that uses device scratch");
     if (bytes <= s->scratch_cap) return QIPGPU_OK;
     if (s->scratch) {
         QIP_HIP(hipStreamSynchronize(s->stream));
@@ -90,6 +97,7 @@ __global__ __launch_bounds__(256) void k_stage(const uint64_t *src, uint64_t *__
 }
 
 int ring_upload(qipgpu_state *s, const void *src, size_t bytes, void **dev) {
+    QIP_NOT_CAPTURING(s, "an operator staged through the pinned ring");
     if (bytes > kRingSlotBytes) return set_error(QIPGPU_EINVAL, "operator of %zu bytes exceeds staging slot", bytes);
     if (!s->d_ring) {
         QIP_HIP(hipMalloc(&s->d_ring, kRingSlots * kRingSlotBytes));
@@ -343,6 +351,50 @@ int qipgpu_device_count(int32_t *count) {
     return QIPGPU_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Amplitude buffers and queues of destroyed small states, kept for the next state of the same size on
+// the same device: creating a state then costs no allocation, and a state made again after another
+// was destroyed gets the same buffer, so gate graphs captured on the first replay on the second. At
+// most kPoolEntries buffers of up to kPoolMaxBytes each are held (2 GiB), until the process ends.
+struct PoolEntry {
+    int device;
+    size_t bytes;
+    void *amps;
+    hipStream_t stream;
+};
+constexpr size_t kPoolMaxBytes = size_t(256) << 20;
+constexpr size_t kPoolEntries = 8;
+std::mutex g_pool_mu;
+std::vector<PoolEntry> g_pool;
+
+// the most recently given buffer of the size first
+bool pool_take(int device, size_t bytes, void **amps, hipStream_t *stream) {
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    for (size_t i = g_pool.size(); i-- > 0;)
+        if (g_pool[i].device == device && g_pool[i].bytes == bytes) {
+            *amps = g_pool[i].amps;
+            *stream = g_pool[i].stream;
+            g_pool.erase(g_pool.begin() + (long)i);
+            return true;
+        }
+    return false;
+}
+
+bool pool_give(int device, size_t bytes, void *amps, hipStream_t stream) {
+    if (bytes > kPoolMaxBytes) return false;
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    if (g_pool.size() >= kPoolEntries) return false;
+    g_pool.push_back(PoolEntry{device, bytes, amps, stream});
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
 int qipgpu_create(int32_t n, int32_t dtype, qipgpu_state **out) {
     *out = nullptr;
     if (n < 1 || n > QIPGPU_MAX_QUBITS) return set_error(QIPGPU_EINVAL, "n=%d outside 1..%d", n, QIPGPU_MAX_QUBITS);
@@ -352,7 +404,14 @@ int qipgpu_create(int32_t n, int32_t dtype, qipgpu_state **out) {
     s->dtype = dtype;
     s->len = 1ull << n;
     s->amp_bytes = dtype == QIPGPU_C128 ? 16 : 8;
-    QIP_HIP(hipGetDevice(&s->device));
+    if (hipGetDevice(&s->device) != hipSuccess) {
+        delete s;
+        return set_error(QIPGPU_EHIP, "hipGetDevice failed");
+    }
+    if (pool_take(s->device, s->len * s->amp_bytes, &s->amps, &s->stream)) {
+        *out = s;
+        return QIPGPU_OK;
+    }
     if (hipMalloc(&s->amps, s->len * s->amp_bytes) != hipSuccess) {
         (void)hipGetLastError();
         delete s;
@@ -370,8 +429,10 @@ int qipgpu_create(int32_t n, int32_t dtype, qipgpu_state **out) {
 
 int qipgpu_destroy(qipgpu_state *s) {
     if (!s) return QIPGPU_OK;
+    if (s->rec_graph) (void)hipGraphDestroy(s->rec_graph);   // an unfinished recording is dropped
     (void)hipStreamSynchronize(s->stream);
-    if (s->amps) (void)hipFree(s->amps);
+    const bool pooled = s->amps && pool_give(s->device, s->len * s->amp_bytes, s->amps, s->stream);
+    if (s->amps && !pooled) (void)hipFree(s->amps);
     if (s->scratch) (void)hipFree(s->scratch);
     if (s->d_ring) (void)hipFree(s->d_ring);
     if (s->h_ring) (void)hipHostFree(s->h_ring);
@@ -384,8 +445,95 @@ int qipgpu_destroy(qipgpu_state *s) {
     if (s->gemm_ev) (void)hipEventDestroy(s->gemm_ev);
     if (s->gemm_tmp) (void)hipFree(s->gemm_tmp);
     recorder_free(s);
-    (void)hipStreamDestroy(s->stream);
+    if (!pooled) (void)hipStreamDestroy(s->stream);
     delete s;
+    return QIPGPU_OK;
+}
+
+// ---- gate graphs (no counterpart in the reference) ----------------------------------------------
+
+struct qipgpu_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;  // recorded after each launch
+    bool launched = false;
+    void *amps = nullptr;       // the state buffer the graph's kernels update
+    uint64_t len = 0;
+    int32_t dtype = 0;
+    int device = 0;
+};
+
+// The graph is built node by node (launch_gate), not by stream capture: no queue enters a capture
+// mode, so other states and threads go on as usual while one state records.
+int qipgpu_capture_begin(qipgpu_state *s) {
+    if (s->capturing) return set_error(QIPGPU_EINVAL, "the state is already capturing");
+    int rc = flush_recorded(s);
+    if (rc) return rc;
+    QIP_HIP(hipGraphCreate(&s->rec_graph, 0));
+    s->rec_last = nullptr;
+    s->capturing = true;
+    s->capture_refused = false;
+    return QIPGPU_OK;
+}
+
+int qipgpu_capture_end(qipgpu_state *s, qipgpu_graph **out) {
+    *out = nullptr;
+    if (!s->capturing) return set_error(QIPGPU_EINVAL, "the state is not capturing");
+    s->capturing = false;
+    hipGraph_t g = s->rec_graph;
+    s->rec_graph = nullptr;
+    s->rec_last = nullptr;
+    if (s->capture_refused) {
+        (void)hipGraphDestroy(g);
+        return set_error(QIPGPU_EUNSUP, "capture dropped: an operation was refused while recording");
+    }
+    qipgpu_graph *q = new qipgpu_graph();
+    q->graph = g;
+    if (hipGraphInstantiate(&q->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipGraphDestroy(g);
+        delete q;
+        return set_error(QIPGPU_EHIP, "hipGraphInstantiate failed");
+    }
+    if (hipEventCreateWithFlags(&q->done, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipGraphExecDestroy(q->exec);
+        (void)hipGraphDestroy(g);
+        delete q;
+        return set_error(QIPGPU_EHIP, "hipEventCreate failed");
+    }
+    q->amps = s->amps;
+    q->len = s->len;
+    q->dtype = s->dtype;
+    q->device = s->device;
+    *out = q;
+    return QIPGPU_OK;
+}
+
+int qipgpu_graph_launch(qipgpu_state *s, qipgpu_graph *g) {
+    if (!g) return set_error(QIPGPU_EINVAL, "null graph");
+    if (s->capturing) return refuse_in_capture(s, "a graph launch");
+    if (g->amps != s->amps || g->len != s->len || g->dtype != s->dtype || g->device != s->device)
+        return set_error(QIPGPU_EINVAL, "the graph was captured on another state buffer");
+    int rc = flush_recorded(s);
+    if (rc) return rc;
+    QIP_HIP(hipGraphLaunch(g->exec, s->stream));
+    QIP_HIP(hipEventRecord(g->done, s->stream));
+    g->launched = true;
+    return QIPGPU_OK;
+}
+
+int qipgpu_graph_matches(const qipgpu_state *s, const qipgpu_graph *g) {
+    return g && g->amps == s->amps && g->len == s->len && g->dtype == s->dtype && g->device == s->device;
+}
+
+int qipgpu_graph_destroy(qipgpu_graph *g) {
+    if (!g) return QIPGPU_OK;
+    if (g->launched) (void)hipEventSynchronize(g->done);   // its last launch has run
+    if (g->done) (void)hipEventDestroy(g->done);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
     return QIPGPU_OK;
 }
 
@@ -396,6 +544,7 @@ int qipgpu_info(const qipgpu_state *s, int32_t *n, int32_t *dtype) {
 }
 
 int qipgpu_synchronize(qipgpu_state *s) {
+    QIP_NOT_CAPTURING(s, "a synchronisation");
     const int rc = flush_recorded(s);
     if (rc) return rc;
     QIP_HIP(hipStreamSynchronize(s->stream));
@@ -409,6 +558,7 @@ int qipgpu_get_stream(qipgpu_state *s, void **stream) {
 
 int qipgpu_init_product(qipgpu_state *s, int32_t n_groups, const int32_t *group_len,
                         const int32_t *group_idx, const double *feeds) {
+    QIP_NOT_CAPTURING(s, "a state initialisation");
     s->fused_prev_count = 0;  // every amplitude is rewritten, so recorded gates and the factor are moot
     recorder_discard(s);
     if (n_groups < 0) return set_error(QIPGPU_EINVAL, "negative group count");
@@ -503,6 +653,7 @@ static int check_range(qipgpu_state *s, int64_t start, int64_t end) {
 }
 
 int qipgpu_get_range(qipgpu_state *s, int64_t start, int64_t end, void *host) {
+    QIP_NOT_CAPTURING(s, "a host copy");
     int rc = check_range(s, start, end);
     if (rc) return rc;
     if (end == start) return QIPGPU_OK;
@@ -515,6 +666,7 @@ int qipgpu_get_range(qipgpu_state *s, int64_t start, int64_t end, void *host) {
 }
 
 int qipgpu_set_range(qipgpu_state *s, int64_t start, int64_t end, const void *host) {
+    QIP_NOT_CAPTURING(s, "a host copy");
     int rc = check_range(s, start, end);
     if (rc) return rc;
     if (end == start) return QIPGPU_OK;
@@ -527,6 +679,7 @@ int qipgpu_set_range(qipgpu_state *s, int64_t start, int64_t end, const void *ho
 }
 
 int qipgpu_add_range(qipgpu_state *s, int64_t start, int64_t end, const void *host) {
+    QIP_NOT_CAPTURING(s, "a host copy");
     int rc = check_range(s, start, end);
     if (rc) return rc;
     if (end == start) return QIPGPU_OK;
@@ -749,9 +902,7 @@ static int launch_dense_reg(qipgpu_state *s, const std::vector<int> &tbits, uint
     const uint32_t nb = blocks_for(a.nbase, 256 * ITEMS);
     a.order = block_order(s->len * s->amp_bytes, s->amp_bytes, nb);
     a.nblocks = nb;
-    hipLaunchKernelGGL((k_dense<R, K, ITEMS>), dim3(capped_grid(nb)), dim3(256), 0, s->stream, a);
-    QIP_LAUNCHED();
-    return QIPGPU_OK;
+    return launch_gate(s, &k_dense<R, K, ITEMS>, dim3(capped_grid(nb)), dim3(256), a);
 }
 
 template <typename R, int K>
@@ -771,9 +922,7 @@ static int launch_diag(qipgpu_state *s, const std::vector<int> &tbits, uint64_t 
     const uint32_t nb = blocks_for(a.count, 256 * ITEMS);
     a.order = block_order(s->len * s->amp_bytes, s->amp_bytes, nb);
     a.nblocks = nb;
-    hipLaunchKernelGGL((k_diag<R, K, ITEMS>), dim3(capped_grid(nb)), dim3(256), 0, s->stream, a);
-    QIP_LAUNCHED();
-    return QIPGPU_OK;
+    return launch_gate(s, &k_diag<R, K, ITEMS>, dim3(capped_grid(nb)), dim3(256), a);
 }
 
 template <typename R>
@@ -1066,6 +1215,7 @@ static int permute_reg2(qipgpu_state *s, int32_t n_reg1, int32_t n_reg2, const i
 // per-pair kernel (which touches only the moved pairs) runs instead of the tiles.
 static int func_apply_dev(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, int32_t n_reg2, const int32_t *reg2,
                           const int32_t *d_table, uint64_t nonzero, int32_t flags) {
+    QIP_NOT_CAPTURING(s, "func_apply (its f table lives outside the graph)");
     const int n = s->n;
     if (n_reg1 < 1 || n_reg2 < 1 || n_reg1 + n_reg2 > n)
         return set_error(QIPGPU_EINVAL, "func_apply: register sizes %d + %d for n=%d", n_reg1, n_reg2, n);

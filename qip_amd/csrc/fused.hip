@@ -406,6 +406,7 @@ int settle(qipgpu_state *s) {
 extern "C" {
 
 int qipgpu_fused_apply(qipgpu_state *s, int32_t k, const int32_t *targets, const double *mat) {
+    QIP_NOT_CAPTURING(s, "a fused block");
     if (s->dtype != QIPGPU_C64) return set_error(QIPGPU_EINVAL, "fused blocks apply to complex64 states");
     if (s->n < kTileBits) return set_error(QIPGPU_EUNSUP, "fused blocks need n >= %d (n=%d)", kTileBits, s->n);
     if (k < 1 || k > 6) return set_error(QIPGPU_EINVAL, "fused block on %d qubits (1..6)", k);
@@ -467,6 +468,7 @@ int qipgpu_fused_apply(qipgpu_state *s, int32_t k, const int32_t *targets, const
 }
 
 int qipgpu_fused_pattern_sweep(qipgpu_state *s, int32_t k, const int32_t *targets) {
+    QIP_NOT_CAPTURING(s, "a fused pattern sweep");
     if (s->dtype != QIPGPU_C64) return set_error(QIPGPU_EINVAL, "fused blocks apply to complex64 states");
     if (s->n < kTileBits) return set_error(QIPGPU_EUNSUP, "fused blocks need n >= %d (n=%d)", kTileBits, s->n);
     if (k < 1 || k > 6) return set_error(QIPGPU_EINVAL, "fused block on %d qubits (1..6)", k);
@@ -495,6 +497,7 @@ int qipgpu_fused_finish(qipgpu_state *s) {
     const int rc = flush_recorded(s);
     if (rc) return rc;
     if (!s->fused_prev_count) return QIPGPU_OK;
+    QIP_NOT_CAPTURING(s, "a pending fused renormalisation");
     hipLaunchKernelGGL(k_fused_norm, dim3(1), dim3(64), 0, s->stream,
                        (const double *)fused_partials_of(s, s->fused_parity ^ 1), s->fused_prev_count, s->d_fused);
     QIP_LAUNCHED();

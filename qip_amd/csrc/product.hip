@@ -246,9 +246,9 @@ __global__ __launch_bounds__(256, (blocks_per_cu<R, kLds>())) void k_product(cx<
 }
 
 template <typename R, bool kLds>
-void launch_product(cx<R> *amps, const ProductArgs &a, hipStream_t stream) {
+int launch_product(qipgpu_state *s, const ProductArgs &a) {
     const uint32_t grid = (uint32_t)std::min<uint64_t>(a.ntiles, (uint64_t)device_cus() * blocks_per_cu<R, kLds>());
-    hipLaunchKernelGGL((k_product<R, kLds>), dim3(grid), dim3(256), 0, stream, amps, a);
+    return launch_gate(s, &k_product<R, kLds>, dim3(grid), dim3(256), (cx<R> *)s->amps, a);
 }
 
 constexpr int kLowBits = kProductLowBits;           // state bits every pass keeps in its tile
@@ -332,15 +332,8 @@ int product_pass(qipgpu_state *s, int32_t k, const int32_t *targets, const doubl
     }
     const bool lds = (a.fmask & 0xffu) != 0;   // a factor below u bit 8 needs a transpose
     a.xsync = !lds && (a.xmask & 0xc0u) != 0;
-    if (s->dtype == QIPGPU_C128) {
-        if (lds) launch_product<double, true>((cx<double> *)s->amps, a, s->stream);
-        else launch_product<double, false>((cx<double> *)s->amps, a, s->stream);
-    } else {
-        if (lds) launch_product<float, true>((cx<float> *)s->amps, a, s->stream);
-        else launch_product<float, false>((cx<float> *)s->amps, a, s->stream);
-    }
-    QIP_LAUNCHED();
-    return QIPGPU_OK;
+    if (s->dtype == QIPGPU_C128) return lds ? launch_product<double, true>(s, a) : launch_product<double, false>(s, a);
+    return lds ? launch_product<float, true>(s, a) : launch_product<float, false>(s, a);
 }
 
 }  // namespace

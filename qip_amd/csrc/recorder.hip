@@ -424,6 +424,7 @@ int qipgpu_fused_set_recording(qipgpu_state *s, int32_t width, int32_t streaming
 
 int qipgpu_fused_record(qipgpu_state *s, int32_t n_gates, const int32_t *gate_k, const int32_t *gate_qubits,
                         const double *mats) {
+    QIP_NOT_CAPTURING(s, "gate recording for fusion");
     int rc = ensure_recorder(s);
     if (rc) return rc;
     if (n_gates < 0) return set_error(QIPGPU_EINVAL, "negative gate count");

@@ -294,6 +294,7 @@ class FusedBackend(GpuBackend):
     """complex64 state; kronselect_dot records into the library's online fusion (module docstring)."""
 
     fusion_width = 6
+    supports_gate_graphs = False      # gates are recorded for fusion, not launched one by one
 
     def __init__(self, n, handle, dtype):
         super().__init__(n, handle, dtype)

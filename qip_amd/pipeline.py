@@ -17,6 +17,14 @@ A schedule depends only on the graph and on the feed's keys, so run keeps compil
 first output node (at most four feed-key sets per node) and plays one again while no node anywhere
 has gained a sink since it was compiled. Graphs with a node that overrides `feed` or
 `remap_index` see the live index map and are walked on every run.
+
+On a GpuBackend state of up to 256 MiB, a replayed schedule's runs of at least GRAPH_MIN_GATES plain
+gate steps are captured into a HIP graph on first use and launched as one graph afterwards (a
+circuit of small gates on a small state is bound by kernel launches). A graph serves a later run
+while the state holds the same buffer (the engine hands a destroyed state's buffer to the next state
+of its size) and every gate node of the run still holds the descriptor, and that descriptor the
+kept plan, it had at capture; otherwise the run is captured again. A run that cannot be captured
+(an operator whose operands do not fit kernel arguments) is applied gate by gate from then on.
 """
 import heapq
 import itertools
@@ -86,6 +94,8 @@ class _Epoch:
 
 _GRAPH_EPOCH = _Epoch()
 _CACHE_LOCK = threading.Lock()
+GRAPH_MIN_GATES = 8          # plain gate steps in a run worth a graph
+GRAPH_KEEP = 4               # graphs kept per run of steps (one per state buffer in use)
 _SCHEDULES_PER_NODE = 4
 
 
@@ -172,7 +182,7 @@ class Schedule:
     every node uses the base `feed` and `remap_index`, the steps as (node, index groups), the
     groups None for nodes that keep the base `feed_indices` (the identity, annotation 0)."""
 
-    __slots__ = ("sources", "nodes", "n", "steps", "epoch")
+    __slots__ = ("sources", "nodes", "n", "steps", "epoch", "segments", "graphs", "lock")
 
     def __init__(self, outputs: Sequence[PipelineObject]):
         sources, nodes = _collect(outputs, {})
@@ -180,6 +190,9 @@ class Schedule:
         self.nodes = set(nodes)
         self.n = sum(s.n for s in self.sources)
         self.epoch = _GRAPH_EPOCH.value
+        self.segments: Optional[Dict[int, int]] = None   # start step -> end of a run of plain gates
+        self.graphs: Dict[int, Any] = {}                 # start step -> captured graphs, or False
+        self.lock = threading.Lock()
         self.steps: Optional[List[Tuple[PipelineObject, Tuple[Tuple[int, ...], ...]]]] = None
         if all(type(v).feed is PipelineObject.feed and type(v).remap_index is PipelineObject.remap_index
                for v in nodes):
@@ -205,7 +218,17 @@ class Schedule:
             _walk(self.sources, self.nodes, acc.feed)
             return acc.state.get_state(), acc.classic_map
         classic = acc.classic_map
-        for node, groups in self.steps:
+        steps = self.steps
+        segments = self._segments() if getattr(state, "graphs_reusable", False) else {}
+        i = 0
+        while i < len(steps):
+            end = segments.get(i)
+            if end is not None and self._play_graph(state, i, end, acc.n):
+                for node, _ in steps[i:end]:
+                    classic[node] = 0        # what a plain gate step annotates
+                i = end
+                continue
+            node, groups = steps[i]
             if groups is None:
                 classic[node] = 0            # the base node: identity, annotation 0, no qubit removed
             else:
@@ -213,7 +236,64 @@ class Schedule:
                 if removed > 0 or note is not None:
                     classic[node] = note
                     acc.n -= removed
+            i += 1
         return state.get_state(), classic
+
+    def _segments(self) -> Dict[int, int]:
+        """Runs of steps that are plain gates (MatrixOp nodes with its own feed_indices) or identity
+        steps, holding at least GRAPH_MIN_GATES gates: start -> end."""
+        if self.segments is None:
+            from .operators import MatrixOp
+            plain = MatrixOp.feed_indices
+            segs, start, gates = {}, None, 0
+            for k, (node, groups) in enumerate(self.steps + [(None, ())]):
+                ok = node is not None and (groups is None or type(node).feed_indices is plain)
+                if ok:
+                    if start is None:
+                        start, gates = k, 0
+                    gates += groups is not None
+                    continue
+                if start is not None and gates >= GRAPH_MIN_GATES:
+                    segs[start] = k
+                start = None
+            self.segments = segs
+        return self.segments
+
+    def _play_graph(self, state, start: int, end: int, n: int) -> bool:
+        """Apply steps[start:end] as a gate graph: a kept one that still holds, else a new capture.
+        False when this run of steps cannot be captured (nothing has been applied then)."""
+        gates = [(node, groups) for node, groups in self.steps[start:end] if groups is not None]
+        with self.lock:
+            kept = self.graphs.get(start)
+        if kept is False:
+            return False
+        for graph, checks in kept or ():
+            if state.graph_matches(graph) and all(node.ms is d and d._plan is p for node, d, p in checks):
+                state.launch_graph(graph)
+                return True
+        state.capture_begin()
+        try:
+            for node, groups in gates:
+                node.feed_indices(state, [list(g) for g in groups], n)
+        except Exception:
+            state.capture_end()          # dropped: nothing recorded has run
+            with self.lock:
+                self.graphs[start] = False
+            return False
+        graph = state.capture_end()
+        if graph is None:
+            with self.lock:
+                self.graphs[start] = False
+            return False
+        checks = [(node, node.ms, getattr(node.ms, "_plan", None)) for node, _ in gates]
+        state.launch_graph(graph)
+        with self.lock:
+            if any(p is None for _, _, p in checks):
+                self.graphs[start] = False   # a descriptor without a kept plan: a change could not be seen
+            else:
+                kept = self.graphs.get(start) or []
+                self.graphs[start] = (kept + [(graph, checks)])[-GRAPH_KEEP:]
+        return True
 
 
 def _schedule_for(outputs: Tuple[PipelineObject, ...], feed_keys: Tuple) -> Schedule:
