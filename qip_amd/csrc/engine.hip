@@ -659,9 +659,10 @@ int qipgpu_get_range(qipgpu_state *s, int64_t start, int64_t end, void *host) {
     if (end == start) return QIPGPU_OK;
     rc = settle(s);
     if (rc) return rc;
+    // on the state's own queue, not the legacy one, which would wait for every blocking queue
+    QIP_HIP(hipMemcpyAsync(host, static_cast<char *>(s->amps) + start * s->amp_bytes, (end - start) * s->amp_bytes,
+                           hipMemcpyDeviceToHost, s->stream));
     QIP_HIP(hipStreamSynchronize(s->stream));
-    QIP_HIP(hipMemcpy(host, static_cast<char *>(s->amps) + start * s->amp_bytes, (end - start) * s->amp_bytes,
-                      hipMemcpyDeviceToHost));
     return QIPGPU_OK;
 }
 
@@ -672,9 +673,9 @@ int qipgpu_set_range(qipgpu_state *s, int64_t start, int64_t end, const void *ho
     if (end == start) return QIPGPU_OK;
     rc = settle(s);
     if (rc) return rc;
-    QIP_HIP(hipStreamSynchronize(s->stream));
-    QIP_HIP(hipMemcpy(static_cast<char *>(s->amps) + start * s->amp_bytes, host, (end - start) * s->amp_bytes,
-                      hipMemcpyHostToDevice));
+    QIP_HIP(hipMemcpyAsync(static_cast<char *>(s->amps) + start * s->amp_bytes, host, (end - start) * s->amp_bytes,
+                           hipMemcpyHostToDevice, s->stream));
+    QIP_HIP(hipStreamSynchronize(s->stream));   // the host data is read before return
     return QIPGPU_OK;
 }
 
@@ -688,8 +689,7 @@ int qipgpu_add_range(qipgpu_state *s, int64_t start, int64_t end, const void *ho
     if (rc) return rc;
     rc = ensure_scratch(s, count * s->amp_bytes);
     if (rc) return rc;
-    QIP_HIP(hipStreamSynchronize(s->stream));
-    QIP_HIP(hipMemcpy(s->scratch, host, count * s->amp_bytes, hipMemcpyHostToDevice));
+    QIP_HIP(hipMemcpyAsync(s->scratch, host, count * s->amp_bytes, hipMemcpyHostToDevice, s->stream));
     const uint32_t nb = std::min<uint32_t>(blocks_for(count, 256), 1u << 20);
     if (s->dtype == QIPGPU_C128)
         hipLaunchKernelGGL(k_add_range<double>, dim3(nb), dim3(256), 0, s->stream,
@@ -1349,12 +1349,12 @@ int qipgpu_func_apply(qipgpu_state *s, int32_t n_reg1, const int32_t *reg1, int3
     const size_t bytes = sizeof(int32_t) << n_reg1;
     int rc = ensure_scratch(s, bytes);
     if (rc) return rc;
-    QIP_HIP(hipStreamSynchronize(s->stream));
-    QIP_HIP(hipMemcpy(s->scratch, table, bytes, hipMemcpyHostToDevice));
+    QIP_HIP(hipMemcpyAsync(s->scratch, table, bytes, hipMemcpyHostToDevice, s->stream));
     rc = func_apply_dev(s, n_reg1, reg1, n_reg2, reg2, (const int32_t *)s->scratch,
                         count_nonzero(table, 1ull << n_reg1), 0);
+    const hipError_t e = hipStreamSynchronize(s->stream);   // the host table is read before return
     if (rc) return rc;
-    QIP_HIP(hipStreamSynchronize(s->stream));  // scratch may be reused by the next call
+    if (e != hipSuccess) return set_error(QIPGPU_EHIP, "hipStreamSynchronize failed: %s", hipGetErrorString(e));
     return QIPGPU_OK;
 }
 
