@@ -68,6 +68,19 @@ MARG_PMC_28Q = {"fetch_kib": 2097182.76, "fetch_scale": 1048576.0 / 524313.75,
                 "write_kib": 3026.64, "write_scale": 1.0}
 
 
+# HBM traffic of one single-qubit sweep (k_dense<double,1,1>, one base per thread) on a 28-qubit
+# complex128 state (4 GiB), measured the same way by two rocprofv3 --pmc runs of `bench.py --workload
+# circuit28 --no-cpu-baseline --copy-calibration-gib 1` (mean over the 619 one-qubit sweeps of the run;
+# FETCH_SIZE reads 524316.1 KiB for the 1 GiB copy, WRITE_SIZE 1048576 KiB). Commands and figures:
+# profiles/k_dense_circuit28_depth20.txt.
+DENSE1_PMC_28Q = {"fetch_kib": 2097166.0, "fetch_scale": 1048576.0 / 524316.1,
+                  "write_kib": 4194304.3, "write_scale": 1.0}
+
+
+def pmc_bytes(p) -> int:
+    return int((p["fetch_kib"] * p["fetch_scale"] + p["write_kib"] * p["write_scale"]) * 1024)
+
+
 def fused6_traffic_bytes(n: int):
     """Measured HBM bytes per k_fused6 launch at n = 30 (None for other sizes)."""
     if n != 30:
@@ -271,7 +284,8 @@ def bench_circuit28(args):
                 infidelity_c64_vs_c128=res["infidelity_c64_vs_c128"],
                 max_abs_err_c64_vs_c128=res["max_abs_err_c64_vs_c128"],
                 roofline={"bound": "hbm", "kernel": d["kernel"], "achieved": d["achieved_gbs"], "peak": HBM_PEAK_GBS,
-                          "unit": "GB/s", "frac": d["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
+                          "unit": "GB/s", "frac": d["achieved_gbs"] / HBM_PEAK_GBS,
+                          "traffic": pmc_bytes(DENSE1_PMC_28Q) if n == 28 else None,
                           "bytes_per_launch": d["bytes_per_launch"], "avg_launch_ms": d["avg_ms"]},
                 roofline_c64=dict(dom["c64"], frac=dom["c64"]["achieved_gbs"] / HBM_PEAK_GBS))
     if not args.no_cpu_baseline:
