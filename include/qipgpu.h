@@ -113,8 +113,9 @@ int qipgpu_apply_diagonal(qipgpu_state *s, int32_t k, const int32_t *targets, co
  * kernels to a graph, each after the one before, instead of launching them (the graph is built node
  * by node; no queue is put in a capture mode, so other states and threads are not affected). Only
  * qipgpu_apply_matrix on up to 3 targets, qipgpu_apply_diagonal on up to 6 and qipgpu_apply_product
- * may be called, whose operands travel as kernel arguments; any other call on the state returns
- * QIPGPU_EUNSUP (or QIPGPU_EINVAL) and makes qipgpu_capture_end drop the recording with
+ * may be called, whose operands travel as kernel arguments; any other call that reads, writes or
+ * stages data for the state (wider operators, func_apply, reductions, copies, initialisation,
+ * fused blocks) returns QIPGPU_EUNSUP and makes qipgpu_capture_end drop the recording with
  * QIPGPU_EUNSUP. Nothing recorded has run: launch the graph to apply it. A graph updates the
  * state buffer it was captured on; it can be launched on any state that holds that buffer
  * (qipgpu_graph_matches). A destroyed state of up to 256 MiB leaves its buffer and queue to the next
