@@ -468,8 +468,9 @@ def bench_qft16(args):
     f_dt = (time.perf_counter() - t) / reps
     fused128 = {"circuits_per_s": 1.0 / f_dt, "ms_per_run": f_dt * 1e3,
                 "max_abs_err_vs_ifft": float(numpy.max(numpy.abs(f_state - expect)))}
-    # the circuit's most frequent kernel: 120 controlled phases (k_diag on the quarter of the state with
-    # both qubits 1), launched back to back between HIP events on a state of the same size
+    # the circuit's most frequent gate: 120 controlled phases, applied back to back between HIP events on a
+    # state of the same size (the engine applies each run of them sharing a control in one k_diag_chain
+    # sweep over the half of the state where that control is 1)
     be = GpuBackend.make_state(n, [list(range(n))], [psi])
     ev = HipEvents()
     from qip_amd import CMat
@@ -482,6 +483,7 @@ def bench_qft16(args):
     ev.record(a, be.stream)
     for d in descs:
         be.kronselect_dot(d)
+    be.flush()
     ev.record(b, be.stream)
     diag_ms = ev.elapsed_ms(a, b) / len(descs)
     be.close()
@@ -497,7 +499,7 @@ def bench_qft16(args):
     out = _line("16-qubit QFT circuits/s (graph run incl. state upload and read-back)", 1.0 / dt, "circuits/s", reps,
                 warm, dt * 1e3, "c128", cfg, max_abs_err_vs_ifft=err, max_abs_err_probabilities=perr,
                 ms_first_run_walking_graph=walk_ms, fused_c128=fused128,
-                roofline={"bound": "launch", "kernel": "k_diag<double,0> (controlled phase, 120 of the 144 gates)",
+                roofline={"bound": "launch", "kernel": "k_diag_chain<double> (controlled phases, 120 of the 144 gates, per gate)",
                           "achieved": diag_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": diag_gbs / HBM_PEAK_GBS,
                           "traffic": None, "bytes_per_launch": diag_bytes, "avg_launch_ms": diag_ms,
                           "note": "a 1 MiB state stays in cache: each launch is bound by launch latency, not HBM"})

@@ -102,7 +102,14 @@ int qipgpu_apply_kron(qipgpu_state *s, int32_t n_groups, const int32_t *group_le
 #define QIPGPU_MAX_PRODUCT 12
 int qipgpu_apply_product(qipgpu_state *s, int32_t k, const int32_t *targets, const double *mats);
 
-/* Diagonal operator diag(d) on `targets`, conditioned on `controls` (k <= QIPGPU_MAX_DENSE). */
+/* Diagonal operator diag(d) on `targets`, conditioned on `controls` (k <= QIPGPU_MAX_DENSE).
+ * Held-back diagonals: an operator that is diagonal on at most 2 targets once its controls are
+ * found (from here or from qipgpu_apply_matrix: CZ, controlled phases, T, S, Rz) is not launched
+ * at once. Up to 16 such gates in a row wait on the state, and the next operation of any other
+ * kind (or qipgpu_synchronize, qipgpu_fused_flush, any read) first applies them, in order. They
+ * go in one sweep over the amplitudes whose bits are 1 on every control the gates share, or as
+ * separate sweeps where those would move fewer bytes. Each amplitude is multiplied by each gate's
+ * factor in turn and rounded after each, so the state is bit for bit what separate launches give. */
 int qipgpu_apply_diagonal(qipgpu_state *s, int32_t k, const int32_t *targets, const double *diag,
                           int32_t n_ctrl, const int32_t *controls);
 
@@ -249,7 +256,8 @@ int qipgpu_fused_record(qipgpu_state *s, int32_t n_gates, const int32_t *gate_k,
 /* Fusion width (1..6, default 6) and streaming (1, default) or deferred (0) launches; launches what
  * is recorded first. */
 int qipgpu_fused_set_recording(qipgpu_state *s, int32_t width, int32_t streaming);
-/* Compose and launch every recorded gate (the blocks back to back). */
+/* Compose and launch every recorded gate (the blocks back to back), after the held-back diagonal
+ * gates (qipgpu_apply_diagonal); on any state, launches what the engine holds back. */
 int qipgpu_fused_flush(qipgpu_state *s);
 /* Recorded gates not launched yet, and blocks launched since the first recording on this state. */
 int qipgpu_fused_recorded(const qipgpu_state *s, int64_t *pending_gates, int64_t *blocks_launched);

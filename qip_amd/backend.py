@@ -579,6 +579,11 @@ class GpuBackend(StateType):
     def synchronize(self) -> None:
         _capi.check(self._lib.qipgpu_synchronize(self._h), "synchronize")
 
+    def flush(self) -> None:
+        """Launch what the engine holds back on this state, without waiting for it: diagonal gates
+        on up to two targets wait for the next other operation, so that a run of them is one sweep."""
+        _capi.check(self._lib.qipgpu_fused_flush(self._h), "flush")
+
     def close(self):
         h, self._h = getattr(self, "_h", None), None
         if h:

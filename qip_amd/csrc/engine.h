@@ -56,6 +56,17 @@ constexpr int kProductHighPerPass = 8;
 constexpr int kRingSlots = 8;
 constexpr size_t kRingSlotBytes = 64 * 64 * 16 + 4096;  // a 64x64 complex128 matrix + offsets
 
+// A diagonal gate on at most two targets, held back on the state (qipgpu_state::chain) so that a run
+// of them is applied by one kernel: the factor d[j] multiplies the amplitudes whose ctrl_mask bits are
+// all 1, j being the value of state bits tpos[0] (MSB) .. tpos[k-1].
+constexpr int kDiagChainMax = 16;
+struct ChainGate {
+    uint64_t ctrl_mask;
+    cx<double> d[4];
+    uint8_t k;
+    uint8_t tpos[2];
+};
+
 }  // namespace qipgpu
 
 struct qipgpu_state {
@@ -107,6 +118,9 @@ struct qipgpu_state {
     hipGraphNode_t rec_last = nullptr;
     // gates recorded for online fusion and not launched yet (complex64 states)
     qipgpu::Recorder *recorder = nullptr;
+    // diagonal gates held back (flush_diag_chain applies them before anything else touches the state)
+    qipgpu::ChainGate chain[qipgpu::kDiagChainMax];
+    int32_t chain_len = 0;
 };
 
 namespace qipgpu {
@@ -159,7 +173,11 @@ int device_cus();
 int check_qubits(const qipgpu_state *s, int32_t k, const int32_t *q, const char *what);
 // Reserve a staging ring slot, copy `bytes` from `src` to the device, return device pointer.
 int ring_upload(qipgpu_state *s, const void *src, size_t bytes, void **dev);
-// Launch every recorded gate (recorder.hip); a no-op while the recorder itself is launching.
+// Apply the diagonal gates held back on the state, in order (engine.hip): every entry point that
+// touches the state does so first, through flush_recorded.
+int flush_diag_chain(qipgpu_state *s);
+// Launch the held-back diagonal gates, then every recorded gate (recorder.hip); the recorded gates are
+// left alone while the recorder itself is launching.
 int flush_recorded(qipgpu_state *s);
 // While s is capturing: mark the capture refused and return QIPGPU_EUNSUP naming `what`.
 int refuse_in_capture(qipgpu_state *s, const char *what);

@@ -479,6 +479,9 @@ int qipgpu_capture_begin(qipgpu_state *s) {
 int qipgpu_capture_end(qipgpu_state *s, qipgpu_graph **out) {
     *out = nullptr;
     if (!s->capturing) return set_error(QIPGPU_EINVAL, "the state is not capturing");
+    // diagonal gates still held back belong to the recording: their node goes last
+    if (!s->capture_refused && flush_diag_chain(s) != QIPGPU_OK) s->capture_refused = true;
+    s->chain_len = 0;
     s->capturing = false;
     hipGraph_t g = s->rec_graph;
     s->rec_graph = nullptr;
@@ -559,7 +562,9 @@ int qipgpu_get_stream(qipgpu_state *s, void **stream) {
 int qipgpu_init_product(qipgpu_state *s, int32_t n_groups, const int32_t *group_len,
                         const int32_t *group_idx, const double *feeds) {
     QIP_NOT_CAPTURING(s, "a state initialisation");
-    s->fused_prev_count = 0;  // every amplitude is rewritten, so recorded gates and the factor are moot
+    // every amplitude is rewritten, so recorded gates, held-back diagonals and the factor are moot
+    s->fused_prev_count = 0;
+    s->chain_len = 0;
     recorder_discard(s);
     if (n_groups < 0) return set_error(QIPGPU_EINVAL, "negative group count");
     uint64_t fed = 0;
@@ -925,6 +930,124 @@ static int launch_diag(qipgpu_state *s, const std::vector<int> &tbits, uint64_t 
     return launch_gate(s, &k_diag<R, K, ITEMS>, dim3(capped_grid(nb)), dim3(256), a);
 }
 
+// A run of held-back diagonal gates in one sweep over the amplitudes whose bits common_mask (the
+// controls every gate of the run has) are 1. Each amplitude meets the gates in order and is rounded
+// after each one, with k_diag's arithmetic, so the result is bit for bit that of launching the gates
+// one by one.
+template <typename R>
+struct DiagChainArgs {
+    cx<R> *amps;
+    uint64_t count;
+    uint64_t common_mask;
+    PosList pl;          // common control positions
+    int32_t ngates;
+    ChainGate g[kDiagChainMax];
+    BlockOrder order;
+    uint32_t nblocks;
+};
+
+template <typename R, int ITEMS>
+__global__ __launch_bounds__(256) void k_diag_chain(const DiagChainArgs<R> a) {
+    for (uint32_t blk = blockIdx.x; blk < a.nblocks; blk += gridDim.x) {
+    const uint64_t g0 = ordered_block(blk, a.order) * (256 * ITEMS) + threadIdx.x;
+    cx<R> v[ITEMS];
+    uint64_t idx[ITEMS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+        const uint64_t g = g0 + (uint64_t)it * 256;
+        idx[it] = insert_zeros(g < a.count ? g : a.count - 1, a.pl) | a.common_mask;
+        v[it] = a.amps[idx[it]];
+    }
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+        if (g0 + (uint64_t)it * 256 < a.count) {
+            cx<R> x = v[it];
+            bool touched = false;
+            for (int q = 0; q < a.ngates; ++q) {
+                if ((idx[it] & a.g[q].ctrl_mask) != a.g[q].ctrl_mask) continue;
+                uint32_t j = 0;
+                for (int b = 0; b < a.g[q].k; ++b) j = (j << 1) | (uint32_t)((idx[it] >> a.g[q].tpos[b]) & 1ull);
+                x = narrow<R>(cmul(widen(x), a.g[q].d[j]));
+                touched = true;
+            }
+            if (touched) a.amps[idx[it]] = x;
+        }
+    }
+    }
+}
+
+// Amplitude-sized cost of one launch (about 4 us of HBM time) when choosing between one sweep for a
+// run of diagonal gates and a launch per gate.
+constexpr uint64_t kLaunchCostAmps = 1ull << 19;
+
+template <typename R>
+static int launch_chain_gate(qipgpu_state *s, const ChainGate &c) {
+    std::vector<int> tbits, cpos;
+    for (int b = 0; b < c.k; ++b) tbits.push_back(c.tpos[b]);
+    for (int p = 0; p < 64; ++p)
+        if (c.ctrl_mask >> p & 1) cpos.push_back(p);
+    std::vector<cd> d((size_t)1 << c.k);
+    for (size_t j = 0; j < d.size(); ++j) d[j] = cd(c.d[j].re, c.d[j].im);
+    switch (c.k) {
+        case 0: return launch_diag<R, 0>(s, tbits, c.ctrl_mask, cpos, d);
+        case 1: return launch_diag<R, 1>(s, tbits, c.ctrl_mask, cpos, d);
+        default: return launch_diag<R, 2>(s, tbits, c.ctrl_mask, cpos, d);
+    }
+}
+
+template <typename R>
+static int flush_chain_typed(qipgpu_state *s) {
+    const int m = s->chain_len;
+    s->chain_len = 0;
+    uint64_t common = ~0ull, separate = 0;
+    for (int q = 0; q < m; ++q) {
+        common &= s->chain[q].ctrl_mask;
+        separate += (s->len >> __builtin_popcountll(s->chain[q].ctrl_mask)) + kLaunchCostAmps;
+    }
+    const uint64_t swept = (s->len >> __builtin_popcountll(common)) + kLaunchCostAmps;
+    if (m == 1 || swept > separate) {   // one gate, or gates on small disjoint subspaces of a large state
+        for (int q = 0; q < m; ++q) {
+            const int rc = launch_chain_gate<R>(s, s->chain[q]);
+            if (rc) return rc;
+        }
+        return QIPGPU_OK;
+    }
+    DiagChainArgs<R> a;
+    std::memset(&a, 0, sizeof(a));
+    a.amps = (cx<R> *)s->amps;
+    a.common_mask = common;
+    std::vector<int> cpos;
+    for (int p = 0; p < s->n; ++p)
+        if (common >> p & 1) cpos.push_back(p);
+    a.count = s->len >> cpos.size();
+    fill_poslist(a.pl, cpos);
+    a.ngates = m;
+    for (int q = 0; q < m; ++q) a.g[q] = s->chain[q];
+    constexpr int ITEMS = 4;
+    const uint32_t nb = blocks_for(a.count, 256 * ITEMS);
+    a.order = block_order(s->len * s->amp_bytes, s->amp_bytes, nb);
+    a.nblocks = nb;
+    return launch_gate(s, &k_diag_chain<R, ITEMS>, dim3(capped_grid(nb)), dim3(256), a);
+}
+
+// Hold a diagonal gate (k <= 2 targets) back on the state; a full run is applied first.
+static int hold_diagonal(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mask, const std::vector<cd> &d) {
+    if (s->chain_len == kDiagChainMax) {
+        const int rc = flush_diag_chain(s);
+        if (rc) return rc;
+    }
+    ChainGate &c = s->chain[s->chain_len++];
+    std::memset(&c, 0, sizeof(c));
+    c.ctrl_mask = ctrl_mask;
+    c.k = (uint8_t)tbits.size();
+    for (size_t b = 0; b < tbits.size(); ++b) c.tpos[b] = (uint8_t)tbits[b];
+    for (size_t j = 0; j < d.size(); ++j) {
+        c.d[j].re = d[j].real();
+        c.d[j].im = d[j].imag();
+    }
+    return QIPGPU_OK;
+}
+
 template <typename R>
 static int dispatch(qipgpu_state *s, const Analysed &an, const std::vector<int32_t> &controls_in) {
     const int n = s->n;
@@ -949,6 +1072,9 @@ static int dispatch(qipgpu_state *s, const Analysed &an, const std::vector<int32
             identity = identity && d[j] == 1.0;
         }
         if (identity) return QIPGPU_OK;
+        if (k <= 2) return hold_diagonal(s, tbits, ctrl_mask, d);
+        const int rc = flush_diag_chain(s);
+        if (rc) return rc;
         switch (k) {
             case 0: return launch_diag<R, 0>(s, tbits, ctrl_mask, cpos, d);
             case 1: return launch_diag<R, 1>(s, tbits, ctrl_mask, cpos, d);
@@ -961,6 +1087,8 @@ static int dispatch(qipgpu_state *s, const Analysed &an, const std::vector<int32
         }
         return diag_large(s, tbits, ctrl_mask, cpos, d.data());
     }
+    const int rc = flush_diag_chain(s);
+    if (rc) return rc;
     switch (k) {
         case 1: return launch_dense_reg<R, 1>(s, tbits, ctrl_mask, allpos, an.U);
         case 2: return launch_dense_reg<R, 2>(s, tbits, ctrl_mask, allpos, an.U);
@@ -978,6 +1106,15 @@ static int check_disjoint(const qipgpu_state *s, int32_t k, const int32_t *t, in
 
 }  // namespace
 
+namespace qipgpu {
+
+int flush_diag_chain(qipgpu_state *s) {
+    if (!s->chain_len) return QIPGPU_OK;
+    return s->dtype == QIPGPU_C128 ? flush_chain_typed<double>(s) : flush_chain_typed<float>(s);
+}
+
+}  // namespace qipgpu
+
 extern "C" {
 
 int qipgpu_apply_matrix(qipgpu_state *s, int32_t k, const int32_t *targets, const double *mat, int32_t n_ctrl,
@@ -987,8 +1124,9 @@ int qipgpu_apply_matrix(qipgpu_state *s, int32_t k, const int32_t *targets, cons
     int rc = check_disjoint(s, k, targets, n_ctrl, controls);
     if (rc) return rc;
     // linear in the state: a pending fused renormalisation (a scalar) stays pending; recorded gates
-    // go first
-    rc = flush_recorded(s);
+    // go first (with the diagonal gates held back before them); held-back diagonal gates otherwise
+    // stay held, and dispatch applies them before anything that is not held
+    rc = recorder_pending(s) ? flush_recorded(s) : QIPGPU_OK;
     if (rc) return rc;
     Analysed an = analyse(k, targets, mat);
     std::vector<int32_t> ctl(controls, controls + n_ctrl);
@@ -1000,9 +1138,8 @@ int qipgpu_apply_diagonal(qipgpu_state *s, int32_t k, const int32_t *targets, co
     if (k < 0 || k > QIPGPU_MAX_DENSE) return set_error(QIPGPU_EINVAL, "diagonal on %d qubits", k);
     int rc = check_disjoint(s, k, targets, n_ctrl, controls);
     if (rc) return rc;
-    // linear in the state: a pending fused renormalisation (a scalar) stays pending; recorded gates
-    // go first
-    rc = flush_recorded(s);
+    // as in qipgpu_apply_matrix
+    rc = recorder_pending(s) ? flush_recorded(s) : QIPGPU_OK;
     if (rc) return rc;
     Analysed an;
     an.targets.assign(targets, targets + k);

@@ -384,12 +384,16 @@ void simulate_recorder(int32_t n, int32_t width, bool streaming, bool use_launch
 }  // namespace
 
 int flush_recorded(qipgpu_state *s) {
+    int rc = flush_diag_chain(s);   // held-back diagonal gates came first
+    if (rc) return rc;
     Recorder *r = s->recorder;
     if (!r || r->in_flush || r->k.empty()) return QIPGPU_OK;
     std::vector<PlannedBlock> blocks;
     plan_recorded(*r, false, blocks);
     r->since_plan = 0;
-    return launch_planned(s, blocks, (int64_t)blocks.size());
+    rc = launch_planned(s, blocks, (int64_t)blocks.size());
+    if (rc) return rc;
+    return flush_diag_chain(s);     // a complex128 block that is diagonal is held like any such gate
 }
 
 void recorder_discard(qipgpu_state *s) {
@@ -425,7 +429,9 @@ int qipgpu_fused_set_recording(qipgpu_state *s, int32_t width, int32_t streaming
 int qipgpu_fused_record(qipgpu_state *s, int32_t n_gates, const int32_t *gate_k, const int32_t *gate_qubits,
                         const double *mats) {
     QIP_NOT_CAPTURING(s, "gate recording for fusion");
-    int rc = ensure_recorder(s);
+    int rc = flush_diag_chain(s);   // diagonal gates held back come before what is recorded now
+    if (rc) return rc;
+    rc = ensure_recorder(s);
     if (rc) return rc;
     if (n_gates < 0) return set_error(QIPGPU_EINVAL, "negative gate count");
     if (n_gates > 0 && (!gate_k || !gate_qubits || !mats)) return set_error(QIPGPU_EINVAL, "null argument");
