@@ -11,7 +11,8 @@ renormalisation, synchronise. `value` = 1780 * steps / timed seconds.
 
 Beside it, in the same run:
   * the plain complex128 GpuBackend (the drop-in) on the same circuit, once, with its gates/s,
-    algorithmic GB/s and the HBM roofline of its dominant kernel (single-qubit dense sweep);
+    effective GB/s (the gates' algorithmic bytes over the time) and the HBM roofline of its dominant
+    kernel (k_tile_run: a held run of single-qubit gates in one pass, each gate applied by itself);
   * the complex128 fused form (Fused128Backend: the same blocks applied in FP64) on the same circuit,
     once, with its gates/s and its difference from the plain complex128 result;
   * 1 - F and max |amplitude difference| of the fused result against the complex128 result;
@@ -46,6 +47,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, published
+EFFECTIVE_NOTE = ("the gates' algorithmic bytes as written (2 S 2^(n-c) per gate) over the run time; held runs "
+                  "of single-qubit or diagonal gates share HBM passes, so this can exceed the HBM peak")
 MFMA_BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 matrix, published
 FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 (vector and matrix), published
 
@@ -66,15 +69,6 @@ FUSED6_PMC_30Q = {"fetch_kib": 4196076.3, "fetch_scale": 1048576.0 / 524314.6,
 # the 1 GiB copy, WRITE_SIZE 1048576 KiB). Commands and figures: profiles/k_marg_pass_measure28.txt.
 MARG_PMC_28Q = {"fetch_kib": 2097182.76, "fetch_scale": 1048576.0 / 524313.75,
                 "write_kib": 3026.64, "write_scale": 1.0}
-
-
-# HBM traffic of one single-qubit sweep (k_dense<double,1,1>, one base per thread) on a 28-qubit
-# complex128 state (4 GiB), measured the same way by two rocprofv3 --pmc runs of `bench.py --workload
-# circuit28 --no-cpu-baseline --copy-calibration-gib 1` (mean over the 619 one-qubit sweeps of the run;
-# FETCH_SIZE reads 524316.1 KiB for the 1 GiB copy, WRITE_SIZE 1048576 KiB). Commands and figures:
-# profiles/k_dense_circuit28_depth20.txt.
-DENSE1_PMC_28Q = {"fetch_kib": 2097166.0, "fetch_scale": 1048576.0 / 524316.1,
-                  "write_kib": 4194304.3, "write_scale": 1.0}
 
 
 def pmc_bytes(p) -> int:
@@ -168,22 +162,32 @@ def fused_pattern_floor_ms(be, ops, ev):
 
 
 def plain_dominant_roofline(be, n, ev, amp: int = 16):
-    """Average duration of the plain single-qubit dense sweep (the plain path's dominant kernel:
-    1200 of 1780 gates) over one gate per qubit, from HIP events on the state's stream."""
+    """Average duration of the plain path's dominant kernel from HIP events on the state's stream:
+    k_tile_run, one HBM pass that applies a held run of single-qubit gates (here 8 Haar gates on the
+    8 highest state bits, the run a 28- or 30-qubit layer is cut into), each gate by itself on the
+    tile in LDS. Algorithmic bytes per launch: one read and one write of the state."""
     import numpy
-    from qip_amd.workloads import random_product_unitaries
-    ops = random_product_unitaries(n, seed=77)
+    from qip_amd.workloads import haar_2x2
+    rng = numpy.random.default_rng(77)
+    run = [{q: haar_2x2(rng)} for q in range(8)]
+    reps = 5
+    for d in run:                                  # warm-up pass
+        be.kronselect_dot(d)
+    be.flush()
     be.synchronize()
     a, b = ev.create(), ev.create()
     ev.record(a, be.stream)
-    for d in ops:
-        be.kronselect_dot(d)
+    for _ in range(reps):
+        for d in run:
+            be.kronselect_dot(d)
+        be.flush()
     ev.record(b, be.stream)
-    ms = ev.elapsed_ms(a, b) / len(ops)
+    ms = ev.elapsed_ms(a, b) / reps
     bytes_per = 2 * amp * 2 ** n
     kind = "double, complex128" if amp == 16 else "float, complex64"
-    return {"kernel": "k_dense<{}> (single-qubit dense sweep)".format(kind), "avg_ms": ms,
-            "bytes_per_launch": bytes_per, "achieved_gbs": bytes_per / (ms * 1e-3) / 1e9}
+    return {"kernel": "k_tile_run<{}> (one pass, 8 held single-qubit gates)".format(kind), "avg_ms": ms,
+            "bytes_per_launch": bytes_per, "gates_per_launch": len(run),
+            "achieved_gbs": bytes_per / (ms * 1e-3) / 1e9}
 
 
 def cpu_baseline(ops, n, seconds: float):
@@ -245,11 +249,11 @@ def _line(metric, value, unit, steps, warmup, ms, dtype, config, **extra):
 
 
 def bench_circuit28(args):
-    """28-qubit, 20-layer random circuit: plain complex128 vs plain complex64 (unfused), one sweep
-    per gate each; 1 - F of complex64 against complex128."""
+    """28-qubit, 20-layer random circuit: plain complex128 vs plain complex64 (unfused: every gate
+    applied by itself, with the bits of a sweep per gate); 1 - F of complex64 against complex128."""
     import numpy
     from qip_amd import GpuBackend, infidelity, max_abs_diff
-    from qip_amd.workloads import algorithmic_bytes_plain, layered_random_circuit, touched_bytes_plain
+    from qip_amd.workloads import algorithmic_bytes_plain, layered_random_circuit
     n, depth = 28, 20
     ops = layered_random_circuit(n, depth, args.seed)
     res = {}
@@ -266,9 +270,8 @@ def bench_circuit28(args):
         be.synchronize()
         dt = time.perf_counter() - t
         gb = algorithmic_bytes_plain(ops, n, amp) / 1e9
-        res[name] = {"gates_per_s": len(ops) / dt, "s_per_circuit": dt, "algorithmic_gbs": gb / dt,
-                     "hbm_frac": gb / dt / HBM_PEAK_GBS,
-                     "touched_gbs": touched_bytes_plain(ops, n, amp) / 1e9 / dt}
+        res[name] = {"gates_per_s": len(ops) / dt, "s_per_circuit": dt, "effective_gbs": gb / dt,
+                     "effective_gbs_note": EFFECTIVE_NOTE}
         states[name] = be
     res["infidelity_c64_vs_c128"] = infidelity(states["c128"], states["c64"])
     res["max_abs_err_c64_vs_c128"] = max_abs_diff(states["c128"], states["c64"])
@@ -276,8 +279,10 @@ def bench_circuit28(args):
     dom = {name: plain_dominant_roofline(states[name], n, ev, amp) for name, amp in (("c128", 16), ("c64", 8))}
     for be in states.values():
         be.close()
-    cfg = {"workload": "28-qubit depth-20 layered random circuit, plain complex128 vs plain complex64 (one sweep "
-                       "per gate)", "n_qubits": n, "depth": depth, "gates": len(ops), "seed": args.seed}
+    cfg = {"workload": "28-qubit depth-20 layered random circuit, plain complex128 vs plain complex64 (unfused: "
+                       "each gate applied by itself, bit for bit a sweep per gate; held runs of single-qubit gates "
+                       "share one HBM pass per tile, runs of diagonal gates one sweep)", "n_qubits": n,
+           "depth": depth, "gates": len(ops), "seed": args.seed}
     d = dom["c128"]
     out = _line("28-qubit gates/s, plain c128 vs unfused c64", res["c128"]["gates_per_s"], "gates/s", 1, 1,
                 res["c128"]["s_per_circuit"] * 1e3, "c128", cfg, plain_c128=res["c128"], plain_c64=res["c64"],
@@ -285,8 +290,7 @@ def bench_circuit28(args):
                 max_abs_err_c64_vs_c128=res["max_abs_err_c64_vs_c128"],
                 roofline={"bound": "hbm", "kernel": d["kernel"], "achieved": d["achieved_gbs"], "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": d["achieved_gbs"] / HBM_PEAK_GBS,
-                          "traffic": pmc_bytes(DENSE1_PMC_28Q) if n == 28 else None,
-                          "bytes_per_launch": d["bytes_per_launch"], "avg_launch_ms": d["avg_ms"]},
+                          "traffic": None, "bytes_per_launch": d["bytes_per_launch"], "avg_launch_ms": d["avg_ms"]},
                 roofline_c64=dict(dom["c64"], frac=dom["c64"]["achieved_gbs"] / HBM_PEAK_GBS))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ops, n, args.cpu_seconds)
@@ -960,7 +964,7 @@ def bench_circuit30(args):
     import numpy
     from qip_amd import GpuBackend, infidelity, max_abs_diff
     from qip_amd.fused import FusedBackend
-    from qip_amd.workloads import algorithmic_bytes_plain, gate_counts, layered_random_circuit, touched_bytes_plain
+    from qip_amd.workloads import algorithmic_bytes_plain, gate_counts, layered_random_circuit
 
     n, depth = args.n, args.depth
     ops = layered_random_circuit(n, depth, args.seed)
@@ -1058,8 +1062,7 @@ def bench_circuit30(args):
         dom = plain_dominant_roofline(pl, n, ev)     # applies further gates to pl: after every comparison
         result["plain_c128"] = {
             "gates_per_s": ngates / plain_s, "s_per_circuit": plain_s,
-            "algorithmic_gbs": plain_bytes / plain_s / 1e9, "hbm_frac": plain_bytes / plain_s / 1e9 / HBM_PEAK_GBS,
-            "touched_gbs": touched_bytes_plain(ops, n, 16) / plain_s / 1e9,
+            "effective_gbs": plain_bytes / plain_s / 1e9, "effective_gbs_note": EFFECTIVE_NOTE,
             "dominant_kernel": dict(dom, frac=dom["achieved_gbs"] / HBM_PEAK_GBS),
         }
         result["speedup_fused_vs_plain"] = plain_s / (elapsed / args.steps)

@@ -103,13 +103,19 @@ int qipgpu_apply_kron(qipgpu_state *s, int32_t n_groups, const int32_t *group_le
 int qipgpu_apply_product(qipgpu_state *s, int32_t k, const int32_t *targets, const double *mats);
 
 /* Diagonal operator diag(d) on `targets`, conditioned on `controls` (k <= QIPGPU_MAX_DENSE).
- * Held-back diagonals: an operator that is diagonal on at most 2 targets once its controls are
- * found (from here or from qipgpu_apply_matrix: CZ, controlled phases, T, S, Rz) is not launched
- * at once. Up to 16 such gates in a row wait on the state, and the next operation of any other
- * kind (or qipgpu_synchronize, qipgpu_fused_flush, any read) first applies them, in order. They
- * go in one sweep over the amplitudes whose bits are 1 on every control the gates share, or as
- * separate sweeps where those would move fewer bytes. Each amplitude is multiplied by each gate's
- * factor in turn and rounded after each, so the state is bit for bit what separate launches give. */
+ * Held-back gates (this entry point and qipgpu_apply_matrix), once controls are found:
+ *   - an operator diagonal on at most 2 targets (CZ, controlled phases, T, S, Rz) joins a run of
+ *     up to 16 such gates, applied in one sweep over the amplitudes whose bits are 1 on every
+ *     control the run shares: each amplitude gets each gate's factor in turn, rounded after each;
+ *   - on a state of 12 qubits or more, a dense single-qubit operator (H, X, CNOT, Toffoli, any
+ *     2x2 under any controls) with at most 8 of its bits above state bit 3 joins a run of up to 32
+ *     such gates whose bits, with state bits 0..3, fit one tile of 2^12 amplitudes: a tile is read
+ *     once, the gates are applied to it one after the other in on-chip memory with the arithmetic
+ *     of their own sweeps (FP64 from zero, rounded to the state's precision after each gate), and
+ *     the tile is written once.
+ * A run waits until a gate that does not fit it or any other call on the state (qipgpu_synchronize,
+ * qipgpu_fused_flush, any read) applies it first, in order. A run goes gate by gate where separate
+ * sweeps would move fewer bytes. The state is bit for bit what one launch per gate gives. */
 int qipgpu_apply_diagonal(qipgpu_state *s, int32_t k, const int32_t *targets, const double *diag,
                           int32_t n_ctrl, const int32_t *controls);
 

@@ -580,8 +580,9 @@ class GpuBackend(StateType):
         _capi.check(self._lib.qipgpu_synchronize(self._h), "synchronize")
 
     def flush(self) -> None:
-        """Launch what the engine holds back on this state, without waiting for it: diagonal gates
-        on up to two targets wait for the next other operation, so that a run of them is one sweep."""
+        """Launch what the engine holds back on this state, without waiting for it: runs of diagonal
+        gates and of single-qubit gates wait for the next operation that does not join them, so that
+        a run is one sweep (diagonals) or one pass per tile (single-qubit gates)."""
         _capi.check(self._lib.qipgpu_fused_flush(self._h), "flush")
 
     def close(self):

@@ -67,6 +67,16 @@ struct ChainGate {
     uint8_t tpos[2];
 };
 
+// A single-qubit gate (2x2 U on state bit tpos where the bits of ctrl_mask are 1), held back on the
+// state (qipgpu_state::run) so that a run of them whose bits share one 4096-amplitude tile is applied
+// in one pass, each gate in turn on the tile in LDS.
+constexpr int kTileRunMax = 32;
+struct RunGate {
+    uint64_t ctrl_mask;
+    cx<double> U[4];
+    uint8_t tpos;
+};
+
 }  // namespace qipgpu
 
 struct qipgpu_state {
@@ -118,9 +128,13 @@ struct qipgpu_state {
     hipGraphNode_t rec_last = nullptr;
     // gates recorded for online fusion and not launched yet (complex64 states)
     qipgpu::Recorder *recorder = nullptr;
-    // diagonal gates held back (flush_diag_chain applies them before anything else touches the state)
+    // gates held back (flush_held applies them before anything else touches the state): a run of
+    // diagonal gates or a run of single-qubit gates, never both at once
     qipgpu::ChainGate chain[qipgpu::kDiagChainMax];
     int32_t chain_len = 0;
+    qipgpu::RunGate run[qipgpu::kTileRunMax];
+    int32_t run_len = 0;
+    uint64_t run_bits = 0;     // target and control bits of the held single-qubit gates
 };
 
 namespace qipgpu {
@@ -173,10 +187,10 @@ int device_cus();
 int check_qubits(const qipgpu_state *s, int32_t k, const int32_t *q, const char *what);
 // Reserve a staging ring slot, copy `bytes` from `src` to the device, return device pointer.
 int ring_upload(qipgpu_state *s, const void *src, size_t bytes, void **dev);
-// Apply the diagonal gates held back on the state, in order (engine.hip): every entry point that
-// touches the state does so first, through flush_recorded.
-int flush_diag_chain(qipgpu_state *s);
-// Launch the held-back diagonal gates, then every recorded gate (recorder.hip); the recorded gates are
+// Apply the gates held back on the state, in order (engine.hip): every entry point that touches the
+// state does so first, through flush_recorded.
+int flush_held(qipgpu_state *s);
+// Apply the held-back gates, then launch every recorded gate (recorder.hip); the recorded gates are
 // left alone while the recorder itself is launching.
 int flush_recorded(qipgpu_state *s);
 // While s is capturing: mark the capture refused and return QIPGPU_EUNSUP naming `what`.

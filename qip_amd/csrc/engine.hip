@@ -479,9 +479,11 @@ int qipgpu_capture_begin(qipgpu_state *s) {
 int qipgpu_capture_end(qipgpu_state *s, qipgpu_graph **out) {
     *out = nullptr;
     if (!s->capturing) return set_error(QIPGPU_EINVAL, "the state is not capturing");
-    // diagonal gates still held back belong to the recording: their node goes last
-    if (!s->capture_refused && flush_diag_chain(s) != QIPGPU_OK) s->capture_refused = true;
+    // gates still held back belong to the recording: their node goes last
+    if (!s->capture_refused && flush_held(s) != QIPGPU_OK) s->capture_refused = true;
     s->chain_len = 0;
+    s->run_len = 0;
+    s->run_bits = 0;
     s->capturing = false;
     hipGraph_t g = s->rec_graph;
     s->rec_graph = nullptr;
@@ -565,6 +567,8 @@ int qipgpu_init_product(qipgpu_state *s, int32_t n_groups, const int32_t *group_
     // every amplitude is rewritten, so recorded gates, held-back diagonals and the factor are moot
     s->fused_prev_count = 0;
     s->chain_len = 0;
+    s->run_len = 0;
+    s->run_bits = 0;
     recorder_discard(s);
     if (n_groups < 0) return set_error(QIPGPU_EINVAL, "negative group count");
     uint64_t fed = 0;
@@ -1030,10 +1034,184 @@ static int flush_chain_typed(qipgpu_state *s) {
     return launch_gate(s, &k_diag_chain<R, ITEMS>, dim3(capped_grid(nb)), dim3(256), a);
 }
 
-// Hold a diagonal gate (k <= 2 targets) back on the state; a full run is applied first.
+// A run of held-back single-qubit gates whose target and control bits all lie in one tile of 4096
+// amplitudes (tile bits P: those bits, state bits 0..3 and free bits, ascending; u bit i <-> state
+// bit P[i]). A workgroup loads a tile into LDS and applies the gates to it one after the other, each
+// to every pair of the tile with k_dense's arithmetic (FP64 from zero, rounded to the state's
+// precision), then stores the tile: the bits of a launch per gate, for one HBM pass.
+struct RunSlot {
+    cx<double> U[4];
+    uint32_t cmask;     // control bits, in tile (u) coordinates
+    uint32_t tb;        // target bit, in tile coordinates
+};
+
+template <typename R>
+struct TileRunArgs {
+    cx<R> *amps;
+    uint64_t ntiles;
+    int32_t ngates;
+    uint8_t pbits[12];
+    RunSlot g[kTileRunMax];
+};
+
+template <typename R>
+__device__ __forceinline__ uint64_t run_tile_base(uint64_t t, const TileRunArgs<R> &a) {
+#pragma unroll
+    for (int i = 0; i < 12; ++i) {
+        const uint32_t p = a.pbits[i];
+        const uint64_t lo = t & ((1ull << p) - 1ull);
+        t = ((t ^ lo) << 1) | lo;
+    }
+    return t;
+}
+
+// Thread t holds the tile's amplitudes u = t + 256 i, i = 0..15: u bits 0..7 are t, bits 8..11 are i.
+template <typename R>
+__device__ __forceinline__ uint64_t run_off_hi(int i, const TileRunArgs<R> &a) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) o |= (uint64_t)((i >> b) & 1) << a.pbits[8 + b];
+    return o;
+}
+
+template <typename R>
+__device__ __forceinline__ void run_tile(uint64_t base, cx<R> *t, const TileRunArgs<R> &a) {
+    const uint32_t tid = threadIdx.x;
+    __syncthreads();  // the previous tile has left LDS
+#pragma unroll
+    for (int i = 0; i < 16; ++i) t[tid + 256u * i] = a.amps[base + run_off_hi(i, a)];
+    __syncthreads();
+    for (int gi = 0; gi < a.ngates; ++gi) {
+        const uint32_t tb = a.g[gi].tb, cm = a.g[gi].cmask;
+        const uint32_t lo = (1u << tb) - 1u;
+        cx<R> x0[8], x1[8];
+        uint32_t u0[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t p = tid + 256u * j;           // pair index 0..2047
+            u0[j] = ((p & ~lo) << 1) | (p & lo);
+            x0[j] = t[u0[j]];
+            x1[j] = t[u0[j] | (1u << tb)];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if ((u0[j] & cm) != cm) continue;
+            cx<double> acc0, acc1;
+            acc0.re = acc0.im = acc1.re = acc1.im = 0.0;
+            cfma(acc0, a.g[gi].U[0], widen(x0[j]));
+            cfma(acc0, a.g[gi].U[1], widen(x1[j]));
+            cfma(acc1, a.g[gi].U[2], widen(x0[j]));
+            cfma(acc1, a.g[gi].U[3], widen(x1[j]));
+            t[u0[j]] = narrow<R>(acc0);
+            t[u0[j] | (1u << tb)] = narrow<R>(acc1);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) a.amps[base + run_off_hi(i, a)] = t[tid + 256u * i];
+}
+
+// Contiguous tile ranges per workgroup; the other workgroups on the CU (two of them for complex128,
+// four for complex64, by LDS) keep HBM busy while one works on its tile.
+template <typename R>
+__global__ __launch_bounds__(256, 2) void k_tile_run(const TileRunArgs<R> a) {
+    __shared__ cx<R> t[4096];
+    const uint32_t tid = threadIdx.x;
+    uint64_t off_lo = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) off_lo |= (uint64_t)((tid >> b) & 1u) << a.pbits[b];
+    const uint64_t per = (a.ntiles + gridDim.x - 1) / gridDim.x;
+    const uint64_t first = (uint64_t)blockIdx.x * per;
+    const uint64_t end = min(first + per, a.ntiles);
+    for (uint64_t tile = first; tile < end; ++tile)   // the same trip count in every thread
+        run_tile(run_tile_base(tile, a) + off_lo, t, a);
+}
+
+template <typename R>
+static int launch_run_gate(qipgpu_state *s, const RunGate &g) {
+    std::vector<int> tbits(1, g.tpos), allpos;
+    for (int p = 0; p < 64; ++p)
+        if (g.ctrl_mask >> p & 1) allpos.push_back(p);
+    allpos.push_back(g.tpos);
+    std::vector<cd> U(4);
+    for (int i = 0; i < 4; ++i) U[i] = cd(g.U[i].re, g.U[i].im);
+    return launch_dense_reg<R, 1>(s, tbits, g.ctrl_mask, allpos, U);
+}
+
+template <typename R>
+static int flush_run_typed(qipgpu_state *s) {
+    const int m = s->run_len;
+    const uint64_t bits = s->run_bits;
+    s->run_len = 0;
+    s->run_bits = 0;
+    uint64_t separate = 0;
+    for (int q = 0; q < m; ++q) separate += (s->len >> __builtin_popcountll(s->run[q].ctrl_mask)) + kLaunchCostAmps;
+    if (m == 1 || s->len + kLaunchCostAmps > separate) {   // one gate, or gates on small subspaces
+        for (int q = 0; q < m; ++q) {
+            const int rc = launch_run_gate<R>(s, s->run[q]);
+            if (rc) return rc;
+        }
+        return QIPGPU_OK;
+    }
+    // tile bits: the gates' bits, state bits 0..3, then free bits in the order 4, 5, 12..17, 6..11, 18..
+    uint64_t pmask = bits | 0xfull;
+    int order[64], no = 0;
+    for (int b = 4; b < 6; ++b) order[no++] = b;
+    for (int b = 12; b < 18; ++b) order[no++] = b;
+    for (int b = 6; b < 12; ++b) order[no++] = b;
+    for (int b = 18; b < 64; ++b) order[no++] = b;
+    for (int i = 0; i < no && __builtin_popcountll(pmask) < 12; ++i)
+        if (order[i] < s->n) pmask |= 1ull << order[i];
+    TileRunArgs<R> a;
+    std::memset(&a, 0, sizeof(a));
+    int ub[64];
+    for (int p = 0, i = 0; p < 64; ++p)
+        if (pmask >> p & 1) {
+            a.pbits[i] = (uint8_t)p;
+            ub[p] = i++;
+        }
+    a.amps = (cx<R> *)s->amps;
+    a.ntiles = s->len >> 12;
+    a.ngates = m;
+    for (int q = 0; q < m; ++q) {
+        const RunGate &g = s->run[q];
+        for (int i = 0; i < 4; ++i) a.g[q].U[i] = g.U[i];
+        a.g[q].tb = (uint32_t)ub[g.tpos];
+        uint32_t cm = 0;
+        for (int p = 0; p < 64; ++p)
+            if (g.ctrl_mask >> p & 1) cm |= 1u << ub[p];
+        a.g[q].cmask = cm;
+    }
+    const uint64_t per_cu = sizeof(R) == 8 ? 2 : 4;   // 64 KB / 32 KB of LDS per workgroup
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(a.ntiles, (uint64_t)device_cus() * per_cu);
+    return launch_gate(s, &k_tile_run<R>, dim3(grid), dim3(256), a);
+}
+
+// Hold a single-qubit gate (target bit tpos, under ctrl_mask) back on the state: held diagonal gates
+// and a run it does not fit (more than 8 bits above bit 3, or full) are applied first.
+static int hold_single(qipgpu_state *s, int tpos, uint64_t ctrl_mask, const std::vector<cd> &U) {
+    const uint64_t gb = ctrl_mask | (1ull << tpos);
+    if (s->chain_len || s->run_len == kTileRunMax || __builtin_popcountll((s->run_bits | gb) & ~0xfull) > 8) {
+        const int rc = flush_held(s);
+        if (rc) return rc;
+    }
+    RunGate &g = s->run[s->run_len++];
+    std::memset(&g, 0, sizeof(g));
+    g.ctrl_mask = ctrl_mask;
+    g.tpos = (uint8_t)tpos;
+    for (int i = 0; i < 4; ++i) {
+        g.U[i].re = U[i].real();
+        g.U[i].im = U[i].imag();
+    }
+    s->run_bits |= gb;
+    return QIPGPU_OK;
+}
+
+// Hold a diagonal gate (k <= 2 targets) back on the state; held single-qubit gates and a full run
+// are applied first.
 static int hold_diagonal(qipgpu_state *s, const std::vector<int> &tbits, uint64_t ctrl_mask, const std::vector<cd> &d) {
-    if (s->chain_len == kDiagChainMax) {
-        const int rc = flush_diag_chain(s);
+    if (s->run_len || s->chain_len == kDiagChainMax) {
+        const int rc = flush_held(s);
         if (rc) return rc;
     }
     ChainGate &c = s->chain[s->chain_len++];
@@ -1073,7 +1251,7 @@ static int dispatch(qipgpu_state *s, const Analysed &an, const std::vector<int32
         }
         if (identity) return QIPGPU_OK;
         if (k <= 2) return hold_diagonal(s, tbits, ctrl_mask, d);
-        const int rc = flush_diag_chain(s);
+        const int rc = flush_held(s);
         if (rc) return rc;
         switch (k) {
             case 0: return launch_diag<R, 0>(s, tbits, ctrl_mask, cpos, d);
@@ -1087,7 +1265,9 @@ static int dispatch(qipgpu_state *s, const Analysed &an, const std::vector<int32
         }
         return diag_large(s, tbits, ctrl_mask, cpos, d.data());
     }
-    const int rc = flush_diag_chain(s);
+    if (k == 1 && s->n >= 12 && __builtin_popcountll((ctrl_mask | (1ull << tbits[0])) & ~0xfull) <= 8)
+        return hold_single(s, tbits[0], ctrl_mask, an.U);
+    const int rc = flush_held(s);
     if (rc) return rc;
     switch (k) {
         case 1: return launch_dense_reg<R, 1>(s, tbits, ctrl_mask, allpos, an.U);
@@ -1108,9 +1288,14 @@ static int check_disjoint(const qipgpu_state *s, int32_t k, const int32_t *t, in
 
 namespace qipgpu {
 
-int flush_diag_chain(qipgpu_state *s) {
-    if (!s->chain_len) return QIPGPU_OK;
-    return s->dtype == QIPGPU_C128 ? flush_chain_typed<double>(s) : flush_chain_typed<float>(s);
+int flush_held(qipgpu_state *s) {
+    const bool c128 = s->dtype == QIPGPU_C128;
+    if (s->chain_len) {
+        const int rc = c128 ? flush_chain_typed<double>(s) : flush_chain_typed<float>(s);
+        if (rc) return rc;
+    }
+    if (s->run_len) return c128 ? flush_run_typed<double>(s) : flush_run_typed<float>(s);
+    return QIPGPU_OK;
 }
 
 }  // namespace qipgpu

@@ -384,7 +384,7 @@ void simulate_recorder(int32_t n, int32_t width, bool streaming, bool use_launch
 }  // namespace
 
 int flush_recorded(qipgpu_state *s) {
-    int rc = flush_diag_chain(s);   // held-back diagonal gates came first
+    int rc = flush_held(s);   // held-back diagonal gates came first
     if (rc) return rc;
     Recorder *r = s->recorder;
     if (!r || r->in_flush || r->k.empty()) return QIPGPU_OK;
@@ -393,7 +393,7 @@ int flush_recorded(qipgpu_state *s) {
     r->since_plan = 0;
     rc = launch_planned(s, blocks, (int64_t)blocks.size());
     if (rc) return rc;
-    return flush_diag_chain(s);     // a complex128 block that is diagonal is held like any such gate
+    return flush_held(s);     // a complex128 block that is diagonal is held like any such gate
 }
 
 void recorder_discard(qipgpu_state *s) {
@@ -429,7 +429,7 @@ int qipgpu_fused_set_recording(qipgpu_state *s, int32_t width, int32_t streaming
 int qipgpu_fused_record(qipgpu_state *s, int32_t n_gates, const int32_t *gate_k, const int32_t *gate_qubits,
                         const double *mats) {
     QIP_NOT_CAPTURING(s, "gate recording for fusion");
-    int rc = flush_diag_chain(s);   // diagonal gates held back come before what is recorded now
+    int rc = flush_held(s);   // diagonal gates held back come before what is recorded now
     if (rc) return rc;
     rc = ensure_recorder(s);
     if (rc) return rc;

@@ -190,3 +190,79 @@ def test_held_diagonals_with_fused_recording():
         inf = 1.0 - abs(numpy.vdot(ref, got)) ** 2 / (numpy.vdot(got, got).real * numpy.vdot(ref, ref).real)
         assert inf < tol * 20, (cls.__name__, inf)
         be.close()
+
+
+def rand_u2():
+    z = (RNG.standard_normal((2, 2)) + 1j * RNG.standard_normal((2, 2))) / numpy.sqrt(2)
+    q, r = numpy.linalg.qr(z)
+    return q * (numpy.diagonal(r) / numpy.abs(numpy.diagonal(r)))
+
+
+def single_run_sequence(n, length):
+    """Single-qubit gates (Haar, H, X), some under one or two controls, with a diagonal now and
+    then: long runs of held single-qubit gates, tile runs split where their bits do not fit."""
+    out = []
+    for _ in range(length):
+        r = RNG.random()
+        qs = [int(q) for q in RNG.permutation(n)[:3]]
+        if r < 0.55:
+            out.append({qs[0]: rand_u2()})
+        elif r < 0.7:
+            out.append({qs[0]: HM if RNG.random() < 0.5 else XM})
+        elif r < 0.85:
+            out.append({(qs[1], qs[0]): CMat(rand_u2())})
+        elif r < 0.93:
+            out.append({(qs[2], qs[1], qs[0]): CMat(CMat(XM))})
+        else:
+            out.append(rand_diag_desc(n, 2))
+    return out
+
+
+@pytest.mark.parametrize("dtype", [numpy.complex128, numpy.complex64])
+@pytest.mark.parametrize("n", [12, 15, 21])
+def test_held_single_qubit_runs_equal_separate_launches(n, dtype):
+    seq = single_run_sequence(n, 150)
+    seq += [{q: rand_u2()} for q in range(n)]            # a whole layer, in qubit order
+    seq += [{q: rand_u2()} for q in range(n - 4, n)] * 9  # 36 gates on the low bits: full runs
+    print("held single-qubit runs n", n, numpy.dtype(dtype).name, len(seq), "gates", flush=True)
+    psi, held, alone = run_both(n, dtype, seq)
+    got = held.get_state()
+    assert numpy.array_equal(got, alone.get_state())
+    if dtype == numpy.complex128 and n <= 15:
+        ora = OracleBackend.make_state(n, [list(range(n))], [psi])
+        for d in seq:
+            ora.kronselect_dot(d)
+        assert numpy.max(numpy.abs(got - ora.get_state())) < 1e-12
+    held.close()
+    alone.close()
+
+
+def test_layered_circuit_bits_and_graph():
+    """A 20-qubit layered random circuit of the benchmarks (Haar layers, CNOT and CZ layers):
+    free-running equals a synchronisation after every gate, bit for bit; and a recorded gate graph
+    of single-qubit runs equals the same gates launched one by one."""
+    from qip_amd.workloads import layered_random_circuit
+    n = 20
+    ops = layered_random_circuit(n, 6, 11)
+    psi, held, alone = run_both(n, numpy.complex128, ops)
+    assert numpy.array_equal(held.get_state(), alone.get_state())
+    held.close()
+    alone.close()
+    n = 14
+    psi = rand_state(n)
+    seq = single_run_sequence(n, 60)
+    be = GpuBackend.make_state(n, [list(range(n))], [psi])
+    ref = GpuBackend.make_state(n, [list(range(n))], [psi])
+    be.capture_begin()
+    for d in seq:
+        be.kronselect_dot(d)
+    graph = be.capture_end()
+    assert graph is not None
+    for launch in range(3):
+        be.launch_graph(graph)
+        for d in seq:
+            ref.kronselect_dot(d)
+            ref.synchronize()
+        assert numpy.array_equal(be.get_state(), ref.get_state()), launch
+    be.close()
+    ref.close()
