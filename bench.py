@@ -71,6 +71,15 @@ MARG_PMC_28Q = {"fetch_kib": 2097182.76, "fetch_scale": 1048576.0 / 524313.75,
                 "write_kib": 3026.64, "write_scale": 1.0}
 
 
+# HBM traffic of one k_tile_run pass (a held run of single-qubit gates, 512 threads per workgroup) on a
+# 28-qubit complex128 state (4 GiB), measured by two rocprofv3 --pmc runs of `bench.py --workload
+# circuit28 --no-cpu-baseline --copy-calibration-gib 1` (mean over the run's 102 complex128 passes;
+# FETCH_SIZE reads 524316.5 KiB for the 1 GiB copy, WRITE_SIZE 1048576 KiB). Commands and figures:
+# profiles/k_dense_circuit28_depth20.txt.
+TILE_RUN_PMC_28Q = {"fetch_kib": 2097190.4, "fetch_scale": 1048576.0 / 524316.5,
+                    "write_kib": 4194312.4, "write_scale": 1.0}
+
+
 def pmc_bytes(p) -> int:
     return int((p["fetch_kib"] * p["fetch_scale"] + p["write_kib"] * p["write_scale"]) * 1024)
 
@@ -290,7 +299,8 @@ def bench_circuit28(args):
                 max_abs_err_c64_vs_c128=res["max_abs_err_c64_vs_c128"],
                 roofline={"bound": "hbm", "kernel": d["kernel"], "achieved": d["achieved_gbs"], "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": d["achieved_gbs"] / HBM_PEAK_GBS,
-                          "traffic": None, "bytes_per_launch": d["bytes_per_launch"], "avg_launch_ms": d["avg_ms"]},
+                          "traffic": pmc_bytes(TILE_RUN_PMC_28Q) if n == 28 else None,
+                          "bytes_per_launch": d["bytes_per_launch"], "avg_launch_ms": d["avg_ms"]},
                 roofline_c64=dict(dom["c64"], frac=dom["c64"]["achieved_gbs"] / HBM_PEAK_GBS))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ops, n, args.cpu_seconds)

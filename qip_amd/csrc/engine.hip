@@ -1065,12 +1065,16 @@ __device__ __forceinline__ uint64_t run_tile_base(uint64_t t, const TileRunArgs<
     return t;
 }
 
-// Thread t holds the tile's amplitudes u = t + 256 i, i = 0..15: u bits 0..7 are t, bits 8..11 are i.
+// 512 threads per workgroup (two waves per SIMD from each workgroup, so one wave's LDS and FP64
+// latency overlaps another's). Thread t holds the tile's amplitudes u = t + 512 i, i = 0..7: u bits
+// 0..8 are t, bits 9..11 are i.
+constexpr uint32_t kRunThreads = 512;
+
 template <typename R>
 __device__ __forceinline__ uint64_t run_off_hi(int i, const TileRunArgs<R> &a) {
     uint64_t o = 0;
 #pragma unroll
-    for (int b = 0; b < 4; ++b) o |= (uint64_t)((i >> b) & 1) << a.pbits[8 + b];
+    for (int b = 0; b < 3; ++b) o |= (uint64_t)((i >> b) & 1) << a.pbits[9 + b];
     return o;
 }
 
@@ -1079,22 +1083,22 @@ __device__ __forceinline__ void run_tile(uint64_t base, cx<R> *t, const TileRunA
     const uint32_t tid = threadIdx.x;
     __syncthreads();  // the previous tile has left LDS
 #pragma unroll
-    for (int i = 0; i < 16; ++i) t[tid + 256u * i] = a.amps[base + run_off_hi(i, a)];
+    for (int i = 0; i < 8; ++i) t[tid + kRunThreads * i] = a.amps[base + run_off_hi(i, a)];
     __syncthreads();
     for (int gi = 0; gi < a.ngates; ++gi) {
         const uint32_t tb = a.g[gi].tb, cm = a.g[gi].cmask;
         const uint32_t lo = (1u << tb) - 1u;
-        cx<R> x0[8], x1[8];
-        uint32_t u0[8];
+        cx<R> x0[4], x1[4];
+        uint32_t u0[4];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const uint32_t p = tid + 256u * j;           // pair index 0..2047
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t p = tid + kRunThreads * j;    // pair index 0..2047
             u0[j] = ((p & ~lo) << 1) | (p & lo);
             x0[j] = t[u0[j]];
             x1[j] = t[u0[j] | (1u << tb)];
         }
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
+        for (int j = 0; j < 4; ++j) {
             if ((u0[j] & cm) != cm) continue;
             cx<double> acc0, acc1;
             acc0.re = acc0.im = acc1.re = acc1.im = 0.0;
@@ -1108,18 +1112,18 @@ __device__ __forceinline__ void run_tile(uint64_t base, cx<R> *t, const TileRunA
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 16; ++i) a.amps[base + run_off_hi(i, a)] = t[tid + 256u * i];
+    for (int i = 0; i < 8; ++i) a.amps[base + run_off_hi(i, a)] = t[tid + kRunThreads * i];
 }
 
 // Contiguous tile ranges per workgroup; the other workgroups on the CU (two of them for complex128,
-// four for complex64, by LDS) keep HBM busy while one works on its tile.
+// up to four for complex64, by LDS) keep HBM busy while one works on its tile.
 template <typename R>
-__global__ __launch_bounds__(256, 2) void k_tile_run(const TileRunArgs<R> a) {
+__global__ __launch_bounds__(kRunThreads, 2) void k_tile_run(const TileRunArgs<R> a) {
     __shared__ cx<R> t[4096];
     const uint32_t tid = threadIdx.x;
     uint64_t off_lo = 0;
 #pragma unroll
-    for (int b = 0; b < 8; ++b) off_lo |= (uint64_t)((tid >> b) & 1u) << a.pbits[b];
+    for (int b = 0; b < 9; ++b) off_lo |= (uint64_t)((tid >> b) & 1u) << a.pbits[b];
     const uint64_t per = (a.ntiles + gridDim.x - 1) / gridDim.x;
     const uint64_t first = (uint64_t)blockIdx.x * per;
     const uint64_t end = min(first + per, a.ntiles);
@@ -1184,7 +1188,7 @@ static int flush_run_typed(qipgpu_state *s) {
     }
     const uint64_t per_cu = sizeof(R) == 8 ? 2 : 4;   // 64 KB / 32 KB of LDS per workgroup
     const uint32_t grid = (uint32_t)std::min<uint64_t>(a.ntiles, (uint64_t)device_cus() * per_cu);
-    return launch_gate(s, &k_tile_run<R>, dim3(grid), dim3(256), a);
+    return launch_gate(s, &k_tile_run<R>, dim3(grid), dim3(kRunThreads), a);
 }
 
 // Hold a single-qubit gate (target bit tpos, under ctrl_mask) back on the state: held diagonal gates
