@@ -75,7 +75,7 @@ MARG_PMC_28Q = {"fetch_kib": 2097182.76, "fetch_scale": 1048576.0 / 524313.75,
 # 28-qubit complex128 state (4 GiB), measured by two rocprofv3 --pmc runs of `bench.py --workload
 # circuit28 --no-cpu-baseline --copy-calibration-gib 1` (mean over the run's 102 complex128 passes;
 # FETCH_SIZE reads 524316.5 KiB for the 1 GiB copy, WRITE_SIZE 1048576 KiB). Commands and figures:
-# profiles/k_dense_circuit28_depth20.txt.
+# profiles/k_tile_run_circuit28_depth20.txt.
 TILE_RUN_PMC_28Q = {"fetch_kib": 2097190.4, "fetch_scale": 1048576.0 / 524316.5,
                     "write_kib": 4194312.4, "write_scale": 1.0}
 
@@ -835,7 +835,8 @@ def bench_product24(args):
 def bench_batch512(args):
     """512 independent layered random circuits, n ~ U{10..22}, depth ~ U{8..32} (seeded), complex64:
     the batching form (one captured graph per run, autotuned) against the plain complex64 path with
-    one engine call (one kernel launch) per gate, and per-circuit 1 - F against plain complex128.
+    one engine call per gate (from 12 qubits on, held runs of small gates share passes), and per-circuit
+    1 - F against plain complex128.
     A step is: reset all 512 states to |0...0>, run every circuit, synchronise."""
     import numpy
     from qip_amd import GpuBackend
@@ -877,7 +878,8 @@ def bench_batch512(args):
     eff_gbs = info["algorithmic_bytes"] / (run_ms * 1e-3) / 1e9
     log("batch: {} steps in {:.3f} s, run {:.2f} ms".format(args.steps, elapsed, run_ms))
 
-    # plain paths, one launch per gate: complex64 (timed) and complex128 (the fidelity reference)
+    # the plain path, one engine call per gate (every gate applied by itself; runs of held small gates
+    # share passes): complex64 (timed) and complex128 (the fidelity reference)
     gates = [packed.gates_of(i) for i in range(count)]
     plain = {}
     fids, errs = [], []
@@ -907,7 +909,7 @@ def bench_batch512(args):
                        "hipGraph per run)", "circuits": count, "seed": args.seed, "gates": info["gates"],
            "qubits_total": int(sum(ns)), "lds_states": info["lds_states"], "tiled_states": info["tiled_states"],
            "blocks": info["blocks"], "waves": info["waves"], "tuning": tuning}
-    out = _line("512 uneven circuits/s, batched c64 vs one launch per gate", circuits_s, "circuits/s", args.steps,
+    out = _line("512 uneven circuits/s, batched c64 vs the plain path", circuits_s, "circuits/s", args.steps,
                 args.warmup, elapsed * 1e3 / args.steps, "c64", cfg,
                 batch={"circuits_per_s": circuits_s, "gpu_ms_per_run": run_ms, "algorithmic_bytes_per_run":
                        info["algorithmic_bytes"], "effective_gbs": eff_gbs, "hbm_frac": eff_gbs / HBM_PEAK_GBS,
@@ -919,8 +921,8 @@ def bench_batch512(args):
                           "note": "the whole captured graph as one unit: algorithmic bytes of every state's passes "
                                   "over the run's GPU time (per-kernel figures: profiles/k_batch6_batch512.txt)"})
     if plain:
-        out.update(plain_per_gate_c64={"circuits_per_s": count / plain["c64"], "s_per_batch": plain["c64"]},
-                   plain_per_gate_c128={"circuits_per_s": count / plain["c128"], "s_per_batch": plain["c128"]},
+        out.update(plain_c64={"circuits_per_s": count / plain["c64"], "s_per_batch": plain["c64"]},
+                   plain_c128={"circuits_per_s": count / plain["c128"], "s_per_batch": plain["c128"]},
                    speedup_vs_plain_c64=plain["c64"] / (elapsed / args.steps),
                    infidelity_vs_c128={"max": float(max(fids)), "median": float(numpy.median(fids))},
                    max_abs_err_vs_c128=float(max(errs)))
