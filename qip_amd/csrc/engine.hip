@@ -1078,6 +1078,20 @@ __device__ __forceinline__ uint64_t run_off_hi(int i, const TileRunArgs<R> &a) {
     return o;
 }
 
+// One gate on one pair (x0: target bit 0, x1: target bit 1), as k_dense<R,1> computes it: each output
+// an FP64 sum from zero of U's row times the pair (cfma), rounded to the state's precision.
+template <typename R>
+__device__ __forceinline__ void run_gate(const RunSlot &g, cx<R> &x0, cx<R> &x1) {
+    cx<double> acc0, acc1;
+    acc0.re = acc0.im = acc1.re = acc1.im = 0.0;
+    cfma(acc0, g.U[0], widen(x0));
+    cfma(acc0, g.U[1], widen(x1));
+    cfma(acc1, g.U[2], widen(x0));
+    cfma(acc1, g.U[3], widen(x1));
+    x0 = narrow<R>(acc0);
+    x1 = narrow<R>(acc1);
+}
+
 template <typename R>
 __device__ __forceinline__ void run_tile(uint64_t base, cx<R> *t, const TileRunArgs<R> &a) {
     const uint32_t tid = threadIdx.x;
@@ -1085,29 +1099,56 @@ __device__ __forceinline__ void run_tile(uint64_t base, cx<R> *t, const TileRunA
 #pragma unroll
     for (int i = 0; i < 8; ++i) t[tid + kRunThreads * i] = a.amps[base + run_off_hi(i, a)];
     __syncthreads();
-    for (int gi = 0; gi < a.ngates; ++gi) {
-        const uint32_t tb = a.g[gi].tb, cm = a.g[gi].cmask;
-        const uint32_t lo = (1u << tb) - 1u;
-        cx<R> x0[4], x1[4];
-        uint32_t u0[4];
+    int gi = 0;
+    // two gates per LDS round trip: each thread owns whole quads {u, u^b1, u^b2, u^b1^b2} (or pairs,
+    // when both gates act on one bit) and applies the first gate to them, then the second
+    for (; gi + 1 < a.ngates; gi += 2) {
+        const RunSlot &g1 = a.g[gi], &g2 = a.g[gi + 1];
+        const uint32_t b1 = 1u << g1.tb, b2 = 1u << g2.tb;
+        if (b1 == b2) {
+            const uint32_t lo = b1 - 1u;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const uint32_t p = tid + kRunThreads * j;    // pair index 0..2047
-            u0[j] = ((p & ~lo) << 1) | (p & lo);
-            x0[j] = t[u0[j]];
-            x1[j] = t[u0[j] | (1u << tb)];
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t p = tid + kRunThreads * j;    // pair index 0..2047
+                const uint32_t u0 = ((p & ~lo) << 1) | (p & lo);
+                cx<R> x0 = t[u0], x1 = t[u0 | b1];
+                if ((u0 & g1.cmask) == g1.cmask) run_gate(g1, x0, x1);
+                if ((u0 & g2.cmask) == g2.cmask) run_gate(g2, x0, x1);
+                t[u0] = x0;
+                t[u0 | b1] = x1;
+            }
+        } else {
+            const uint32_t bl = b1 < b2 ? b1 : b2, bh = b1 < b2 ? b2 : b1;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                uint32_t q = tid + kRunThreads * j;          // quad index 0..1023
+                q = ((q & ~(bl - 1u)) << 1) | (q & (bl - 1u));
+                const uint32_t u = ((q & ~(bh - 1u)) << 1) | (q & (bh - 1u));
+                cx<R> a00 = t[u], a10 = t[u | b1], a01 = t[u | b2], a11 = t[u | b1 | b2];
+                if ((u & g1.cmask) == g1.cmask) run_gate(g1, a00, a10);
+                if (((u | b2) & g1.cmask) == g1.cmask) run_gate(g1, a01, a11);
+                if ((u & g2.cmask) == g2.cmask) run_gate(g2, a00, a01);
+                if (((u | b1) & g2.cmask) == g2.cmask) run_gate(g2, a10, a11);
+                t[u] = a00;
+                t[u | b1] = a10;
+                t[u | b2] = a01;
+                t[u | b1 | b2] = a11;
+            }
         }
+        __syncthreads();
+    }
+    if (gi < a.ngates) {
+        const RunSlot &g = a.g[gi];
+        const uint32_t b = 1u << g.tb, lo = b - 1u;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            if ((u0[j] & cm) != cm) continue;
-            cx<double> acc0, acc1;
-            acc0.re = acc0.im = acc1.re = acc1.im = 0.0;
-            cfma(acc0, a.g[gi].U[0], widen(x0[j]));
-            cfma(acc0, a.g[gi].U[1], widen(x1[j]));
-            cfma(acc1, a.g[gi].U[2], widen(x0[j]));
-            cfma(acc1, a.g[gi].U[3], widen(x1[j]));
-            t[u0[j]] = narrow<R>(acc0);
-            t[u0[j] | (1u << tb)] = narrow<R>(acc1);
+            const uint32_t p = tid + kRunThreads * j;
+            const uint32_t u0 = ((p & ~lo) << 1) | (p & lo);
+            if ((u0 & g.cmask) != g.cmask) continue;
+            cx<R> x0 = t[u0], x1 = t[u0 | b];
+            run_gate(g, x0, x1);
+            t[u0] = x0;
+            t[u0 | b] = x1;
         }
         __syncthreads();
     }
