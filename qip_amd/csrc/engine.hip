@@ -1092,12 +1092,34 @@ __device__ __forceinline__ void run_gate(const RunSlot &g, cx<R> &x0, cx<R> &x1)
     x1 = narrow<R>(acc1);
 }
 
+// A thread's 8 amplitudes of a tile in registers, as separate real and imaginary parts (plain scalar
+// arrays stay in registers where arrays of the complex struct would go to scratch memory).
 template <typename R>
-__device__ __forceinline__ void run_tile(uint64_t base, cx<R> *t, const TileRunArgs<R> &a) {
+struct RunRegs {
+    R re[8], im[8];
+};
+
+template <typename R>
+__device__ __forceinline__ void run_load(RunRegs<R> &v, uint64_t base, const TileRunArgs<R> &a) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const cx<R> x = a.amps[base + run_off_hi(i, a)];
+        v.re[i] = x.re;
+        v.im[i] = x.im;
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void run_tile(const RunRegs<R> &v, uint64_t base, cx<R> *t, const TileRunArgs<R> &a) {
     const uint32_t tid = threadIdx.x;
     __syncthreads();  // the previous tile has left LDS
 #pragma unroll
-    for (int i = 0; i < 8; ++i) t[tid + kRunThreads * i] = a.amps[base + run_off_hi(i, a)];
+    for (int i = 0; i < 8; ++i) {
+        cx<R> x;
+        x.re = v.re[i];
+        x.im = v.im[i];
+        t[tid + kRunThreads * i] = x;
+    }
     __syncthreads();
     int gi = 0;
     // two gates per LDS round trip: each thread owns whole quads {u, u^b1, u^b2, u^b1^b2} (or pairs,
@@ -1156,8 +1178,7 @@ __device__ __forceinline__ void run_tile(uint64_t base, cx<R> *t, const TileRunA
     for (int i = 0; i < 8; ++i) a.amps[base + run_off_hi(i, a)] = t[tid + kRunThreads * i];
 }
 
-// Contiguous tile ranges per workgroup; the other workgroups on the CU (two of them for complex128,
-// up to four for complex64, by LDS) keep HBM busy while one works on its tile.
+// Contiguous tile ranges per workgroup, two (complex128) or four (complex64) workgroups per CU by LDS.
 template <typename R>
 __global__ __launch_bounds__(kRunThreads, 2) void k_tile_run(const TileRunArgs<R> a) {
     __shared__ cx<R> t[4096];
@@ -1166,10 +1187,22 @@ __global__ __launch_bounds__(kRunThreads, 2) void k_tile_run(const TileRunArgs<R
 #pragma unroll
     for (int b = 0; b < 9; ++b) off_lo |= (uint64_t)((tid >> b) & 1u) << a.pbits[b];
     const uint64_t per = (a.ntiles + gridDim.x - 1) / gridDim.x;
-    const uint64_t first = (uint64_t)blockIdx.x * per;
-    const uint64_t end = min(first + per, a.ntiles);
-    for (uint64_t tile = first; tile < end; ++tile)   // the same trip count in every thread
-        run_tile(run_tile_base(tile, a) + off_lo, t, a);
+    uint64_t tile = (uint64_t)blockIdx.x * per;
+    const uint64_t end = min(tile + per, a.ntiles);
+    if (tile >= end) return;   // the same for every thread of the workgroup
+    const uint64_t last = end - 1;
+    // two register buffers alternate: the next tile's loads are in flight while one is processed
+    // (the last prefetch re-reads an in-range tile)
+    RunRegs<R> va, vb;
+    run_load(va, run_tile_base(tile, a) + off_lo, a);
+    for (;;) {
+        run_load(vb, run_tile_base(min(tile + 1, last), a) + off_lo, a);
+        run_tile(va, run_tile_base(tile, a) + off_lo, t, a);
+        if (++tile >= end) break;
+        run_load(va, run_tile_base(min(tile + 1, last), a) + off_lo, a);
+        run_tile(vb, run_tile_base(tile, a) + off_lo, t, a);
+        if (++tile >= end) break;
+    }
 }
 
 template <typename R>
