@@ -71,13 +71,14 @@ MARG_PMC_28Q = {"fetch_kib": 2097182.76, "fetch_scale": 1048576.0 / 524313.75,
                 "write_kib": 3026.64, "write_scale": 1.0}
 
 
-# HBM traffic of one k_tile_run pass (a held run of single-qubit gates, 512 threads per workgroup) on a
-# 28-qubit complex128 state (4 GiB), measured by two rocprofv3 --pmc runs of `bench.py --workload
-# circuit28 --no-cpu-baseline --copy-calibration-gib 1` (mean over the run's 102 complex128 passes;
-# FETCH_SIZE reads 524316.5 KiB for the 1 GiB copy, WRITE_SIZE 1048576 KiB). Commands and figures:
+# HBM traffic of one k_tile_run pass (a held run of single-qubit gates) on a 28-qubit complex128 state
+# (4 GiB), measured by two rocprofv3 --pmc runs of `bench.py --workload circuit28 --no-cpu-baseline
+# --copy-calibration-gib 1` (mean over the run's 102 complex128 passes; FETCH_SIZE reads 524315.2 KiB
+# for the 1 GiB copy, WRITE_SIZE 1048576 KiB): 1.0031x the pass's algorithmic bytes, the excess being the
+# last prefetch of each workgroup, which re-reads a tile. Commands and figures:
 # profiles/k_tile_run_circuit28_depth20.txt.
-TILE_RUN_PMC_28Q = {"fetch_kib": 2097190.4, "fetch_scale": 1048576.0 / 524316.5,
-                    "write_kib": 4194312.4, "write_scale": 1.0}
+TILE_RUN_PMC_28Q = {"fetch_kib": 2110371.9, "fetch_scale": 1048576.0 / 524315.2,
+                    "write_kib": 4194310.4, "write_scale": 1.0}
 
 
 def pmc_bytes(p) -> int:
